@@ -1,0 +1,225 @@
+/*
+ * poe2d.h -- C ABI of the MI355X (gfx950) 2-D poroelastic (Biot) C-PML solver.
+ *
+ * The reference (thejatmik/poe2d) has no callable interface: its time loop, device
+ * arrays and scalars all live in one main() (seismic_CPML_2D_poroelastic_withCUDA.cu:470-1922,
+ * seismic_CPML_2d_poroelastic.cpp:43-929).  Each entry point below names the piece of that
+ * main() it replaces, so a maintainer can swap the reference loop for these calls.
+ *
+ * Conventions
+ *   - Every function returns a poe2d_status and never exits the process; the message of the
+ *     last failure on the calling thread is poe2d_last_error().
+ *   - The caller owns every host pointer.  Inputs are copied during the call; outputs go into
+ *     caller-allocated buffers whose size the call checks.
+ *   - One solver handle = one GPU device + one HIP stream.  A handle is not re-entrant; distinct
+ *     handles may be used from distinct threads.
+ *   - Grids follow the reference's indexing: NX, NZ are the largest point indices, so a grid has
+ *     (NX+1) x (NZ+1) points, stored row-major with x fastest; z (the reference's "z", the
+ *     configs' "y") increases with depth.
+ *   - FP32 and FP64 are chosen per handle (poe2d_grid.precision), not per library.
+ */
+#ifndef POE2D_H
+#define POE2D_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define POE2D_ABI_VERSION 1
+
+typedef enum {
+    POE2D_OK = 0,
+    POE2D_E_INVALID = 1,  /* bad argument or out-of-range geometry                        */
+    POE2D_E_CFL = 2,      /* Courant number above 6/7/sqrt(2): the reference refuses to run  */
+    POE2D_E_NOMEM = 3,    /* device or host allocation failed                                */
+    POE2D_E_HIP = 4,      /* HIP runtime error (no device, launch failure, ...)              */
+    POE2D_E_IO = 5,       /* file could not be read or written                               */
+    POE2D_E_UNSTABLE = 6, /* non-finite or runaway wavefield detected                        */
+    POE2D_E_STATE = 7     /* call out of order (e.g. run before set_source)                  */
+} poe2d_status;
+
+enum { POE2D_F32 = 32, POE2D_F64 = 64 };
+
+/* Grid, time step and C-PML parameters.
+ * Replaces the hard-coded constants of seismic_CPML_2D_poroelastic_withCUDA.cu:486-510, 840-845
+ * (NX, NZ, DELTAX, DELTAZ, DELTAT, NPOINTS_PML, NPOWER, K_MAX_PML, ALPHA_MAX_PML, Rcoef at :992). */
+typedef struct {
+    int nx, nz;        /* NX, NZ: point indices run 0..nx (x) and 0..nz (z)                       */
+    double dx, dz, dt; /* grid spacing (m) and time step (s)                                      */
+    int npml;          /* C-PML thickness in grid points (NPOINTS_PML)                             */
+    double npower;     /* profile power (NPOWER), reference 2                                      */
+    double rcoef;      /* target reflection coefficient (Rcoef), reference 1e-3                    */
+    double kmax;       /* K_MAX_PML, reference 1                                                   */
+    double alpha_max;  /* ALPHA_MAX_PML; <= 0 selects the reference's pi*f0                        */
+    double f0;         /* dominant frequency (Hz): sets alpha_max and the default wavelet          */
+    double cp_max;     /* P velocity for d0 and the Courant check; <= 0: derived from the
+                          coefficients as max sqrt((lambdac + 2 mu + alpha^2 M) / rho)             */
+    int precision;     /* POE2D_F32 or POE2D_F64                                                  */
+    int check_cfl;     /* nonzero: refuse Courant > 6/7/sqrt(2) like the reference (:1211)         */
+} poe2d_grid;
+
+/* The eight per-point coefficient arrays the time loop reads (the reference's rho, rhof, rsm,
+ * rmu, rlambdac, rbM, alpha, etaokappa uploaded at seismic_CPML_2D_poroelastic_withCUDA.cu:1433-1461).
+ * Each array has (nz+1) rows of (nx+1) values, x fastest, rows `pitch` elements apart. */
+typedef struct {
+    int dtype;         /* POE2D_F32 or POE2D_F64: element type of the arrays                       */
+    const void *rho;   /* bulk density rho = (1-phi) rho_s + phi rho_f                             */
+    const void *rhof;  /* fluid density rho_f                                                      */
+    const void *m;     /* m = a rho_f / phi (reference rsm)                                        */
+    const void *mu;    /* shear modulus (reference rmu)                                            */
+    const void *lambdac; /* lambda_c = lambda_o + alpha^2 M (reference rlambdac)                  */
+    const void *M;     /* Biot modulus (reference rbM)                                             */
+    const void *alpha; /* Biot-Willis coefficient                                                  */
+    const void *b;     /* eta / kappa (reference etaokappa)                                        */
+    size_t pitch;      /* elements between rows; 0 means nx+1                                      */
+} poe2d_coeffs;
+
+/* Source kinds.
+ *   PRESSURE_DGAUSS: the reference's source, amp * 2a(t-t0) exp(-a (t-t0)^2), a = pi^2 f0^2,
+ *     added to the pore pressure p at (ix, iz) after the stress update of each step, only while
+ *     t < t_cut (seismic_CPML_2D_poroelastic_withCUDA.cu:90-111; amp = faktor = 1e10 at :45).
+ *   FORCE_Y_RICKER: vertical body force density amp * (1 - 2a(t-t0)^2) exp(-a (t-t0)^2) added
+ *     to d(sigma_xy)/dx + d(sigma_yy)/dz in the v_y / w_y update at (ix, iz); no counterpart in
+ *     the reference (its v_y source is commented out at :357-383). */
+enum { POE2D_SRC_PRESSURE_DGAUSS = 0, POE2D_SRC_FORCE_Y_RICKER = 1 };
+
+typedef struct {
+    int kind;      /* POE2D_SRC_*                                                                 */
+    int ix, iz;    /* source point (ISOURCE, JSOURCE)                                             */
+    double amp;    /* amplitude                                                                   */
+    double t0;     /* wavelet delay (s); <= 0 selects the reference's 1/f0                        */
+    double t_cut;  /* the wavelet is zero for t >= t_cut; <= 0 means no cut (reference 0.4 s)     */
+} poe2d_source;
+
+/* Receiver components, recorded at each receiver's own staggered position (no interpolation). */
+enum { POE2D_VX = 1, POE2D_VY = 2, POE2D_P = 4 };
+
+/* Receivers.  Slot k of a trace holds the wavefield at the start of step k*decimate: v after
+ * that many steps and p of that step's stress update (the reference's save1_kernel slot
+ * convention, seismic_CPML_2D_poroelastic_withCUDA.cu:114-201 with i = k+1 at :1627). */
+typedef struct {
+    int n;                 /* number of receivers                                                 */
+    const int *ix, *iz;    /* receiver points                                                     */
+    unsigned comp_mask;    /* POE2D_VX | POE2D_VY | POE2D_P                                       */
+    int decimate;          /* record every decimate-th step (reference samprek)                   */
+    int nsamp;             /* samples per trace to keep (capacity)                                */
+} poe2d_receivers;
+
+/* Fields for poe2d_snapshot.  P, SXX and SYY are recomputed from the state exactly as the
+ * stress update computes them. */
+enum {
+    POE2D_FIELD_VX = 0, POE2D_FIELD_VY = 1, POE2D_FIELD_WX = 2, POE2D_FIELD_WY = 3,
+    POE2D_FIELD_EXX = 4, POE2D_FIELD_EYY = 5, POE2D_FIELD_ZETA = 6, POE2D_FIELD_SXY = 7,
+    POE2D_FIELD_P = 8, POE2D_FIELD_SXX = 9, POE2D_FIELD_SYY = 10
+};
+
+typedef struct poe2d_solver poe2d_solver;
+
+/* Library version / device probe.  Returns POE2D_OK when a gfx950 device is usable. */
+int poe2d_abi_version(void);
+poe2d_status poe2d_device_info(int *device_count, char *arch, size_t arch_len);
+
+/* Create a solver: build the C-PML profiles, check stability, upload the coefficients and
+ * allocate the zeroed state.  Replaces seismic_CPML_2D_poroelastic_withCUDA.cu:985-1212 (C-PML
+ * profiles, Courant check) and :1251-1592 (cudaMalloc / H2D of every array). */
+poe2d_status poe2d_create(const poe2d_grid *grid, const poe2d_coeffs *coeffs, poe2d_solver **out);
+
+/* Set the source and compute its wavelet table on the host in FP64 for steps 0..nsteps_max-1.
+ * Replaces ISOURCE/JSOURCE/f0 (:757-758, :830-836) and the device-side source term (:90-111). */
+poe2d_status poe2d_set_source(poe2d_solver *s, const poe2d_source *src);
+
+/* Optional: replace the wavelet with caller samples w[0..n-1] (w[k] is used at step k; steps
+ * beyond n see 0).  amp and t_cut of the source still apply. */
+poe2d_status poe2d_set_wavelet(poe2d_solver *s, const double *w, int n);
+
+/* Set the receivers and allocate the device trace buffer.  Replaces panj/panj2/a0_pos/a1_pos/
+ * samprek (:496-502, :759-762) and the save1_kernel gather (:114-201). */
+poe2d_status poe2d_set_receivers(poe2d_solver *s, const poe2d_receivers *rec);
+
+/* Advance nsteps time steps on the handle's stream and wait for them.  Each step performs the
+ * reference's stress1 -> save1 -> stress2 -> stress3 -> vx -> vz sequence
+ * (seismic_CPML_2D_poroelastic_withCUDA.cu:1625-1636). */
+poe2d_status poe2d_run(poe2d_solver *s, int nsteps);
+
+/* Device time (ms, hipEvent on the handle's stream) of the last poe2d_run, and how many
+ * steps it covered. */
+poe2d_status poe2d_last_run_ms(poe2d_solver *s, double *ms, int *nsteps);
+
+/* Number of steps taken since create/reset. */
+poe2d_status poe2d_step_count(poe2d_solver *s, int *nsteps);
+
+/* Copy the recorded traces to the host as out[comp][rec][sample] (components in the order
+ * VX, VY, P restricted to comp_mask), in the handle's precision.  out_bytes must be at least
+ * ncomp*nrec*nsamp*sizeof(real).  Replaces the seismogram D2H copies (:1737-1836). */
+poe2d_status poe2d_seismograms(poe2d_solver *s, void *out, size_t out_bytes);
+
+/* Copy one field to the host as (nz+1) rows of (nx+1) values in the handle's precision.
+ * Replaces the per-chunk D2H of vz/vx (:1642-1645) behind the .vzo/.vxo snapshots. */
+poe2d_status poe2d_snapshot(poe2d_solver *s, int field, void *out, size_t out_bytes);
+
+/* Full state (wavefield, C-PML memory, step counter, recorded traces) for bitwise
+ * checkpoint/resume.  No counterpart in the reference. */
+poe2d_status poe2d_checkpoint_size(poe2d_solver *s, size_t *bytes);
+poe2d_status poe2d_checkpoint(poe2d_solver *s, void *blob, size_t bytes);
+poe2d_status poe2d_restore(poe2d_solver *s, const void *blob, size_t bytes);
+
+/* Zero the wavefield, C-PML memory, traces and step counter.  Replaces zero_kernel
+ * (:387-422, called at :1736). */
+poe2d_status poe2d_reset(poe2d_solver *s);
+
+void poe2d_destroy(poe2d_solver *s);
+
+/* Message of the last failure on this thread.  Replaces HANDLE_ERROR's print-and-exit
+ * (book.h, included at :37). */
+const char *poe2d_last_error(void);
+
+/* ---------------------------------------------------------------------------------------
+ * Host-side set-up helpers (no GPU).  They are the same routines poe2d_create uses.
+ * --------------------------------------------------------------------------------------- */
+
+/* C-PML 1-D profiles for one axis (seismic_CPML_2D_poroelastic_withCUDA.cu:985-1159).
+ * axis 0 = x (n = nx+1 points, origins left npml*dx+3dx and right (nx-2)dx-npml*dx), axis 1 = z
+ * (origins top npml*dz, whose alpha carries an extra 0.1*alpha_max, and bottom (nz-2)dz-npml*dz).
+ * precision POE2D_F32 evaluates in the reference's float arithmetic (float variables, float
+ * libm overloads, double literals) and widens the results to double exactly; POE2D_F64
+ * evaluates in double.  Each output has n values. */
+poe2d_status poe2d_cpml_profile(int axis, int n, double delta, double dt, int npml, double npower,
+                                double rcoef, double kmax, double alpha_max, double cp_max,
+                                int precision, double *out_a, double *out_b, double *out_K,
+                                double *out_a_half, double *out_b_half, double *out_K_half);
+
+/* Biot front end: per-point (rho_s, rho_f, phi, kappa, eta, tortuosity a, K_s, K_f, K_fr, mu_fr)
+ * to the eight coefficient arrays, with K_b := K_fr and rho = (1-phi) rho_s + phi rho_f, and
+ * the remaining algebra of seismic_CPML_2d_poroelastic.cpp:194-205.  n points, FP64 in/out. */
+poe2d_status poe2d_material_biot(size_t n, const double *rho_s, const double *rho_f,
+                                 const double *phi, const double *kappa, const double *eta,
+                                 const double *tort, const double *K_s, const double *K_f,
+                                 const double *K_fr, const double *mu_fr,
+                                 double *rho, double *rhof, double *m, double *mu,
+                                 double *lambdac, double *M, double *alpha, double *b);
+
+/* The reference's own front end (seismic_CPML_2d_poroelastic.cpp:162-211): per-point c_p, c_s,
+ * saturated density and medium index, plus per-medium tables of K_f, eta, kappa, phi, a and
+ * rho_f (nmed entries), to the eight float coefficient arrays, in the reference's float
+ * arithmetic.  Points whose medium index matches no table entry keep zero fluid properties
+ * (the reference leaves them uninitialised); the call reports them with POE2D_E_INVALID. */
+poe2d_status poe2d_material_reference(size_t n, const float *cp, const float *cs,
+                                      const float *rho_sat, const float *medium, int nmed,
+                                      const float *kf_tab, const float *eta_tab,
+                                      const float *kappa_tab, const float *phi_tab,
+                                      const float *a_tab, const float *rhof_tab,
+                                      float *rho, float *rhof, float *m, float *mu,
+                                      float *lambdac, float *M, float *alpha, float *b);
+
+/* Wavelet samples w[k] = f(k*dt) of a source kind (amp applied, t_cut applied), FP64. */
+poe2d_status poe2d_wavelet(int kind, double f0, double t0, double amp, double t_cut, double dt,
+                           int n, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* POE2D_H */

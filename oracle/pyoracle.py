@@ -1,0 +1,130 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes wrapper of the oracle (oracle/libpoe2d_oracle.so).
+
+Used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as the checker.  It
+drives the oracle through the same call set as poe2d_amd.solver.Solver drives the product.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from poe2d_amd import _abi
+from poe2d_amd.solver import HandleSolver, Problem
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpoe2d_oracle.so")
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: run make -C oracle")
+    lib = ctypes.CDLL(LIB_PATH)
+    vp, i, d, sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_size_t
+    dp, fp = _abi.c_double_p, _abi.c_float_p
+    sigs = {
+        "create": ([ctypes.POINTER(_abi.Grid), ctypes.POINTER(_abi.Coeffs), ctypes.POINTER(vp)], i),
+        "set_source": ([vp, ctypes.POINTER(_abi.Source)], i),
+        "set_wavelet": ([vp, dp, i], i),
+        "set_receivers": ([vp, ctypes.POINTER(_abi.Receivers)], i),
+        "set_threads": ([vp, i], i),
+        "set_reference_scans": ([vp, i, i, i, i], i),
+        "run": ([vp, i], i),
+        "step_count": ([vp, _abi.c_int_p], i),
+        "seismograms": ([vp, vp, sz], i),
+        "snapshot": ([vp, i, vp, sz], i),
+        "set_field": ([vp, i, vp, sz], i),
+        "cpml_table": ([vp, i, i, dp, i], i),
+        "destroy": ([vp], None),
+        "last_error": ([], ctypes.c_char_p),
+        "cpml_profile": ([i, i, d, d, i, d, d, d, d, d, i] + [dp] * 6, i),
+        "material_biot": ([sz] + [dp] * 18, i),
+        "material_reference": ([sz, fp, fp, fp, fp, i] + [fp] * 14, i),
+        "wavelet": ([i, d, d, d, d, d, i, dp], i),
+    }
+    for name, (args, res) in sigs.items():
+        fn = getattr(lib, "poe2d_oracle_" + name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+def _check(status, what):
+    if status != 0:
+        raise _abi.Poe2dError(status, f"oracle {what}: {_lib.poe2d_oracle_last_error().decode()}")
+
+
+class OracleSolver(HandleSolver):
+    prefix = "poe2d_oracle_"
+
+    def __init__(self, problem: Problem, precision=_abi.F32, threads=None, coeff_dtype=None):
+        super().__init__(load(), problem, precision, coeff_dtype)
+        n = threads if threads is not None else min(16, os.cpu_count() or 1)
+        self._check(self._lib.poe2d_oracle_set_threads(self._h, int(n)), "set_threads")
+
+    def set_field(self, fieldname, values):
+        f = _abi.FIELD_NAMES[fieldname] if isinstance(fieldname, str) else fieldname
+        a = np.ascontiguousarray(values, dtype=self.real)
+        assert a.shape == (self.problem.nz + 1, self.problem.nx + 1)
+        self._check(self._lib.poe2d_oracle_set_field(self._h, f, a.ctypes.data, a.nbytes), "set_field")
+
+    def set_reference_scans(self, panj, panj2, a0, a1):
+        self._check(self._lib.poe2d_oracle_set_reference_scans(self._h, panj, panj2, a0, a1),
+                    "set_reference_scans")
+
+    def cpml_table(self, axis, which):
+        n = (self.problem.nx if axis == 0 else self.problem.nz) + 1
+        out = np.zeros(n)
+        self._check(self._lib.poe2d_oracle_cpml_table(self._h, axis, which,
+                                                      out.ctypes.data_as(_abi.c_double_p), n),
+                    "cpml_table")
+        return out
+
+
+def cpml_profile(axis, n, delta, dt, npml, npower, rcoef, kmax, alpha_max, cp_max, precision):
+    lib = load()
+    outs = [np.zeros(n) for _ in range(6)]
+    _check(lib.poe2d_oracle_cpml_profile(axis, n, delta, dt, npml, npower, rcoef, kmax, alpha_max,
+                                         cp_max, precision,
+                                         *[o.ctypes.data_as(_abi.c_double_p) for o in outs]),
+           "cpml_profile")
+    return dict(zip(("a", "b", "K", "a_half", "b_half", "K_half"), outs))
+
+
+def material_biot(**p):
+    """Per-point Biot parameters (arrays or scalars) -> dict of the eight coefficients (FP64)."""
+    lib = load()
+    names = ("rho_s", "rho_f", "phi", "kappa", "eta", "tort", "K_s", "K_f", "K_fr", "mu_fr")
+    arrs = np.broadcast_arrays(*[np.asarray(p[k], dtype=np.float64) for k in names])
+    shape = arrs[0].shape
+    ins = [np.ascontiguousarray(a, dtype=np.float64).ravel() for a in arrs]
+    outs = [np.zeros(ins[0].size) for _ in range(8)]
+    _check(lib.poe2d_oracle_material_biot(ins[0].size, *[a.ctypes.data_as(_abi.c_double_p) for a in ins],
+                                          *[o.ctypes.data_as(_abi.c_double_p) for o in outs]),
+           "material_biot")
+    return {k: o.reshape(shape) for k, o in zip(_abi.COEFF_NAMES, outs)}
+
+
+def material_reference(cp, cs, rho_sat, medium, tables):
+    lib = load()
+    ins = [np.ascontiguousarray(a, dtype=np.float32).ravel() for a in (cp, cs, rho_sat, medium)]
+    tabs = [np.ascontiguousarray(tables[k], dtype=np.float32) for k in ("kf", "eta", "kappa", "phi", "a", "rhof")]
+    n = ins[0].size
+    outs = [np.zeros(n, dtype=np.float32) for _ in range(8)]
+    _check(lib.poe2d_oracle_material_reference(n, *[a.ctypes.data_as(_abi.c_float_p) for a in ins],
+                                               len(tabs[0]), *[t.ctypes.data_as(_abi.c_float_p) for t in tabs],
+                                               *[o.ctypes.data_as(_abi.c_float_p) for o in outs]),
+           "material_reference")
+    return {k: o.reshape(np.shape(cp)) for k, o in zip(_abi.COEFF_NAMES, outs)}
+
+
+def wavelet(kind, f0, t0, amp, t_cut, dt, n):
+    lib = load()
+    out = np.zeros(n)
+    _check(lib.poe2d_oracle_wavelet(kind, f0, t0, amp, t_cut, dt, n, out.ctypes.data_as(_abi.c_double_p)),
+           "wavelet")
+    return out

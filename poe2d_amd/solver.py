@@ -1,0 +1,213 @@
+"""Python handle over the poe2d C ABI (include/poe2d.h).
+
+`Problem` describes one simulation: grid and C-PML parameters, the eight coefficient arrays, the
+source and the receivers.  `Solver` creates a libpoe2d handle for it, runs steps and returns the
+traces and snapshots as numpy arrays.  The same call sequence the reference's main() performs
+(set-up, upload, stepping, gather, D2H) maps onto create / set_source / set_receivers / run /
+seismograms.
+"""
+import ctypes
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+from . import _abi
+from ._abi import COEFF_NAMES
+
+
+@dataclass
+class SourceSpec:
+    kind: int = _abi.SRC_FORCE_Y_RICKER
+    ix: int = 0
+    iz: int = 0
+    amp: float = 1.0e10
+    t0: float = 0.0      # <= 0: 1/f0
+    t_cut: float = 0.0   # <= 0: no cut
+
+
+@dataclass
+class ReceiverSpec:
+    ix: np.ndarray
+    iz: np.ndarray
+    comp_mask: int = _abi.VX | _abi.VY | _abi.P
+    decimate: int = 1
+    nsamp: int = 0
+
+    @property
+    def n(self):
+        return int(len(self.ix))
+
+    @property
+    def ncomp(self):
+        return bin(self.comp_mask & 7).count("1")
+
+
+@dataclass
+class Problem:
+    nx: int
+    nz: int
+    dx: float
+    dz: float
+    dt: float
+    nt: int
+    coeffs: dict                 # name -> (nz+1, nx+1) array, names in COEFF_NAMES
+    source: Optional[SourceSpec] = None
+    receivers: Optional[ReceiverSpec] = None
+    npml: int = 20
+    npower: float = 2.0
+    rcoef: float = 1.0e-3
+    kmax: float = 1.0
+    alpha_max: float = 0.0       # <= 0: pi*f0
+    f0: float = 80.0
+    cp_max: float = 0.0          # <= 0: from the coefficients
+    check_cfl: int = 1
+    wavelet: Optional[np.ndarray] = None
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def npoints(self):
+        return (self.nx + 1) * (self.nz + 1)
+
+    def grid_struct(self, precision):
+        return _abi.Grid(self.nx, self.nz, self.dx, self.dz, self.dt, self.npml, self.npower,
+                         self.rcoef, self.kmax, self.alpha_max, self.f0, self.cp_max,
+                         precision, self.check_cfl)
+
+
+class CoeffBuffers:
+    """Keeps the coefficient arrays alive while a Coeffs struct points at them."""
+
+    def __init__(self, coeffs, nx, nz, dtype=np.float32):
+        self.arrays = {}
+        for name in COEFF_NAMES:
+            a = np.ascontiguousarray(coeffs[name], dtype=dtype)
+            if a.shape != (nz + 1, nx + 1):
+                raise ValueError(f"coefficient {name} has shape {a.shape}, need {(nz + 1, nx + 1)}")
+            self.arrays[name] = a
+        code = _abi.F32 if dtype == np.float32 else _abi.F64
+        self.struct = _abi.Coeffs(code, *[self.arrays[n].ctypes.data for n in COEFF_NAMES], nx + 1)
+
+
+class HandleSolver:
+    """A solver handle behind a C ABI with the poe2d call set; `prefix` selects the library's
+    symbol family."""
+
+    prefix = "poe2d_"
+
+    def __init__(self, lib, problem: Problem, precision=_abi.F32, coeff_dtype=None):
+        self._lib = lib
+        self.problem = problem
+        self.precision = precision
+        self.real = np.float32 if precision == _abi.F32 else np.float64
+        cdt = coeff_dtype or (np.float32 if precision == _abi.F32 else np.float64)
+        bufs = CoeffBuffers(problem.coeffs, problem.nx, problem.nz, cdt)
+        grid = problem.grid_struct(precision)
+        h = ctypes.c_void_p()
+        self._check(self._fn("create")(ctypes.byref(grid), ctypes.byref(bufs.struct), ctypes.byref(h)),
+                    "create")
+        self._h = h
+        if problem.source is not None:
+            self.set_source(problem.source)
+        if problem.wavelet is not None:
+            self.set_wavelet(problem.wavelet)
+        if problem.receivers is not None:
+            self.set_receivers(problem.receivers, problem.nt)
+
+    # -- plumbing ----------------------------------------------------------------------------
+    def _fn(self, name):
+        return getattr(self._lib, self.prefix + name)
+
+    def _check(self, status, what):
+        if status != 0:
+            msg = self._fn("last_error")()
+            raise _abi.Poe2dError(status, f"{what}: {msg.decode() if msg else ''}")
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._fn("destroy")(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- API ---------------------------------------------------------------------------------
+    def set_source(self, src: SourceSpec):
+        s = _abi.Source(src.kind, src.ix, src.iz, src.amp, src.t0, src.t_cut)
+        self._check(self._fn("set_source")(self._h, ctypes.byref(s)), "set_source")
+
+    def set_wavelet(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        self._check(self._fn("set_wavelet")(self._h, w.ctypes.data_as(_abi.c_double_p), len(w)),
+                    "set_wavelet")
+
+    def set_receivers(self, rec: ReceiverSpec, nt=None):
+        self._rix = np.ascontiguousarray(rec.ix, dtype=np.int32)
+        self._riz = np.ascontiguousarray(rec.iz, dtype=np.int32)
+        nsamp = rec.nsamp or -(-(nt or self.problem.nt) // rec.decimate)
+        self._rec = rec
+        self._nsamp = nsamp
+        r = _abi.Receivers(len(self._rix), self._rix.ctypes.data_as(_abi.c_int_p),
+                           self._riz.ctypes.data_as(_abi.c_int_p), rec.comp_mask, rec.decimate, nsamp)
+        self._check(self._fn("set_receivers")(self._h, ctypes.byref(r)), "set_receivers")
+
+    def run(self, nsteps=None):
+        n = self.problem.nt if nsteps is None else nsteps
+        self._check(self._fn("run")(self._h, int(n)), "run")
+
+    def step_count(self):
+        n = ctypes.c_int()
+        self._check(self._fn("step_count")(self._h, ctypes.byref(n)), "step_count")
+        return n.value
+
+    def seismograms(self):
+        """Traces as an array [ncomp, nrec, nsamp] (components VX, VY, P within comp_mask)."""
+        rec = self._rec
+        out = np.zeros((rec.ncomp, rec.n, self._nsamp), dtype=self.real)
+        self._check(self._fn("seismograms")(self._h, out.ctypes.data, out.nbytes), "seismograms")
+        return out
+
+    def snapshot(self, fieldname):
+        f = _abi.FIELD_NAMES[fieldname] if isinstance(fieldname, str) else fieldname
+        out = np.zeros((self.problem.nz + 1, self.problem.nx + 1), dtype=self.real)
+        self._check(self._fn("snapshot")(self._h, f, out.ctypes.data, out.nbytes), "snapshot")
+        return out
+
+
+class Solver(HandleSolver):
+    """A libpoe2d (GPU) handle."""
+
+    prefix = "poe2d_"
+
+    def __init__(self, problem: Problem, precision=_abi.F32, coeff_dtype=None):
+        super().__init__(_abi.load_library(), problem, precision, coeff_dtype)
+
+    def last_run_ms(self):
+        ms = ctypes.c_double()
+        n = ctypes.c_int()
+        self._check(self._lib.poe2d_last_run_ms(self._h, ctypes.byref(ms), ctypes.byref(n)),
+                    "last_run_ms")
+        return ms.value, n.value
+
+    def reset(self):
+        self._check(self._lib.poe2d_reset(self._h), "reset")
+
+    def checkpoint(self):
+        n = ctypes.c_size_t()
+        self._check(self._lib.poe2d_checkpoint_size(self._h, ctypes.byref(n)), "checkpoint_size")
+        blob = np.zeros(n.value, dtype=np.uint8)
+        self._check(self._lib.poe2d_checkpoint(self._h, blob.ctypes.data, blob.nbytes), "checkpoint")
+        return blob
+
+    def restore(self, blob):
+        blob = np.ascontiguousarray(blob, dtype=np.uint8)
+        self._check(self._lib.poe2d_restore(self._h, blob.ctypes.data, blob.nbytes), "restore")
