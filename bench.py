@@ -1,0 +1,153 @@
+"""Benchmark: FP32 Biot C-PML stepping on configuration 1 (4001 x 4001 points, patchy gas 0.15
+over brine rock, 6000 steps, 801 receivers x (vx, vy, p)) on one MI355X.
+
+A "step" of this benchmark is one full solve (nt = 6000 time steps) with the inputs resident in
+HBM; set-up, uploads and trace download are outside the timed region.  value = Gcell-updates/s =
+4001^2 * 6000 * K / (time of K solves).  The roofline figure uses the algorithmic traffic of the
+time step: 104 B per point-update (9 state fields read + written, 8 coefficients read, FP32) plus
+48 B per C-PML strip point and 96 B per corner point, divided by the device time per time step
+(hipEvent on the solver's stream).  The CPU baseline is the oracle in its reference-structured
+form (five sweeps plus three full-grid receiver scans per step, one thread) timed on a bounded
+sample of the same model.
+"""
+import argparse
+import json
+import os
+import platform
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_POINT = 104.0
+BYTES_PER_STRIP_POINT = 48.0
+
+
+def algorithmic_bytes_per_step(nx, nz, npml_cols, npml_rows):
+    """Per time step: 104 B per point + 48 B per strip point (6 memory variables read + written)
+    in the x strips and again in the z strips (corners count twice: 96 B)."""
+    npts = (nx + 1) * (nz + 1)
+    return BYTES_PER_POINT * npts + BYTES_PER_STRIP_POINT * (npml_cols * (nz + 1) + npml_rows * (nx + 1))
+
+
+def strip_counts(prob):
+    from poe2d_amd.material import cpml_profile
+    from poe2d_amd import F32
+    cols = rows = 0
+    for axis, n, d in ((0, prob.nx + 1, prob.dx), (1, prob.nz + 1, prob.dz)):
+        t = cpml_profile(axis, n, d, prob.dt, prob.npml, prob.npower, prob.rcoef, prob.kmax,
+                         np.pi * prob.f0, 3605.0, F32)
+        active = (t["a"] != 0) | (t["a_half"] != 0) | (t["K"] != 1) | (t["K_half"] != 1)
+        if axis == 0:
+            cols = int(active.sum())
+        else:
+            rows = int(active.sum())
+    return cols, rows
+
+
+def cpu_baseline(prob, steps):
+    from oracle.pyoracle import OracleSolver
+    s = OracleSolver(prob, threads=1)
+    s.set_reference_scans(122, 680, prob.nz // 2 + 100, prob.nz // 2 + 200)
+    s.run(1)  # untimed first step (page-in)
+    t = time.perf_counter()
+    s.run(steps)
+    dt = time.perf_counter() - t
+    npts = (prob.nx + 1) * (prob.nz + 1)
+    cpu_model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu_model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": npts * steps / dt / 1e9, "unit": "Gcell-updates/s", "cores": 1, "kind": "port",
+            "sample": f"{steps} time steps of the {prob.nx + 1}x{prob.nz + 1} configuration-1 model "
+                      f"({dt:.1f} s), reference-structured oracle (5 sweeps + 3 full-grid receiver scans), "
+                      f"1 thread; host {cpu_model or platform.processor()}, nproc {os.cpu_count()}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3, help="timed full solves")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed full solves")
+    ap.add_argument("--n", type=int, default=4000, help="NX = NZ (grid has n+1 points per side)")
+    ap.add_argument("--nt", type=int, default=6000)
+    ap.add_argument("--cpu-steps", type=int, default=12, help="time steps of the CPU baseline sample (0: skip)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (--gpus {args.gpus} requested): every box here has one "
+              "GPU; the multi-GPU slab decomposition is a design, not a measured path.", file=sys.stderr)
+        sys.exit(2)
+
+    from poe2d_amd import F32
+    from poe2d_amd.configs import config1
+    from poe2d_amd.solver import Solver
+
+    prob = config1(nt=args.nt, n=args.n)
+    npts = (prob.nx + 1) * (prob.nz + 1)
+    cols, rows = strip_counts(prob)
+    bytes_step = algorithmic_bytes_per_step(prob.nx, prob.nz, cols, rows)
+
+    solver = Solver(prob, precision=F32)
+    for _ in range(args.warmup):
+        solver.reset()
+        solver.run(args.nt)
+    dev_ms, wall_ms = [], []
+    for _ in range(args.steps):
+        solver.reset()
+        t = time.perf_counter()
+        solver.run(args.nt)
+        wall_ms.append((time.perf_counter() - t) * 1e3)
+        ms, n = solver.last_run_ms()
+        assert n == args.nt
+        dev_ms.append(ms)
+    seis = solver.seismograms()
+    finite = bool(np.isfinite(seis).all())
+    solver.close()
+
+    ms_solve = statistics.median(dev_ms)
+    gcell = npts * args.nt / (ms_solve / 1e3) / 1e9
+    step_s = ms_solve / 1e3 / args.nt
+    achieved = bytes_step / step_s / 1e9
+    out = {
+        "metric": "Gcell-updates/s (FP32 Biot C-PML time loop, 4001x4001, 6000 steps)",
+        "value": round(gcell * 1.0, 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_solve, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": prob.meta["workload"] + f", {args.nt} steps, "
+                               f"{prob.receivers.n} receivers x (vx, vy, p)",
+                   "grid": [prob.nx + 1, prob.nz + 1], "nt": args.nt, "npml": prob.npml,
+                   "gas_fraction": prob.meta.get("gas_fraction_realised"),
+                   "step": "one full solve", "wall_ms_median": round(statistics.median(wall_ms), 3),
+                   "traces_finite": finite},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "per_launch": f"one time step: {bytes_step / 1e9:.4f} GB algorithmic "
+                                   f"(104 B x {npts} points + 48 B x strip points, {cols} strip columns, "
+                                   f"{rows} strip rows)"},
+    }
+    if args.cpu_steps > 0:
+        from oracle.pyoracle import material_biot
+        cprob = config1(nt=args.cpu_steps + 1, n=args.n, material_fn=material_biot)
+        out["cpu_baseline"] = cpu_baseline(cprob, args.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

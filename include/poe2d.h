@@ -139,6 +139,14 @@ poe2d_status poe2d_set_wavelet(poe2d_solver *s, const double *w, int n);
  * samprek (:496-502, :759-762) and the save1_kernel gather (:114-201). */
 poe2d_status poe2d_set_receivers(poe2d_solver *s, const poe2d_receivers *rec);
 
+/* Choose how a time step is executed on this handle (no counterpart in the reference, whose
+ * six launches per step are fixed at seismic_CPML_2D_poroelastic_withCUDA.cu:1628-1634).
+ *   schedule 0 (default): one fused launch per step; tiles of block_w - 8 columns (block_w =
+ *     128, 256 or 512 threads, default 256) by tile_rows rows (default 64).
+ *   schedule 1: a stress launch and a velocity launch per step, intermediates through memory.
+ * Results agree to rounding; the choice only changes speed. */
+poe2d_status poe2d_set_schedule(poe2d_solver *s, int schedule, int block_w, int tile_rows);
+
 /* Advance nsteps time steps on the handle's stream and wait for them.  Each step performs the
  * reference's stress1 -> save1 -> stress2 -> stress3 -> vx -> vz sequence
  * (seismic_CPML_2D_poroelastic_withCUDA.cu:1625-1636). */
@@ -217,6 +225,18 @@ poe2d_status poe2d_material_reference(size_t n, const float *cp, const float *cs
 /* Wavelet samples w[k] = f(k*dt) of a source kind (amp applied, t_cut applied), FP64. */
 poe2d_status poe2d_wavelet(int kind, double f0, double t0, double amp, double t_cut, double dt,
                            int n, double *out);
+
+/* Seeded random patchy-saturation mask on (nz+1) x (nx+1) points (x fastest): disks with
+ * uniform random centres over [0, nx dx] x [0, nz dz] and log-uniform radii in [rmin, rmax]
+ * metres are painted (value 1) until at least gas_fraction of the points are covered.  No
+ * counterpart in the reference, whose patchymodel.m only draws fixed lattices (below). */
+poe2d_status poe2d_patchy_mask(int nx, int nz, double dx, double dz, double gas_fraction,
+                               uint64_t seed, double rmin, double rmax, uint8_t *mask,
+                               double *realised);
+
+/* One of the six disk lattices of patchymodel.m:13-42 and the per-type blocks after it
+ * (radius r points, rows x cols layer, 1 inside a disk).  type = 1..6. */
+poe2d_status poe2d_patchy_lattice(int type, int rows, int cols, int r, uint8_t *layer);
 
 #ifdef __cplusplus
 }

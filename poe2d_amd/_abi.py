@@ -109,6 +109,7 @@ def _declare(lib):
         "poe2d_set_wavelet": ([vp, c_double_p, i], i),
         "poe2d_set_receivers": ([vp, ctypes.POINTER(Receivers)], i),
         "poe2d_run": ([vp, i], i),
+        "poe2d_set_schedule": ([vp, i, i, i], i),
         "poe2d_last_run_ms": ([vp, c_double_p, c_int_p], i),
         "poe2d_step_count": ([vp, c_int_p], i),
         "poe2d_seismograms": ([vp, vp, sz], i),
@@ -124,6 +125,9 @@ def _declare(lib):
         "poe2d_material_reference": ([sz, c_float_p, c_float_p, c_float_p, c_float_p, i]
                                      + [c_float_p] * 14, i),
         "poe2d_wavelet": ([i, d, d, d, d, d, i, c_double_p], i),
+        "poe2d_patchy_mask": ([i, i, d, d, d, ctypes.c_uint64, d, d,
+                               ctypes.POINTER(ctypes.c_uint8), c_double_p], i),
+        "poe2d_patchy_lattice": ([i, i, i, i, ctypes.POINTER(ctypes.c_uint8)], i),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(lib, name)

@@ -1,0 +1,127 @@
+// Random patchy-saturation model generator.
+//
+// The reference builds its patchy media in MATLAB (patchymodel.m): radius-8 disks on six
+// deterministic lattices, painted into a medium-index map.  The workloads here need seeded
+// random media instead: disks with uniform random centres and log-uniform radii are painted
+// until the painted fraction of grid points reaches a target.  The lattice layouts of
+// patchymodel.m are kept as a separate mode (poe2d_patchy_lattice).
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+
+#include "common.h"
+
+namespace {
+
+// SplitMix64 (Steele, Lea, Flood 2014): a 64-bit counter-based generator, used directly.
+struct SplitMix64 {
+    uint64_t s;
+    uint64_t next()
+    {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    double uniform() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }  // [0, 1)
+};
+
+// Paints the points within distance r of (cx, cz) (metres); returns how many were newly painted.
+size_t paint_disk(uint8_t *mask, int nx, int nz, double dx, double dz, double cx, double cz, double r)
+{
+    const int x0 = std::max(0, (int)std::ceil((cx - r) / dx));
+    const int x1 = std::min(nx, (int)std::floor((cx + r) / dx));
+    const int z0 = std::max(0, (int)std::ceil((cz - r) / dz));
+    const int z1 = std::min(nz, (int)std::floor((cz + r) / dz));
+    size_t added = 0;
+    for (int z = z0; z <= z1; ++z) {
+        const double ddz = z * dz - cz;
+        for (int x = x0; x <= x1; ++x) {
+            const double ddx = x * dx - cx;
+            if (ddx * ddx + ddz * ddz <= r * r) {
+                uint8_t &m = mask[(size_t)z * (nx + 1) + x];
+                if (!m) {
+                    m = 1;
+                    ++added;
+                }
+            }
+        }
+    }
+    return added;
+}
+
+}  // namespace
+
+using poe2d::fail;
+
+extern "C" {
+
+// Fills mask[(nz+1)*(nx+1)] (x fastest) with 1 for patch points.  Disks are drawn with centres
+// uniform over [0, nx dx] x [0, nz dz] and radii log-uniform in [rmin, rmax] metres until at least
+// gas_fraction of all points are painted.  Same seed, same mask.
+poe2d_status poe2d_patchy_mask(int nx, int nz, double dx, double dz, double gas_fraction,
+                               uint64_t seed, double rmin, double rmax, uint8_t *mask,
+                               double *realised)
+{
+    if (nx < 1 || nz < 1 || !(dx > 0) || !(dz > 0) || !mask)
+        return fail(POE2D_E_INVALID, "patchy_mask: bad grid");
+    if (!(gas_fraction >= 0 && gas_fraction < 1))
+        return fail(POE2D_E_INVALID, "patchy_mask: gas_fraction must be in [0, 1)");
+    if (!(rmin > 0) || !(rmax >= rmin)) return fail(POE2D_E_INVALID, "patchy_mask: need 0 < rmin <= rmax");
+    const size_t npts = (size_t)(nx + 1) * (nz + 1);
+    std::memset(mask, 0, npts);
+    const size_t target = (size_t)std::ceil(gas_fraction * (double)npts);
+    SplitMix64 rng{seed};
+    size_t painted = 0;
+    const double lr0 = std::log(rmin), lr1 = std::log(rmax);
+    const double W = nx * dx, H = nz * dz;
+    size_t draws = 0;
+    while (painted < target) {
+        const double cx = rng.uniform() * W;
+        const double cz = rng.uniform() * H;
+        const double r = std::exp(lr0 + (lr1 - lr0) * rng.uniform());
+        painted += paint_disk(mask, nx, nz, dx, dz, cx, cz, r);
+        if (++draws > 100000000ull) return fail(POE2D_E_INVALID, "patchy_mask: target fraction not reachable");
+    }
+    if (realised) *realised = (double)painted / (double)npts;
+    return POE2D_OK;
+}
+
+// The six lattice layouts of patchymodel.m (types 1..6): disks of radius r points on a
+// (rows x cols) layer, value 1 inside a disk and 0 outside.  Lattice steps for each type (first
+// centre, step along x, first centre, step along z), with dia = 2r, 1-based centres as in the
+// MATLAB loops; types 4-6 superpose two lattices:
+//   1: (r+1, dia)   x (r+1, dia)        2: (r+1, 2dia)  x (r+1, 2dia)
+//   3: (r+1, 4dia)  x (r+1, 2dia)       4: type 2 + (3r+1, 2dia) x (3r+1, 2dia)
+//   5: (r+1, 2dia) x (3r+1, 2dia) + (r+1, dia) x (r+1, 2dia)
+//   6: (r+1, 4dia) x (r+1, 2dia) + (5r+1, 4dia) x (3r+1, 2dia)
+poe2d_status poe2d_patchy_lattice(int type, int rows, int cols, int r, uint8_t *layer)
+{
+    if (type < 1 || type > 6 || rows < 1 || cols < 1 || r < 1 || !layer)
+        return fail(POE2D_E_INVALID, "patchy_lattice: bad arguments");
+    std::memset(layer, 0, (size_t)rows * cols);
+    const int dia = 2 * r;
+    struct Lat { int x0, sx, z0, sz; };
+    Lat lats[2];
+    int nl = 1;
+    switch (type) {
+    case 1: lats[0] = {r + 1, dia, r + 1, dia}; break;
+    case 2: lats[0] = {r + 1, 2 * dia, r + 1, 2 * dia}; break;
+    case 3: lats[0] = {r + 1, 4 * dia, r + 1, 2 * dia}; break;
+    case 4: lats[0] = {r + 1, 2 * dia, r + 1, 2 * dia}; lats[1] = {3 * r + 1, 2 * dia, 3 * r + 1, 2 * dia}; nl = 2; break;
+    case 5: lats[0] = {r + 1, 2 * dia, 3 * r + 1, 2 * dia}; lats[1] = {r + 1, dia, r + 1, 2 * dia}; nl = 2; break;
+    default: lats[0] = {r + 1, 4 * dia, r + 1, 2 * dia}; lats[1] = {5 * r + 1, 4 * dia, 3 * r + 1, 2 * dia}; nl = 2; break;
+    }
+    for (int l = 0; l < nl; ++l)
+        for (int ci = lats[l].x0; ci <= cols; ci += lats[l].sx)
+            for (int cj = lats[l].z0; cj <= rows; cj += lats[l].sz)
+                for (int j = std::max(1, cj - r); j <= std::min(rows, cj + r); ++j)
+                    for (int i = std::max(1, ci - r); i <= std::min(cols, ci + r); ++i) {
+                        const double d = std::sqrt((double)(i - ci) * (i - ci) + (double)(j - cj) * (j - cj));
+                        if (d <= r) layer[(size_t)(j - 1) * cols + (i - 1)] = 1;
+                    }
+    return POE2D_OK;
+}
+
+}  // extern "C"

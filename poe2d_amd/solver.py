@@ -201,6 +201,11 @@ class Solver(HandleSolver):
     def reset(self):
         self._check(self._lib.poe2d_reset(self._h), "reset")
 
+    def set_schedule(self, schedule=0, block_w=256, tile_rows=64):
+        """0: single fused launch per step (tiles of block_w-8 columns x tile_rows rows);
+        1: separate stress and velocity launches."""
+        self._check(self._lib.poe2d_set_schedule(self._h, schedule, block_w, tile_rows), "set_schedule")
+
     def checkpoint(self):
         n = ctypes.c_size_t()
         self._check(self._lib.poe2d_checkpoint_size(self._h, ctypes.byref(n)), "checkpoint_size")
