@@ -73,6 +73,25 @@ def cpu_baseline(prob, steps):
                       f"1 thread; host {cpu_model or platform.processor()}, nproc {os.cpu_count()}"}
 
 
+def copy_probe(mib, reps=20):
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    moved = 2 * n * 4
+    print(json.dumps({"copy_probe_mib": mib, "bytes_read_per_copy": n * 4, "bytes_written_per_copy": n * 4,
+                      "ms_per_copy": round(ms, 4), "GBps_read_plus_write": round(moved / ms / 1e6, 1)}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,7 +100,14 @@ def main():
     ap.add_argument("--n", type=int, default=4000, help="NX = NZ (grid has n+1 points per side)")
     ap.add_argument("--nt", type=int, default=6000)
     ap.add_argument("--cpu-steps", type=int, default=12, help="time steps of the CPU baseline sample (0: skip)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="instead of the benchmark: time a plain device copy of MIB MiB (torch), the "
+                         "reference point for achievable HBM bandwidth and for scaling the FETCH_SIZE / "
+                         "WRITE_SIZE counters")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (--gpus {args.gpus} requested): every box here has one "
               "GPU; the multi-GPU slab decomposition is a design, not a measured path.", file=sys.stderr)

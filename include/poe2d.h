@@ -142,10 +142,20 @@ poe2d_status poe2d_set_receivers(poe2d_solver *s, const poe2d_receivers *rec);
 /* Choose how a time step is executed on this handle (no counterpart in the reference, whose
  * six launches per step are fixed at seismic_CPML_2D_poroelastic_withCUDA.cu:1628-1634).
  *   schedule 0 (default): one fused launch per step; tiles of block_w - 8 columns (block_w =
- *     128, 256 or 512 threads, default 256) by tile_rows rows (default 64).
+ *     128, 256 or 512 threads, default 256) by tile_rows rows (default 64).  When the grid holds
+ *     at most 256 distinct coefficient tuples (see poe2d_material_palette) the coefficients are
+ *     read through a one-byte material index per point.
  *   schedule 1: a stress launch and a velocity launch per step, intermediates through memory.
- * Results agree to rounding; the choice only changes speed. */
+ *   schedule 2: as schedule 0, always reading the per-point coefficient arrays.
+ * Tile heights do not change a bit; schedules and block widths agree to rounding.  The choice
+ * only changes speed. */
 poe2d_status poe2d_set_schedule(poe2d_solver *s, int schedule, int block_w, int tile_rows);
+
+/* Number of distinct coefficient tuples of the grid when it has at most 256 (the palette the
+ * fused step reads through a one-byte material index), else 0.  No counterpart in the
+ * reference, which reads every coefficient array per point (seismic_CPML_2D_poroelastic_withCUDA.cu:
+ * 60-111, 289-340). */
+poe2d_status poe2d_material_palette(poe2d_solver *s, int *entries);
 
 /* Advance nsteps time steps on the handle's stream and wait for them.  Each step performs the
  * reference's stress1 -> save1 -> stress2 -> stress3 -> vx -> vz sequence

@@ -110,6 +110,7 @@ def _declare(lib):
         "poe2d_set_receivers": ([vp, ctypes.POINTER(Receivers)], i),
         "poe2d_run": ([vp, i], i),
         "poe2d_set_schedule": ([vp, i, i, i], i),
+        "poe2d_material_palette": ([vp, c_int_p], i),
         "poe2d_last_run_ms": ([vp, c_double_p, c_int_p], i),
         "poe2d_step_count": ([vp, c_int_p], i),
         "poe2d_seismograms": ([vp, vp, sz], i),

@@ -1,31 +1,34 @@
-// Launchers of the auxiliary kernels (receiver gather, derived fields, stability reduction).
+// Launchers of the auxiliary kernels (receiver gather, derived fields, component extraction,
+// stability reduction).
 #pragma once
 
 #include "step_common.h"
 
 namespace poe2d {
 
-template <class T>
-struct GatherArgs {
-    int nx, nz, pitch, nrec;
+struct Receivers {
+    int n;
     unsigned mask;
-    const int *rix, *riz;
-    const T *vx, *vz;            // velocities at the start of the step
-    const T *exx, *eyy, *zeta;   // strains / fluid content after the step's stress update
-    const T *al, *M;
-    int src_kind, src_x, src_z;
-    T src_val;
-    T *out;                      // [ncomp][nrec] of this slot
+    const int *ix, *iz;
 };
 
+// Gather of one slot: velocities from state buffer `vbuf` (the state before the step, v^n) and
+// pore pressure recomputed from the strains / fluid content of buffer `sbuf` (after the step's
+// stress update) plus the pressure source of the step.  out = [ncomp][nrec].
 template <class T>
-void launch_gather(const GatherArgs<T> &G, hipStream_t stream);
+void launch_gather(const StepArgs<T> &A, int vbuf, int sbuf, const Receivers &R, T *out, hipStream_t stream);
 
+// p, sxx or syy (what = 0, 1, 2) of buffer `buf`'s state into a dense (nz+1) x (nx+1) array.
 template <class T>
-void launch_derived(const GatherArgs<T> &G, const T *lc, const T *mu, int what, T *out, hipStream_t stream);
+void launch_derived(const StepArgs<T> &A, int buf, int what, T *out, hipStream_t stream);
 
+// Component `comp` of group `g` into a dense (nz+1) x (nx+1) array.
 template <class T>
-void launch_absmax(const T *a, const T *b, size_t n, float *partials, int nblocks, float *result,
+void launch_extract(const StepArgs<T> &A, int g, int comp, T *out, hipStream_t stream);
+
+// Largest |vx|, |vy| of buffer `buf` (non-finite counts as +inf) into result[0].
+template <class T>
+void launch_absmax(const StepArgs<T> &A, int buf, float *partials, int nblocks, float *result,
                    hipStream_t stream);
 
 }  // namespace poe2d

@@ -1,67 +1,80 @@
-// Small kernels around the time step: receiver gather, derived-field recompute for snapshots,
-// and the stability reduction.
+// Small kernels around the time step: receiver gather, derived-field recompute and component
+// extraction for snapshots, and the stability reduction.
 #include "aux_kernels.h"
 
 namespace poe2d {
 
-// Gather of step n's slot: velocities from the state before the step (v^n) and pore pressure
-// recomputed from the strains / fluid content the step produced (p^(n+1/2)), plus the pressure
-// source when it sits on a receiver.
+namespace {
+
 template <class T>
-__global__ void gather_kernel(GatherArgs<T> G)
+__device__ T pressure_at(const StepArgs<T> &A, const Vec4<T> *S, const Vec4<T> *MS, int x, int z, long long o)
 {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= G.nrec) return;
-    const int x = G.rix[r], z = G.riz[r];
-    const ptrdiff_t o = (ptrdiff_t)z * G.pitch + x;
-    int c = 0;
-    if (G.mask & POE2D_VX) G.out[(size_t)(c++) * G.nrec + r] = G.vx[o];
-    if (G.mask & POE2D_VY) G.out[(size_t)(c++) * G.nrec + r] = G.vz[o];
-    if (G.mask & POE2D_P) {
-        T p = 0;
-        if (x >= 1 && x < G.nx && z >= 2 && z < G.nz - 1) {
-            p = pore_pressure(G.exx[o], G.eyy[o], G.zeta[o], G.al[o], G.M[o]);
-            if (G.src_kind == POE2D_SRC_PRESSURE_DGAUSS && x == G.src_x && z == G.src_z) p += G.src_val;
-        }
-        G.out[(size_t)c * G.nrec + r] = p;
-    }
+    if (!(x >= 1 && x < A.nx && z >= 2 && z < A.nz - 1)) return T(0);
+    const Vec4<T> s = S[o], m = MS[o];
+    T p = pore_pressure(s.x, s.y, s.z, m.x, m.y);
+    if (A.src_kind == POE2D_SRC_PRESSURE_DGAUSS && x == A.src_x && z == A.src_z) p += A.src_val;
+    return p;
 }
 
-// p, sxx or syy of the last stress update, recomputed from the state (what = 0, 1, 2), into a
-// dense (nz+1) x (nx+1) array.
 template <class T>
-__global__ void derived_kernel(GatherArgs<T> G, const T *lc, const T *mu, int what, T *out)
+__global__ void gather_kernel(StepArgs<T> A, int vbuf, int sbuf, Receivers R, T *out)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R.n) return;
+    const int x = R.ix[r], z = R.iz[r];
+    const long long o = (long long)z * A.pitch + x;
+    const Vec4<T> v = group_c(A, group_v(vbuf))[o];
+    int c = 0;
+    if (R.mask & POE2D_VX) out[(size_t)(c++) * R.n + r] = v.x;
+    if (R.mask & POE2D_VY) out[(size_t)(c++) * R.n + r] = v.y;
+    if (R.mask & POE2D_P) out[(size_t)c * R.n + r] = pressure_at(A, group_c(A, group_s(sbuf)), group_c(A, G_MS), x, z, o);
+}
+
+template <class T>
+__global__ void derived_kernel(StepArgs<T> A, int buf, int what, T *out)
 {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int z = blockIdx.y;
-    if (x > G.nx || z > G.nz) return;
-    const ptrdiff_t o = (ptrdiff_t)z * G.pitch + x;
-    T v = 0;
-    if (x >= 1 && x < G.nx && z >= 2 && z < G.nz - 1) {
-        const T al = G.al[o];
-        T p = pore_pressure(G.exx[o], G.eyy[o], G.zeta[o], al, G.M[o]);
-        if (G.src_kind == POE2D_SRC_PRESSURE_DGAUSS && x == G.src_x && z == G.src_z) p += G.src_val;
-        if (what == 0) {
-            v = p;
-        } else if (x < G.nx - 1) {
-            const T l = lc[o], l2m = l + T(2) * mu[o];
-            v = what == 1 ? l2m * G.exx[o] + l * G.eyy[o] - al * p : l * G.exx[o] + l2m * G.eyy[o] - al * p;
+    if (x > A.nx || z > A.nz) return;
+    const long long o = (long long)z * A.pitch + x;
+    const Vec4<T> *S = group_c(A, group_s(buf));
+    const Vec4<T> *MS = group_c(A, G_MS);
+    const T p = pressure_at(A, S, MS, x, z, o);
+    T v = p;
+    if (what != 0) {
+        v = 0;
+        if (x >= 1 && x < A.nx - 1 && z >= 2 && z < A.nz - 1) {
+            const Vec4<T> s = S[o], m = MS[o];
+            const T l2m = m.z + T(2) * m.w;
+            v = what == 1 ? l2m * s.x + m.z * s.y - m.x * p : m.z * s.x + l2m * s.y - m.x * p;
         }
     }
-    out[(size_t)z * (G.nx + 1) + x] = v;
+    out[(size_t)z * (A.nx + 1) + x] = v;
 }
 
-// Largest |v| per block (non-finite values count as +inf), then one block reduces the partials.
 template <class T>
-__global__ void __launch_bounds__(256) absmax_blocks(const T *a, const T *b, size_t n, float *out)
+__global__ void extract_kernel(StepArgs<T> A, int g, int comp, T *out)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int z = blockIdx.y;
+    if (x > A.nx || z > A.nz) return;
+    const Vec4<T> v = group_c(A, g)[(long long)z * A.pitch + x];
+    out[(size_t)z * (A.nx + 1) + x] = comp == 0 ? v.x : comp == 1 ? v.y : comp == 2 ? v.z : v.w;
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) absmax_blocks(StepArgs<T> A, int buf, float *partial)
 {
     __shared__ float red[256];
+    const Vec4<T> *V = group_c(A, group_v(buf));
+    const long long npts = (long long)(A.nz + 1) * A.pitch;
     float m = 0.f;
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        float va = fabsf((float)a[i]), vb = fabsf((float)b[i]);
-        if (!(va <= 3.0e38f)) va = INFINITY;
-        if (!(vb <= 3.0e38f)) vb = INFINITY;
-        m = fmaxf(m, fmaxf(va, vb));
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < npts; i += (long long)gridDim.x * blockDim.x) {
+        const Vec4<T> v = V[i];  // rows 0..nz, pad columns included (zero)
+        float a = fabsf((float)v.x), b = fabsf((float)v.y);
+        if (!(a <= 3.0e38f)) a = INFINITY;
+        if (!(b <= 3.0e38f)) b = INFINITY;
+        m = fmaxf(m, fmaxf(a, b));
     }
     red[threadIdx.x] = m;
     __syncthreads();
@@ -69,7 +82,7 @@ __global__ void __launch_bounds__(256) absmax_blocks(const T *a, const T *b, siz
         if ((int)threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
 __global__ void __launch_bounds__(256) absmax_final(const float *in, int n, float *out)
@@ -86,30 +99,40 @@ __global__ void __launch_bounds__(256) absmax_final(const float *in, int n, floa
     if (threadIdx.x == 0) out[0] = red[0];
 }
 
+}  // namespace
+
 template <class T>
-void launch_gather(const GatherArgs<T> &G, hipStream_t stream)
+void launch_gather(const StepArgs<T> &A, int vbuf, int sbuf, const Receivers &R, T *out, hipStream_t stream)
 {
-    gather_kernel<T><<<(G.nrec + 255) / 256, 256, 0, stream>>>(G);
+    gather_kernel<T><<<(R.n + 255) / 256, 256, 0, stream>>>(A, vbuf, sbuf, R, out);
 }
 
 template <class T>
-void launch_derived(const GatherArgs<T> &G, const T *lc, const T *mu, int what, T *out, hipStream_t stream)
+void launch_derived(const StepArgs<T> &A, int buf, int what, T *out, hipStream_t stream)
 {
-    derived_kernel<T><<<dim3((G.nx + 1 + 255) / 256, G.nz + 1), 256, 0, stream>>>(G, lc, mu, what, out);
+    derived_kernel<T><<<dim3((A.nx + 1 + 255) / 256, A.nz + 1), 256, 0, stream>>>(A, buf, what, out);
 }
 
 template <class T>
-void launch_absmax(const T *a, const T *b, size_t n, float *partials, int nblocks, float *result, hipStream_t stream)
+void launch_extract(const StepArgs<T> &A, int g, int comp, T *out, hipStream_t stream)
 {
-    absmax_blocks<T><<<nblocks, 256, 0, stream>>>(a, b, n, partials);
+    extract_kernel<T><<<dim3((A.nx + 1 + 255) / 256, A.nz + 1), 256, 0, stream>>>(A, g, comp, out);
+}
+
+template <class T>
+void launch_absmax(const StepArgs<T> &A, int buf, float *partials, int nblocks, float *result, hipStream_t stream)
+{
+    absmax_blocks<T><<<nblocks, 256, 0, stream>>>(A, buf, partials);
     absmax_final<<<1, 256, 0, stream>>>(partials, nblocks, result);
 }
 
-template void launch_gather<float>(const GatherArgs<float> &, hipStream_t);
-template void launch_gather<double>(const GatherArgs<double> &, hipStream_t);
-template void launch_derived<float>(const GatherArgs<float> &, const float *, const float *, int, float *, hipStream_t);
-template void launch_derived<double>(const GatherArgs<double> &, const double *, const double *, int, double *, hipStream_t);
-template void launch_absmax<float>(const float *, const float *, size_t, float *, int, float *, hipStream_t);
-template void launch_absmax<double>(const double *, const double *, size_t, float *, int, float *, hipStream_t);
+#define POE2D_AUX_INSTANTIATE(T)                                                                      \
+    template void launch_gather<T>(const StepArgs<T> &, int, int, const Receivers &, T *, hipStream_t); \
+    template void launch_derived<T>(const StepArgs<T> &, int, int, T *, hipStream_t);                 \
+    template void launch_extract<T>(const StepArgs<T> &, int, int, T *, hipStream_t);                 \
+    template void launch_absmax<T>(const StepArgs<T> &, int, float *, int, float *, hipStream_t);
+POE2D_AUX_INSTANTIATE(float)
+POE2D_AUX_INSTANTIATE(double)
+#undef POE2D_AUX_INSTANTIATE
 
 }  // namespace poe2d

@@ -1,133 +1,138 @@
 // Two-pass time step: a stress pass (strain / fluid-content / shear-stress integration with their
-// C-PML memory variables, pore pressure and normal stresses) and a velocity pass (solid and
-// relative-fluid velocities with Biot inertial coupling and Crank-Nicolson viscous term).
+// C-PML memory variables, pore pressure and normal stresses, written to the intermediate group I)
+// and a velocity pass (solid and relative-fluid velocities with Biot inertial coupling and the
+// Crank-Nicolson viscous term).  Both work in place on the current state buffer.  This schedule is
+// kept beside the single-pass kernel as an independent GPU evaluation of the same step.
 //
-// One thread per grid point; every field is SoA with a padded row pitch whose pad columns are
-// zero and never written.  C-PML memory variables live only in compact strip arrays: columns
-// with a nonzero x profile (xmap >= 0) and rows with a nonzero z profile (zmap >= 0).  Outside
-// the strips a = 0, b = 1 and K = 1, so the memory variable stays zero and the C-PML derivative
-// d/K + psi equals d exactly; the interior therefore skips the memory update and the division.
+// One thread per grid point.  C-PML memory variables live only in compact strip arrays: columns
+// with a nonzero x profile (xmap >= 0) and rows with a nonzero z profile (zmap >= 0).  Outside the
+// strips a = 0, b = 1 and K = 1, so the memory variable stays zero and the C-PML derivative
+// d/K + psi equals d exactly (the tables hold 1/K); the interior therefore skips the memory update and the division.
 #include "step_common.h"
 
 namespace poe2d {
 
-// Stress pass of step n.  Index ranges are those of the reference kernels
-// (seismic_CPML_2D_poroelastic_withCUDA.cu): stress1 x in [1,NX), z in [2,NZ-1) (:60-62);
-// stress2 x in [2,NX), z in [1,NZ-1) (:212-214); stress3 x in [1,NX-1), z in [2,NZ) (:237-239).
 template <class T>
 __global__ void __launch_bounds__(256) stress_pass(StepArgs<T> A)
 {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int z = blockIdx.y * blockDim.y + threadIdx.y;
     if (x > A.nx || z > A.nz) return;
-    const size_t P = A.pitch;
-    const size_t o = (size_t)z * P + x;
+    const long long P = A.pitch;
+    const long long o = (long long)z * P + x;
+    const Vec4<T> *V = group_c(A, group_v(A.in));
+    Vec4<T> *S = group_w(A, group_s(A.in));
+    Vec4<T> *I = group_w(A, G_I);
+    const Vec4<T> *MS = group_c(A, G_MS);
     const int sx = A.xmap[x];
     const int sz = A.zmap[z];
+    const int b = A.in;
+    const T *tab = A.tab;
+    const int tn = A.tn;
+    Vec4<T> st = S[o];
 
-    const bool r1 = x >= 1 && x < A.nx && z >= 2 && z < A.nz - 1;
-    if (r1) {
-        T dvx = d4(A.vx[o + 1], A.vx[o], A.vx[o + 2], A.vx[o - 1], A.rdx);
-        T dwx = d4(A.wx[o + 1], A.wx[o], A.wx[o + 2], A.wx[o - 1], A.rdx);
-        T dvz = d4(A.vz[o], A.vz[o - P], A.vz[o + P], A.vz[o - 2 * P], A.rdz);
-        T dwz = d4(A.wz[o], A.wz[o - P], A.wz[o + P], A.wz[o - 2 * P], A.rdz);
+    if (x >= 1 && x < A.nx && z >= 2 && z < A.nz - 1) {
+        T dvx = d4(V[o + 1].x, V[o].x, V[o + 2].x, V[o - 1].x, A.rdx);
+        T dwx = d4(V[o + 1].z, V[o].z, V[o + 2].z, V[o - 1].z, A.rdx);
+        T dvz = d4(V[o].y, V[o - P].y, V[o + P].y, V[o - 2 * P].y, A.rdz);
+        T dwz = d4(V[o].w, V[o - P].w, V[o + P].w, V[o - 2 * P].w, A.rdz);
         if (sx >= 0) {
-            const size_t k = (size_t)z * A.wxs + sx;
-            dvx = cpml(dvx, A.psx[PX_DVX] + k, A.axh[x], A.bxh[x], A.Kxh[x]);
-            dwx = cpml(dwx, A.psx[PX_DWX] + k, A.axh[x], A.bxh[x], A.Kxh[x]);
+            const long long k = (long long)z * A.wxs + sx;
+            dvx = cpml(dvx, psi_x(A, b, k) + PX_DVX, psi_x(A, b, k) + PX_DVX, true, tab[TB_AXH * tn + x], tab[TB_BXH * tn + x], tab[TB_RKXH * tn + x]);
+            dwx = cpml(dwx, psi_x(A, b, k) + PX_DWX, psi_x(A, b, k) + PX_DWX, true, tab[TB_AXH * tn + x], tab[TB_BXH * tn + x], tab[TB_RKXH * tn + x]);
         }
         if (sz >= 0) {
-            const size_t k = (size_t)sz * P + x;
-            dvz = cpml(dvz, A.psz[PZ_DVZ] + k, A.az[z], A.bz[z], A.Kz[z]);
-            dwz = cpml(dwz, A.psz[PZ_DWZ] + k, A.az[z], A.bz[z], A.Kz[z]);
+            const long long k = (long long)sz * P + x;
+            dvz = cpml(dvz, psi_z(A, b, k) + PZ_DVZ, psi_z(A, b, k) + PZ_DVZ, true, tab[TB_AZ * tn + z], tab[TB_BZ * tn + z], tab[TB_RKZ * tn + z]);
+            dwz = cpml(dwz, psi_z(A, b, k) + PZ_DWZ, psi_z(A, b, k) + PZ_DWZ, true, tab[TB_AZ * tn + z], tab[TB_BZ * tn + z], tab[TB_RKZ * tn + z]);
         }
-        const T exx = A.exx[o] + A.dt * dvx;
-        const T eyy = A.eyy[o] + A.dt * dvz;
-        const T zeta = (A.zeta[o] - dwx * A.dt) - dwz * A.dt;
-        A.exx[o] = exx;
-        A.eyy[o] = eyy;
-        A.zeta[o] = zeta;
-        const T al = A.al[o], M = A.M[o];
-        T p = -al * M * (exx + eyy) + M * zeta;
+        st.x = st.x + A.dt * dvx;
+        st.y = st.y + A.dt * dvz;
+        st.z = (st.z - dwx * A.dt) - dwz * A.dt;
+        const Vec4<T> m = MS[o];
+        T p = pore_pressure(st.x, st.y, st.z, m.x, m.y);
         if (A.src_kind == POE2D_SRC_PRESSURE_DGAUSS && x == A.src_x && z == A.src_z) p += A.src_val;
-        A.p[o] = p;
+        T sxx = 0, syy = 0;
         if (x < A.nx - 1) {  // stress3 range minus its row NZ-1, where strains and p stay zero
-            const T lc = A.lc[o], mu = A.mu[o];
-            const T l2m = lc + T(2) * mu;
-            A.sxx[o] = l2m * exx + lc * eyy - al * p;
-            A.syy[o] = lc * exx + l2m * eyy - al * p;
+            const T l2m = m.z + T(2) * m.w;
+            sxx = l2m * st.x + m.z * st.y - m.x * p;
+            syy = m.z * st.x + l2m * st.y - m.x * p;
         }
+        I[o] = Vec4<T>{p, sxx, syy, T(0)};
     }
-    const bool r2 = x >= 2 && x < A.nx && z >= 1 && z < A.nz - 1;
-    if (r2) {
-        T ddx = d4(A.vz[o], A.vz[o - 1], A.vz[o + 1], A.vz[o - 2], A.rdx);
-        T ddz = d4(A.vx[o + P], A.vx[o], A.vx[o + 2 * P], A.vx[o - P], A.rdz);
+    if (x >= 2 && x < A.nx && z >= 1 && z < A.nz - 1) {
+        T ddx = d4(V[o].y, V[o - 1].y, V[o + 1].y, V[o - 2].y, A.rdx);
+        T ddz = d4(V[o + P].x, V[o].x, V[o + 2 * P].x, V[o - P].x, A.rdz);
         if (sx >= 0) {
-            const size_t k = (size_t)z * A.wxs + sx;
-            ddx = cpml(ddx, A.psx[PX_DVY] + k, A.ax[x], A.bx[x], A.Kx[x]);
+            const long long k = (long long)z * A.wxs + sx;
+            ddx = cpml(ddx, psi_x(A, b, k) + PX_DVY, psi_x(A, b, k) + PX_DVY, true, tab[TB_AX * tn + x], tab[TB_BX * tn + x], tab[TB_RKX * tn + x]);
         }
         if (sz >= 0) {
-            // memory variable on the half grid, division by the full-grid K_z as written in
-            // the reference (seismic_CPML_2D_poroelastic_withCUDA.cu:224-226)
-            const size_t k = (size_t)sz * P + x;
-            ddz = cpml(ddz, A.psz[PZ_DVX] + k, A.azh[z], A.bzh[z], A.Kz[z]);
+            // memory variable on the half grid, division by the full-grid K_z as written in the
+            // reference (seismic_CPML_2D_poroelastic_withCUDA.cu:224-226)
+            const long long k = (long long)sz * P + x;
+            ddz = cpml(ddz, psi_z(A, b, k) + PZ_DVX, psi_z(A, b, k) + PZ_DVX, true, tab[TB_AZH * tn + z], tab[TB_BZH * tn + z], tab[TB_RKZ * tn + z]);
         }
-        A.sxy[o] = A.sxy[o] + A.mu[o + P] * (ddx + ddz) * A.dt;
+        st.w = st.w + MS[o + P].w * (ddx + ddz) * A.dt;
     }
+    S[o] = st;
 }
 
-// Velocity pass.  vx x in [2,NX), z in [2,NZ) (:289-291); vz x in [1,NX-1), z in [1,NZ-1)
-// (:326-328) with the material of point (x, z+1) (:330-333).
 template <class T>
 __global__ void __launch_bounds__(256) velocity_pass(StepArgs<T> A)
 {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int z = blockIdx.y * blockDim.y + threadIdx.y;
     if (x > A.nx || z > A.nz) return;
-    const size_t P = A.pitch;
-    const size_t o = (size_t)z * P + x;
+    const long long P = A.pitch;
+    const long long o = (long long)z * P + x;
+    Vec4<T> *V = group_w(A, group_v(A.in));
+    const Vec4<T> *S = group_c(A, group_s(A.in));
+    const Vec4<T> *I = group_c(A, G_I);
+    const Vec4<T> *MV = group_c(A, G_MV);
     const int sx = A.xmap[x];
     const int sz = A.zmap[z];
+    const int b = A.in;
+    const T *tab = A.tab;
+    const int tn = A.tn;
+    Vec4<T> v = V[o];
 
     if (x >= 2 && x < A.nx && z >= 2 && z < A.nz) {
-        T e1 = d4(A.sxx[o], A.sxx[o - 1], A.sxx[o + 1], A.sxx[o - 2], A.rdx);
-        T e2 = d4(A.p[o], A.p[o - 1], A.p[o + 1], A.p[o - 2], A.rdx);
-        T e3 = d4(A.sxy[o], A.sxy[o - P], A.sxy[o + P], A.sxy[o - 2 * P], A.rdz);
+        T e1 = d4(I[o].y, I[o - 1].y, I[o + 1].y, I[o - 2].y, A.rdx);
+        T e2 = d4(I[o].x, I[o - 1].x, I[o + 1].x, I[o - 2].x, A.rdx);
+        T e3 = d4(S[o].w, S[o - P].w, S[o + P].w, S[o - 2 * P].w, A.rdz);
         if (sx >= 0) {
-            const size_t k = (size_t)z * A.wxs + sx;
-            e1 = cpml(e1, A.psx[PX_DSXX] + k, A.ax[x], A.bx[x], A.Kx[x]);
-            e2 = cpml(e2, A.psx[PX_DPX] + k, A.ax[x], A.bx[x], A.Kx[x]);
+            const long long k = (long long)z * A.wxs + sx;
+            e1 = cpml(e1, psi_x(A, b, k) + PX_DSXX, psi_x(A, b, k) + PX_DSXX, true, tab[TB_AX * tn + x], tab[TB_BX * tn + x], tab[TB_RKX * tn + x]);
+            e2 = cpml(e2, psi_x(A, b, k) + PX_DPX, psi_x(A, b, k) + PX_DPX, true, tab[TB_AX * tn + x], tab[TB_BX * tn + x], tab[TB_RKX * tn + x]);
         }
         if (sz >= 0) {
-            const size_t k = (size_t)sz * P + x;
-            e3 = cpml(e3, A.psz[PZ_DSXY] + k, A.az[z], A.bz[z], A.Kz[z]);
+            const long long k = (long long)sz * P + x;
+            e3 = cpml(e3, psi_z(A, b, k) + PZ_DSXY, psi_z(A, b, k) + PZ_DSXY, true, tab[TB_AZ * tn + z], tab[TB_BZ * tn + z], tab[TB_RKZ * tn + z]);
         }
-        biot_velocity(A.vx[o], A.wx[o], e1 + e3, e2, A.rho[o], A.rhof[o], A.m[o], A.b[o], A.inv_dt);
+        const Vec4<T> m = MV[o];
+        biot_velocity(v.x, v.z, e1 + e3, e2, m.x, m.y, m.z, m.w, A.inv_dt);
     }
     if (x >= 1 && x < A.nx - 1 && z >= 1 && z < A.nz - 1) {
-        const size_t t = o + P;
-        T e1 = d4(A.sxy[o + 1], A.sxy[o], A.sxy[o + 2], A.sxy[o - 1], A.rdx);
-        T e2 = d4(A.syy[o + P], A.syy[o], A.syy[o + 2 * P], A.syy[o - P], A.rdz);
-        T e3 = d4(A.p[o + P], A.p[o], A.p[o + 2 * P], A.p[o - P], A.rdz);
+        T e1 = d4(S[o + 1].w, S[o].w, S[o + 2].w, S[o - 1].w, A.rdx);
+        T e2 = d4(I[o + P].z, I[o].z, I[o + 2 * P].z, I[o - P].z, A.rdz);
+        T e3 = d4(I[o + P].x, I[o].x, I[o + 2 * P].x, I[o - P].x, A.rdz);
         if (sx >= 0) {
-            const size_t k = (size_t)z * A.wxs + sx;
-            e1 = cpml(e1, A.psx[PX_DSXY] + k, A.axh[x], A.bxh[x], A.Kxh[x]);
+            const long long k = (long long)z * A.wxs + sx;
+            e1 = cpml(e1, psi_x(A, b, k) + PX_DSXY, psi_x(A, b, k) + PX_DSXY, true, tab[TB_AXH * tn + x], tab[TB_BXH * tn + x], tab[TB_RKXH * tn + x]);
         }
         if (sz >= 0) {
-            const size_t k = (size_t)sz * P + x;
-            e2 = cpml(e2, A.psz[PZ_DSYY] + k, A.azh[z], A.bzh[z], A.Kzh[z]);
-            e3 = cpml(e3, A.psz[PZ_DPZ] + k, A.azh[z], A.bzh[z], A.Kzh[z]);
+            const long long k = (long long)sz * P + x;
+            e2 = cpml(e2, psi_z(A, b, k) + PZ_DSYY, psi_z(A, b, k) + PZ_DSYY, true, tab[TB_AZH * tn + z], tab[TB_BZH * tn + z], tab[TB_RKZH * tn + z]);
+            e3 = cpml(e3, psi_z(A, b, k) + PZ_DPZ, psi_z(A, b, k) + PZ_DPZ, true, tab[TB_AZH * tn + z], tab[TB_BZH * tn + z], tab[TB_RKZH * tn + z]);
         }
         T div = e1 + e2;
         if (A.src_kind == POE2D_SRC_FORCE_Y_RICKER && x == A.src_x && z == A.src_z) div += A.src_val;
-        biot_velocity(A.vz[o], A.wz[o], div, e3, A.rho[t], A.rhof[t], A.m[t], A.b[t], A.inv_dt);
+        const Vec4<T> m = MV[o + P];  // material of point (x, z+1) (reference :330-333)
+        biot_velocity(v.y, v.w, div, e3, m.x, m.y, m.z, m.w, A.inv_dt);
     }
+    V[o] = v;
 }
-
-template __global__ void stress_pass<float>(StepArgs<float>);
-template __global__ void stress_pass<double>(StepArgs<double>);
-template __global__ void velocity_pass<float>(StepArgs<float>);
-template __global__ void velocity_pass<double>(StepArgs<double>);
 
 template <class T>
 void launch_twopass(const StepArgs<T> &A, hipStream_t stream, int phase)

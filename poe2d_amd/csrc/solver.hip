@@ -1,18 +1,25 @@
 // Solver handle behind the C ABI of include/poe2d.h: device memory, the time loop, gather,
 // snapshots, checkpoint/restore and the stability check.
 //
-// Device layout.  Every field is SoA with row pitch P (a multiple of 64 elements) and zero pad
-// rows/columns around the (nz+1) x (nx+1) grid: 4 pad rows above and below, 4 pad columns on the
-// left and at least one on the right (the reference's stress1 reads column nx+1 at x = nx-1, which
-// in its flat array is column 0 of the next row, always zero; the pad reproduces that).  Pointers
-// handed to kernels address point (0, 0).  The wavefield state (vx, vy, wx, wy, e_xx, e_yy, zeta,
-// sxy) and the C-PML memory variables are double-buffered: a step reads buffer `cur` and writes
-// buffer 1-cur.
+// Device memory of a handle (field layout described in step_common.h):
+//   arena   NGROUPS groups of 4-vectors (coefficients MS, MV; state V, S for buffers 0 and 1;
+//           two-pass intermediates I), each (nz+1+2*kPadRows) rows x pitch points, zero pads;
+//   mid/pal when the grid holds at most kMaxPalette distinct coefficient tuples: a one-byte
+//           material index per point (group layout, pads select entry 0) and the palette of
+//           (MS, MV) entries, which the single-pass kernel reads instead of the MS/MV groups;
+//   psi     C-PML memory variables: 6 x-strip and 6 z-strip arrays per state buffer;
+//   tables  the 12 1-D C-PML profiles;
+//   maps    strip index of each column and row.
+// A single-pass step reads state buffer `cur` and writes buffer 1-cur.  The reference's stress1
+// reads column nx+1 at x = nx-1, which in its flat array is column 0 of the next row and always
+// zero; the zero pad column nx+1 gives the same value.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <new>
 #include <string>
 #include <vector>
@@ -27,7 +34,6 @@ namespace {
 
 const float kRefPiF = 3.1415926535897932f;  // the reference's float PI (seismic_CPML_2d_poroelastic.cpp:9)
 const double kStabilityThreshold = 1.0e25;  // STABILITY_THRESHOLD of the reference (declared, unused there)
-const int kPadRows = 4, kPadCols = 4;
 
 #define HIP_TRY(expr)                                                                              \
     do {                                                                                           \
@@ -36,12 +42,8 @@ const int kPadRows = 4, kPadCols = 4;
             return fail(POE2D_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));           \
     } while (0)
 
-enum { ST_VX, ST_VZ, ST_WX, ST_WZ, ST_EXX, ST_EYY, ST_ZETA, ST_SXY, NSTATE };
-enum { IN_P, IN_SXX, IN_SYY, NINTER };
-enum { MT_RHO, MT_RHOF, MT_M, MT_MU, MT_LC, MT_M_BIOT, MT_ALPHA, MT_B, NMAT };
-enum { SCHED_FUSED = 0, SCHED_TWOPASS = 1 };
-
-size_t real_size(int precision) { return precision == POE2D_F32 ? 4 : 8; }
+// single-pass (palette when the grid has one), two-pass, single-pass always on the MS/MV groups
+enum { SCHED_FUSED = 0, SCHED_TWOPASS = 1, SCHED_FUSED_GROUPS = 2 };
 
 }  // namespace
 
@@ -52,22 +54,22 @@ struct poe2d_solver {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    size_t pitch = 0, alloc_rows = 0;  // padded row length and allocated rows (nz+1+2*pad)
+    size_t pitch = 0, alloc_rows = 0;  // points per row, rows per group (with pads)
     int wxs = 0, wzs = 0;              // x-strip columns, z-strip rows
+    int tn = 0;                        // entries per profile table
     double cp_max = 0;
     int cur = 0;
-    void *st[2][NSTATE] = {};
-    void *inter[NINTER] = {};
-    void *mat[NMAT] = {};
-    void *psx[2][NPSI_X] = {};
-    void *psz[2][NPSI_Z] = {};
-    void *tabx[6] = {};
-    void *tabz[6] = {};
-    std::vector<double> htabx[6], htabz[6];
+    void *arena = nullptr;
+    void *psi = nullptr;
+    void *tab = nullptr;
     int *xmap = nullptr, *zmap = nullptr;
+    unsigned char *mid = nullptr;  // material index per point (palette mode)
+    void *pal = nullptr;           // 2 * kMaxPalette Vec4 entries
+    int npal = 0;                  // palette entries, 0 when the grid has more than kMaxPalette
     float *red = nullptr;  // reduction scratch: [0] result, [1..] per-block partials
     int red_blocks = 1024;
-    void *dense = nullptr;  // (nz+1) x (nx+1) scratch for derived-field snapshots, made on first use
+    void *dense = nullptr;  // (nz+1) x (nx+1) scratch for snapshots, made on first use
+    std::vector<double> htab[NTABLES];
     // schedule
     int schedule = SCHED_FUSED;
     int block_w = 256, tile_rows = 64;
@@ -86,13 +88,14 @@ struct poe2d_solver {
     double last_ms = 0;
     int last_steps = 0;
 
-    size_t field_elems() const { return pitch * alloc_rows; }
-    size_t field_bytes() const { return field_elems() * rs; }
-    size_t origin() const { return (size_t)kPadRows * pitch + kPadCols; }  // elements to point (0,0)
-    size_t psx_bytes() const { return (size_t)std::max(wxs, 1) * (grid.nz + 1) * rs; }
-    size_t psz_bytes() const { return (size_t)std::max(wzs, 1) * pitch * rs; }
+    size_t group_points() const { return pitch * alloc_rows; }
+    size_t group_bytes() const { return group_points() * 4 * rs; }
+    size_t origin() const { return (size_t)kPadRows * pitch + kPadCols; }
+    size_t pxn() const { return (size_t)std::max(wxs, 1) * (grid.nz + 1); }
+    size_t pzn() const { return (size_t)std::max(wzs, 1) * pitch; }
+    size_t psi_half_elems() const { return NPSI_X * pxn() + NPSI_Z * pzn(); }  // one buffer's worth
     size_t trace_bytes() const { return (size_t)nsamp * ncomp * nrec * rs; }
-    template <class T> T *at0(void *p) const { return (T *)p + origin(); }
+    unsigned char *group_ptr(int g) const { return (unsigned char *)arena + (size_t)g * group_bytes(); }
 
     double source_value(int n) const
     {
@@ -112,17 +115,12 @@ void free_all(poe2d_solver *s)
         if (p) (void)hipFree(p);
         p = nullptr;
     };
-    for (auto &b : s->st)
-        for (auto &p : b) f(p);
-    for (auto &p : s->inter) f(p);
-    for (auto &p : s->mat) f(p);
-    for (auto &b : s->psx)
-        for (auto &p : b) f(p);
-    for (auto &b : s->psz)
-        for (auto &p : b) f(p);
-    for (auto &p : s->tabx) f(p);
-    for (auto &p : s->tabz) f(p);
-    void *q = s->xmap; f(q); s->xmap = nullptr;
+    f(s->arena);
+    f(s->psi);
+    f(s->tab);
+    f(s->pal);
+    void *q = s->mid; f(q); s->mid = nullptr;
+    q = s->xmap; f(q); s->xmap = nullptr;
     q = s->zmap; f(q); s->zmap = nullptr;
     q = s->red; f(q); s->red = nullptr;
     q = s->rix; f(q); s->rix = nullptr;
@@ -148,52 +146,104 @@ poe2d_status alloc_zero(poe2d_solver *s, void **p, size_t bytes)
     return POE2D_OK;
 }
 
+// Coefficients into the MS (alpha, M, lambda_c, mu) and MV (rho, rho_f, m, b) groups, and, when
+// the grid holds at most kMaxPalette distinct tuples (bitwise, after rounding to T), into the
+// palette and the one-byte material index.
 template <class T>
 poe2d_status upload_coeffs(poe2d_solver *s, const poe2d_coeffs *c)
 {
     const poe2d_grid &g = s->grid;
     const size_t cpitch = c->pitch ? c->pitch : (size_t)g.nx + 1;
-    const void *srcs[NMAT] = {c->rho, c->rhof, c->m, c->mu, c->lambdac, c->M, c->alpha, c->b};
-    auto at = [&](int k, size_t i) {
-        return c->dtype == POE2D_F64 ? ((const double *)srcs[k])[i] : (double)((const float *)srcs[k])[i];
+    const void *ms_src[4] = {c->alpha, c->M, c->lambdac, c->mu};
+    const void *mv_src[4] = {c->rho, c->rhof, c->m, c->b};
+    auto at = [&](const void *a, size_t i) {
+        return c->dtype == POE2D_F64 ? ((const double *)a)[i] : (double)((const float *)a)[i];
     };
-    // the velocity update divides by det = rho m - rho_f^2 (the 2x2 mass matrix) and needs b >= 0
-    for (int z = 0; z <= g.nz; ++z)
-        for (int x = 0; x <= g.nx; ++x) {
-            const size_t i = (size_t)z * cpitch + x;
-            const double det = at(MT_RHO, i) * at(MT_M, i) - at(MT_RHOF, i) * at(MT_RHOF, i);
-            if (!(det > 0) || !(at(MT_B, i) >= 0) || !(at(MT_RHO, i) > 0))
-                return fail(POE2D_E_INVALID, "create: coefficients need rho > 0, rho m > rho_f^2 and b >= 0 (point " +
-                                                 std::to_string(x) + ", " + std::to_string(z) + ")");
-        }
-    std::vector<T> host(s->field_elems(), T(0));
-    for (int k = 0; k < NMAT; ++k) {
+    std::vector<Vec4<T>> host(s->group_points(), Vec4<T>{T(0), T(0), T(0), T(0)});
+    for (int grp = 0; grp < 2; ++grp) {
+        const void **src = grp == 0 ? ms_src : mv_src;
         for (int z = 0; z <= g.nz; ++z)
             for (int x = 0; x <= g.nx; ++x) {
-                const double v = at(k, (size_t)z * cpitch + x);
-                if (!std::isfinite(v)) return fail(POE2D_E_INVALID, "create: non-finite coefficient");
-                host[s->origin() + (size_t)z * s->pitch + x] = (T)v;
+                const size_t i = (size_t)z * cpitch + x;
+                double v[4];
+                for (int k = 0; k < 4; ++k) {
+                    v[k] = at(src[k], i);
+                    if (!std::isfinite(v[k])) return fail(POE2D_E_INVALID, "create: non-finite coefficient");
+                }
+                if (grp == 1) {
+                    // the velocity update divides by rho m - rho_f^2 (the 2x2 mass matrix)
+                    const double det = v[0] * v[2] - v[1] * v[1];
+                    if (!(det > 0) || !(v[3] >= 0) || !(v[0] > 0))
+                        return fail(POE2D_E_INVALID, "create: coefficients need rho > 0, rho m > rho_f^2 and "
+                                                     "b >= 0 (point " + std::to_string(x) + ", " + std::to_string(z) + ")");
+                }
+                host[s->origin() + (size_t)z * s->pitch + x] = Vec4<T>{(T)v[0], (T)v[1], (T)v[2], (T)v[3]};
             }
-        HIP_TRY(hipMemcpy(s->mat[k], host.data(), s->field_bytes(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(s->group_ptr(grp == 0 ? G_MS : G_MV), host.data(), s->group_bytes(), hipMemcpyHostToDevice));
     }
+
+    // palette: distinct 8-tuples in order of first appearance
+    using Key = std::array<T, 8>;
+    auto less = [](const Key &a, const Key &b) { return std::memcmp(a.data(), b.data(), sizeof(Key)) < 0; };
+    std::map<Key, int, decltype(less)> seen(less);
+    std::vector<Key> pal;
+    std::vector<unsigned char> idx(s->group_points(), 0);
+    Key last{};
+    int last_id = -1;
+    for (int z = 0; z <= g.nz && pal.size() <= (size_t)kMaxPalette; ++z)
+        for (int x = 0; x <= g.nx; ++x) {
+            const size_t i = (size_t)z * cpitch + x;
+            Key k;
+            for (int q = 0; q < 4; ++q) {
+                k[q] = (T)at(ms_src[q], i);
+                k[4 + q] = (T)at(mv_src[q], i);
+            }
+            if (last_id < 0 || std::memcmp(k.data(), last.data(), sizeof(Key)) != 0) {
+                auto it = seen.find(k);
+                if (it == seen.end()) {
+                    it = seen.emplace(k, (int)pal.size()).first;
+                    pal.push_back(k);
+                    if (pal.size() > (size_t)kMaxPalette) break;
+                }
+                last = k;
+                last_id = it->second;
+            }
+            idx[s->origin() + (size_t)z * s->pitch + x] = (unsigned char)last_id;
+        }
+    if (pal.size() > (size_t)kMaxPalette) return POE2D_OK;  // no palette: the groups serve every schedule
+    std::vector<Vec4<T>> hp(2 * kMaxPalette, Vec4<T>{T(0), T(0), T(0), T(0)});
+    for (size_t e = 0; e < pal.size(); ++e) {
+        hp[e] = Vec4<T>{pal[e][0], pal[e][1], pal[e][2], pal[e][3]};
+        hp[kMaxPalette + e] = Vec4<T>{pal[e][4], pal[e][5], pal[e][6], pal[e][7]};
+    }
+    void *q = nullptr;
+    poe2d_status st = alloc_zero(s, &q, s->group_points());
+    if (st) return st;
+    s->mid = (unsigned char *)q;
+    if ((st = alloc_zero(s, &s->pal, hp.size() * sizeof(Vec4<T>)))) return st;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipMemcpy(s->mid, idx.data(), idx.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s->pal, hp.data(), hp.size() * sizeof(Vec4<T>), hipMemcpyHostToDevice));
+    s->npal = (int)pal.size();
     return POE2D_OK;
 }
 
 template <class T>
 poe2d_status upload_tables(poe2d_solver *s)
 {
-    std::vector<T> h;
-    for (int k = 0; k < 6; ++k) {
-        h.assign(s->htabx[k].begin(), s->htabx[k].end());
-        HIP_TRY(hipMemcpy(s->tabx[k], h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
-        h.assign(s->htabz[k].begin(), s->htabz[k].end());
-        HIP_TRY(hipMemcpy(s->tabz[k], h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    // the K tables go to the device as 1/K, so the kernels multiply
+    std::vector<T> h((size_t)NTABLES * s->tn, T(0));
+    for (int k = 0; k < NTABLES; ++k) {
+        const bool recip = k == TB_RKX || k == TB_RKXH || k == TB_RKZ || k == TB_RKZH;
+        for (size_t i = 0; i < s->htab[k].size(); ++i)
+            h[(size_t)k * s->tn + i] = (T)(recip ? 1.0 / s->htab[k][i] : s->htab[k][i]);
     }
+    HIP_TRY(hipMemcpy(s->tab, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
     return POE2D_OK;
 }
 
 // Strip map of one axis: index into the strip arrays where any profile deviates from the
-// no-absorption values (a = 0, K = 1), else -1.
+// no-absorption values (a = 0, K = 1), else -1.  t = {a, b, K, a_half, b_half, K_half}.
 std::vector<int> strip_map(const std::vector<double> *t, int n, int *count)
 {
     std::vector<int> map(n, -1);
@@ -207,155 +257,73 @@ std::vector<int> strip_map(const std::vector<double> *t, int n, int *count)
 }
 
 template <class T>
-void set_tables(poe2d_solver *s, const T **x6, const T **z6)
-{
-    for (int k = 0; k < 6; ++k) {
-        x6[k] = (const T *)s->tabx[k];
-        z6[k] = (const T *)s->tabz[k];
-    }
-}
-
-template <class T>
-StepArgs<T> twopass_args(poe2d_solver *s)
+StepArgs<T> step_args(poe2d_solver *s, int in, int out)
 {
     StepArgs<T> A{};
     const poe2d_grid &g = s->grid;
-    const int b = s->cur;
     A.nx = g.nx;
     A.nz = g.nz;
     A.pitch = (int)s->pitch;
     A.wxs = s->wxs;
+    A.origin = (long long)s->origin();
+    A.group_stride = (long long)s->group_points();
+    A.pxn = (long long)s->pxn();
+    A.pzn = (long long)s->pzn();
+    A.tn = s->tn;
+    A.in = in;
+    A.out = out;
     A.rdx = T(1.0 / (24.0 * g.dx));
     A.rdz = T(1.0 / (24.0 * g.dz));
     A.dt = T(g.dt);
     A.inv_dt = T(1.0 / g.dt);
-    T **st[NSTATE] = {&A.vx, &A.vz, &A.wx, &A.wz, &A.exx, &A.eyy, &A.zeta, &A.sxy};
-    for (int k = 0; k < NSTATE; ++k) *st[k] = s->at0<T>(s->st[b][k]);
-    A.p = s->at0<T>(s->inter[IN_P]);
-    A.sxx = s->at0<T>(s->inter[IN_SXX]);
-    A.syy = s->at0<T>(s->inter[IN_SYY]);
-    const T **mt[NMAT] = {&A.rho, &A.rhof, &A.m, &A.mu, &A.lc, &A.M, &A.al, &A.b};
-    for (int k = 0; k < NMAT; ++k) *mt[k] = s->at0<T>(s->mat[k]);
-    for (int k = 0; k < NPSI_X; ++k) A.psx[k] = (T *)s->psx[b][k];
-    for (int k = 0; k < NPSI_Z; ++k) A.psz[k] = (T *)s->psz[b][k];
-    const T *x6[6], *z6[6];
-    set_tables<T>(s, x6, z6);
-    A.ax = x6[0]; A.bx = x6[1]; A.Kx = x6[2]; A.axh = x6[3]; A.bxh = x6[4]; A.Kxh = x6[5];
-    A.az = z6[0]; A.bz = z6[1]; A.Kz = z6[2]; A.azh = z6[3]; A.bzh = z6[4]; A.Kzh = z6[5];
+    A.grp = (Vec4<T> *)s->arena;
+    A.psi = (T *)s->psi;
+    A.tab = (const T *)s->tab;
     A.xmap = s->xmap;
     A.zmap = s->zmap;
+    A.mid = s->mid;
+    A.pal = (const Vec4<T> *)s->pal;
+    A.npal = s->npal;
     A.src_kind = s->have_src ? s->src.kind : -1;
     A.src_x = s->have_src ? s->src.ix : -1;
     A.src_z = s->have_src ? s->src.iz : -1;
     A.src_val = T(0);
-    return A;
-}
-
-template <class T>
-FusedArgs<T> fused_args(poe2d_solver *s, int in, int out)
-{
-    FusedArgs<T> A{};
-    const poe2d_grid &g = s->grid;
-    A.nx = g.nx;
-    A.nz = g.nz;
-    A.pitch = (int)s->pitch;
-    A.wxs = s->wxs;
     const int W = fused_tile_width(s->block_w);
     A.th = s->tile_rows;
     A.ntx = (g.nx + 1 + W - 1) / W;
     const int ntz = (g.nz + 1 + A.th - 1) / A.th;
     A.ntiles = A.ntx * ntz;
     A.per_xcd = (A.ntiles + 7) / 8;
-    A.rdx = T(1.0 / (24.0 * g.dx));
-    A.rdz = T(1.0 / (24.0 * g.dz));
-    A.dt = T(g.dt);
-    A.inv_dt = T(1.0 / g.dt);
-    const T **so[NSTATE] = {&A.vx_o, &A.vz_o, &A.wx_o, &A.wz_o, &A.exx_o, &A.eyy_o, &A.zeta_o, &A.sxy_o};
-    T **sn[NSTATE] = {&A.vx_n, &A.vz_n, &A.wx_n, &A.wz_n, &A.exx_n, &A.eyy_n, &A.zeta_n, &A.sxy_n};
-    for (int k = 0; k < NSTATE; ++k) {
-        *so[k] = s->at0<T>(s->st[in][k]);
-        *sn[k] = s->at0<T>(s->st[out][k]);
-    }
-    const T **mt[NMAT] = {&A.rho, &A.rhof, &A.m, &A.mu, &A.lc, &A.M, &A.al, &A.b};
-    for (int k = 0; k < NMAT; ++k) *mt[k] = s->at0<T>(s->mat[k]);
-    for (int k = 0; k < NPSI_X; ++k) {
-        A.psx_o[k] = (const T *)s->psx[in][k];
-        A.psx_n[k] = (T *)s->psx[out][k];
-    }
-    for (int k = 0; k < NPSI_Z; ++k) {
-        A.psz_o[k] = (const T *)s->psz[in][k];
-        A.psz_n[k] = (T *)s->psz[out][k];
-    }
-    const T *x6[6], *z6[6];
-    set_tables<T>(s, x6, z6);
-    A.ax = x6[0]; A.bx = x6[1]; A.Kx = x6[2]; A.axh = x6[3]; A.bxh = x6[4]; A.Kxh = x6[5];
-    A.az = z6[0]; A.bz = z6[1]; A.Kz = z6[2]; A.azh = z6[3]; A.bzh = z6[4]; A.Kzh = z6[5];
-    A.xmap = s->xmap;
-    A.zmap = s->zmap;
-    A.src_kind = s->have_src ? s->src.kind : -1;
-    A.src_x = s->have_src ? s->src.ix : -1;
-    A.src_z = s->have_src ? s->src.iz : -1;
-    A.src_val = T(0);
     return A;
-}
-
-// Gather/derived-field arguments: velocities from buffer `vbuf`, strains from buffer `sbuf`.
-template <class T>
-GatherArgs<T> gather_args(poe2d_solver *s, int vbuf, int sbuf, T src_val)
-{
-    GatherArgs<T> G{};
-    G.nx = s->grid.nx;
-    G.nz = s->grid.nz;
-    G.pitch = (int)s->pitch;
-    G.nrec = s->nrec;
-    G.mask = s->mask;
-    G.rix = s->rix;
-    G.riz = s->riz;
-    G.vx = s->at0<T>(s->st[vbuf][ST_VX]);
-    G.vz = s->at0<T>(s->st[vbuf][ST_VZ]);
-    G.exx = s->at0<T>(s->st[sbuf][ST_EXX]);
-    G.eyy = s->at0<T>(s->st[sbuf][ST_EYY]);
-    G.zeta = s->at0<T>(s->st[sbuf][ST_ZETA]);
-    G.al = s->at0<T>(s->mat[MT_ALPHA]);
-    G.M = s->at0<T>(s->mat[MT_M_BIOT]);
-    G.src_kind = s->have_src ? s->src.kind : -1;
-    G.src_x = s->have_src ? s->src.ix : -1;
-    G.src_z = s->have_src ? s->src.iz : -1;
-    G.src_val = src_val;
-    G.out = nullptr;
-    return G;
 }
 
 template <class T>
 poe2d_status run_steps(poe2d_solver *s, int nsteps)
 {
+    Receivers R{s->nrec, s->mask, s->rix, s->riz};
     HIP_TRY(hipEventRecord(s->ev0, s->stream));
     if (s->schedule == SCHED_TWOPASS) {
-        StepArgs<T> A = twopass_args<T>(s);
+        StepArgs<T> A = step_args<T>(s, s->cur, s->cur);
         for (int k = 0; k < nsteps; ++k) {
             const int n = s->step + k;
             A.src_val = (T)s->source_value(n);
             launch_twopass<T>(A, s->stream, 0);
-            if (s->nrec > 0 && n % s->dec == 0 && n / s->dec < s->nsamp) {
+            if (s->nrec > 0 && n % s->dec == 0 && n / s->dec < s->nsamp)
                 // in place: v is still v^n, strains are already those of step n
-                GatherArgs<T> G = gather_args<T>(s, s->cur, s->cur, A.src_val);
-                G.out = (T *)s->traces + (size_t)(n / s->dec) * s->ncomp * s->nrec;
-                launch_gather<T>(G, s->stream);
-            }
+                launch_gather<T>(A, s->cur, s->cur, R, (T *)s->traces + (size_t)(n / s->dec) * s->ncomp * s->nrec,
+                                 s->stream);
             launch_twopass<T>(A, s->stream, 1);
         }
     } else {
-        FusedArgs<T> F[2] = {fused_args<T>(s, 0, 1), fused_args<T>(s, 1, 0)};
+        StepArgs<T> F[2] = {step_args<T>(s, 0, 1), step_args<T>(s, 1, 0)};
         for (int k = 0; k < nsteps; ++k) {
             const int n = s->step + k;
-            FusedArgs<T> &A = F[s->cur];
+            StepArgs<T> &A = F[s->cur];
             A.src_val = (T)s->source_value(n);
-            HIP_TRY(launch_fused<T>(A, s->block_w, s->stream));
-            if (s->nrec > 0 && n % s->dec == 0 && n / s->dec < s->nsamp) {
-                GatherArgs<T> G = gather_args<T>(s, s->cur, 1 - s->cur, A.src_val);
-                G.out = (T *)s->traces + (size_t)(n / s->dec) * s->ncomp * s->nrec;
-                launch_gather<T>(G, s->stream);
-            }
+            HIP_TRY(launch_fused<T>(A, s->block_w, s->schedule == SCHED_FUSED && s->npal > 0, s->stream));
+            if (s->nrec > 0 && n % s->dec == 0 && n / s->dec < s->nsamp)
+                launch_gather<T>(A, s->cur, 1 - s->cur, R, (T *)s->traces + (size_t)(n / s->dec) * s->ncomp * s->nrec,
+                                 s->stream);
             s->cur = 1 - s->cur;
         }
     }
@@ -367,9 +335,8 @@ poe2d_status run_steps(poe2d_solver *s, int nsteps)
     s->last_ms = ms;
     s->last_steps = nsteps;
     s->step += nsteps;
-    // stability check: largest |v| over the grid (pads are zero)
-    launch_absmax<T>((const T *)s->st[s->cur][ST_VX], (const T *)s->st[s->cur][ST_VZ], s->field_elems(),
-                     s->red + 1, s->red_blocks, s->red, s->stream);
+    // stability check: largest |v| over the grid
+    launch_absmax<T>(step_args<T>(s, s->cur, s->cur), s->cur, s->red + 1, s->red_blocks, s->red, s->stream);
     float vmax = 0.f;
     HIP_TRY(hipMemcpyAsync(&vmax, s->red, sizeof(float), hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
@@ -379,18 +346,27 @@ poe2d_status run_steps(poe2d_solver *s, int nsteps)
     return POE2D_OK;
 }
 
+// One field of the current state (or p / sxx / syy recomputed from it) to the host.
 template <class T>
-poe2d_status derived_snapshot(poe2d_solver *s, int what, void *out)
+poe2d_status field_to_host(poe2d_solver *s, int field, void *out)
 {
-    const double sv = s->step > 0 ? s->source_value(s->step - 1) : 0.0;
-    GatherArgs<T> G = gather_args<T>(s, s->cur, s->cur, (T)sv);
     const size_t bytes = (size_t)(s->grid.nx + 1) * (s->grid.nz + 1) * sizeof(T);
     if (!s->dense) {
         poe2d_status st = alloc_zero(s, &s->dense, bytes);
         if (st) return st;
     }
+    StepArgs<T> A = step_args<T>(s, s->cur, s->cur);
     T *dense = (T *)s->dense;
-    launch_derived<T>(G, s->at0<T>(s->mat[MT_LC]), s->at0<T>(s->mat[MT_MU]), what, dense, s->stream);
+    if (field >= POE2D_FIELD_P) {
+        A.src_val = (T)(s->step > 0 ? s->source_value(s->step - 1) : 0.0);
+        launch_derived<T>(A, s->cur, field - POE2D_FIELD_P, dense, s->stream);
+    } else {
+        // V = (vx, vy, wx, wy), S = (e_xx, e_yy, zeta, sxy)
+        static const int grp_of[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+        static const int comp_of[8] = {0, 1, 2, 3, 0, 1, 2, 3};
+        const int g = grp_of[field] == 0 ? group_v(s->cur) : group_s(s->cur);
+        launch_extract<T>(A, g, comp_of[field], dense, s->stream);
+    }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(s->stream));
     HIP_TRY(hipMemcpy(out, dense, bytes, hipMemcpyDeviceToHost));
@@ -410,12 +386,16 @@ struct CkptHeader {
     unsigned long long pitch;
 };
 
+// The buffers that make up the state of step `step`: V and S groups of buffer b, its memory
+// variables, and the traces.
 template <class F>
 void for_each_buffer(poe2d_solver *s, int b, F f)
 {
-    for (auto p : s->st[b]) f(p, s->field_bytes());
-    for (auto p : s->psx[b]) f(p, s->psx_bytes());
-    for (auto p : s->psz[b]) f(p, s->psz_bytes());
+    f(s->group_ptr(group_v(b)), s->group_bytes());
+    f(s->group_ptr(group_s(b)), s->group_bytes());
+    unsigned char *psi = (unsigned char *)s->psi;
+    f(psi + (size_t)b * NPSI_X * s->pxn() * s->rs, NPSI_X * s->pxn() * s->rs);
+    f(psi + 2 * NPSI_X * s->pxn() * s->rs + (size_t)b * NPSI_Z * s->pzn() * s->rs, NPSI_Z * s->pzn() * s->rs);
     if (s->traces) f(s->traces, s->trace_bytes());
 }
 
@@ -462,7 +442,7 @@ poe2d_status poe2d_create(const poe2d_grid *g, const poe2d_coeffs *c, poe2d_solv
     if (!s) return fail(POE2D_E_NOMEM, "create: out of host memory");
     s->grid = *g;
     s->precision = g->precision;
-    s->rs = real_size(g->precision);
+    s->rs = g->precision == POE2D_F32 ? 4 : 8;
     auto bail = [&](poe2d_status st) {
         free_all(s);
         delete s;
@@ -497,7 +477,7 @@ poe2d_status poe2d_create(const poe2d_grid *g, const poe2d_coeffs *c, poe2d_solv
                                          : 3.14159265358979323846 * g->f0;
     for (int axis = 0; axis < 2; ++axis) {
         const int n = (axis == 0 ? g->nx : g->nz) + 1;
-        std::vector<double> *t = axis == 0 ? s->htabx : s->htabz;
+        std::vector<double> *t = s->htab + (axis == 0 ? TB_AX : TB_AZ);
         for (int k = 0; k < 6; ++k) t[k].assign(n, 0.0);
         poe2d_status st = poe2d_cpml_profile(axis, n, axis == 0 ? g->dx : g->dz, g->dt, g->npml,
                                              g->npower, g->rcoef, g->kmax, amax, s->cp_max,
@@ -505,29 +485,21 @@ poe2d_status poe2d_create(const poe2d_grid *g, const poe2d_coeffs *c, poe2d_solv
                                              t[3].data(), t[4].data(), t[5].data());
         if (st != POE2D_OK) return bail(st);
     }
-    std::vector<int> xm = strip_map(s->htabx, g->nx + 1, &s->wxs);
-    std::vector<int> zm = strip_map(s->htabz, g->nz + 1, &s->wzs);
+    std::vector<int> xm = strip_map(s->htab + TB_AX, g->nx + 1, &s->wxs);
+    std::vector<int> zm = strip_map(s->htab + TB_AZ, g->nz + 1, &s->wzs);
+    s->tn = std::max(g->nx, g->nz) + 1;
 
     s->pitch = (((size_t)kPadCols + g->nx + 2 + 63) / 64) * 64;
     s->alloc_rows = (size_t)g->nz + 1 + 2 * kPadRows;
-    poe2d_status st = POE2D_OK;
-    for (auto &b : s->st)
-        for (auto &p : b)
-            if ((st = alloc_zero(s, &p, s->field_bytes()))) return bail(st);
-    for (auto &p : s->inter)
-        if ((st = alloc_zero(s, &p, s->field_bytes()))) return bail(st);
-    for (auto &p : s->mat)
-        if ((st = alloc_zero(s, &p, s->field_bytes()))) return bail(st);
-    for (auto &b : s->psx)
-        for (auto &p : b)
-            if ((st = alloc_zero(s, &p, s->psx_bytes()))) return bail(st);
-    for (auto &b : s->psz)
-        for (auto &p : b)
-            if ((st = alloc_zero(s, &p, s->psz_bytes()))) return bail(st);
-    for (int k = 0; k < 6; ++k) {
-        if ((st = alloc_zero(s, &s->tabx[k], (size_t)(g->nx + 1) * s->rs))) return bail(st);
-        if ((st = alloc_zero(s, &s->tabz[k], (size_t)(g->nz + 1) * s->rs))) return bail(st);
+    if (s->group_bytes() >= (1ull << 32)) {
+        // the single-pass kernel addresses a group with 32-bit byte offsets
+        fail(POE2D_E_INVALID, "create: grid too large (one field group exceeds 4 GiB)");
+        return bail(POE2D_E_INVALID);
     }
+    poe2d_status st = POE2D_OK;
+    if ((st = alloc_zero(s, &s->arena, NGROUPS * s->group_bytes()))) return bail(st);
+    if ((st = alloc_zero(s, &s->psi, 2 * s->psi_half_elems() * s->rs))) return bail(st);
+    if ((st = alloc_zero(s, &s->tab, (size_t)NTABLES * s->tn * s->rs))) return bail(st);
     void *q = nullptr;
     if ((st = alloc_zero(s, &q, xm.size() * sizeof(int)))) return bail(st);
     s->xmap = (int *)q;
@@ -553,14 +525,24 @@ poe2d_status poe2d_set_schedule(poe2d_solver *s, int schedule, int block_w, int 
 {
     poe2d_status st = check_handle(s);
     if (st) return st;
-    if (schedule != SCHED_FUSED && schedule != SCHED_TWOPASS)
-        return fail(POE2D_E_INVALID, "set_schedule: schedule must be 0 (fused) or 1 (two-pass)");
+    if (schedule != SCHED_FUSED && schedule != SCHED_TWOPASS && schedule != SCHED_FUSED_GROUPS)
+        return fail(POE2D_E_INVALID, "set_schedule: schedule must be 0 (single-pass), 1 (two-pass) or 2 "
+                                     "(single-pass on the coefficient groups)");
     if (block_w != 128 && block_w != 256 && block_w != 512)
         return fail(POE2D_E_INVALID, "set_schedule: block width must be 128, 256 or 512");
     if (tile_rows < 4 || tile_rows > 1 << 20) return fail(POE2D_E_INVALID, "set_schedule: tile rows out of range");
     s->schedule = schedule;
     s->block_w = block_w;
     s->tile_rows = tile_rows;
+    return POE2D_OK;
+}
+
+poe2d_status poe2d_material_palette(poe2d_solver *s, int *entries)
+{
+    poe2d_status st = check_handle(s);
+    if (st) return st;
+    if (!entries) return fail(POE2D_E_INVALID, "material_palette: null output");
+    *entries = s->npal;
     return POE2D_OK;
 }
 
@@ -680,32 +662,10 @@ poe2d_status poe2d_snapshot(poe2d_solver *s, int field, void *out, size_t bytes)
     poe2d_status st = check_handle(s);
     if (st) return st;
     if (!out) return fail(POE2D_E_INVALID, "snapshot: null output");
-    const size_t w = (size_t)s->grid.nx + 1;
-    const size_t need = w * (s->grid.nz + 1) * s->rs;
+    if (field < POE2D_FIELD_VX || field > POE2D_FIELD_SYY) return fail(POE2D_E_INVALID, "snapshot: unknown field");
+    const size_t need = (size_t)(s->grid.nx + 1) * (s->grid.nz + 1) * s->rs;
     if (bytes < need) return fail(POE2D_E_INVALID, "snapshot: output buffer too small");
-    int idx = -1;
-    switch (field) {
-    case POE2D_FIELD_VX: idx = ST_VX; break;
-    case POE2D_FIELD_VY: idx = ST_VZ; break;
-    case POE2D_FIELD_WX: idx = ST_WX; break;
-    case POE2D_FIELD_WY: idx = ST_WZ; break;
-    case POE2D_FIELD_EXX: idx = ST_EXX; break;
-    case POE2D_FIELD_EYY: idx = ST_EYY; break;
-    case POE2D_FIELD_ZETA: idx = ST_ZETA; break;
-    case POE2D_FIELD_SXY: idx = ST_SXY; break;
-    case POE2D_FIELD_P:
-    case POE2D_FIELD_SXX:
-    case POE2D_FIELD_SYY: {
-        const int what = field - POE2D_FIELD_P;
-        return s->precision == POE2D_F32 ? derived_snapshot<float>(s, what, out)
-                                         : derived_snapshot<double>(s, what, out);
-    }
-    default: return fail(POE2D_E_INVALID, "snapshot: unknown field");
-    }
-    HIP_TRY(hipStreamSynchronize(s->stream));
-    const unsigned char *src = (const unsigned char *)s->st[s->cur][idx] + s->origin() * s->rs;
-    HIP_TRY(hipMemcpy2D(out, w * s->rs, src, s->pitch * s->rs, w * s->rs, s->grid.nz + 1, hipMemcpyDeviceToHost));
-    return POE2D_OK;
+    return s->precision == POE2D_F32 ? field_to_host<float>(s, field, out) : field_to_host<double>(s, field, out);
 }
 
 poe2d_status poe2d_checkpoint_size(poe2d_solver *s, size_t *bytes)
@@ -726,7 +686,7 @@ poe2d_status poe2d_checkpoint(poe2d_solver *s, void *blob, size_t bytes)
     if (!blob || bytes < need) return fail(POE2D_E_INVALID, "checkpoint: buffer too small");
     CkptHeader h{};
     std::memcpy(h.magic, "POE2DCK", 8);
-    h.version = 1; h.precision = s->precision; h.nx = s->grid.nx; h.nz = s->grid.nz;
+    h.version = 2; h.precision = s->precision; h.nx = s->grid.nx; h.nz = s->grid.nz;
     h.step = s->step; h.nrec = s->nrec; h.ncomp = s->ncomp; h.nsamp = s->nsamp;
     h.wxs = s->wxs; h.wzs = s->wzs; h.pitch = s->pitch;
     std::memcpy(blob, &h, sizeof h);
@@ -750,7 +710,7 @@ poe2d_status poe2d_restore(poe2d_solver *s, const void *blob, size_t bytes)
     if (!blob || bytes < need) return fail(POE2D_E_INVALID, "restore: blob too small");
     CkptHeader h;
     std::memcpy(&h, blob, sizeof h);
-    if (std::memcmp(h.magic, "POE2DCK", 8) || h.version != 1 || h.precision != s->precision ||
+    if (std::memcmp(h.magic, "POE2DCK", 8) || h.version != 2 || h.precision != s->precision ||
         h.nx != s->grid.nx || h.nz != s->grid.nz || h.nrec != s->nrec || h.ncomp != s->ncomp ||
         h.nsamp != s->nsamp || h.wxs != s->wxs || h.wzs != s->wzs || h.pitch != s->pitch)
         return fail(POE2D_E_INVALID, "restore: checkpoint does not match this solver's geometry");
@@ -770,13 +730,9 @@ poe2d_status poe2d_reset(poe2d_solver *s)
 {
     poe2d_status st = check_handle(s);
     if (st) return st;
-    for (auto &b : s->st)
-        for (auto p : b) HIP_TRY(hipMemsetAsync(p, 0, s->field_bytes(), s->stream));
-    for (auto p : s->inter) HIP_TRY(hipMemsetAsync(p, 0, s->field_bytes(), s->stream));
-    for (auto &b : s->psx)
-        for (auto p : b) HIP_TRY(hipMemsetAsync(p, 0, s->psx_bytes(), s->stream));
-    for (auto &b : s->psz)
-        for (auto p : b) HIP_TRY(hipMemsetAsync(p, 0, s->psz_bytes(), s->stream));
+    // state groups V0, S0, V1, S1 and the intermediates are contiguous after MS, MV
+    HIP_TRY(hipMemsetAsync(s->group_ptr(G_V0), 0, (NGROUPS - G_V0) * s->group_bytes(), s->stream));
+    HIP_TRY(hipMemsetAsync(s->psi, 0, 2 * s->psi_half_elems() * s->rs, s->stream));
     if (s->traces) HIP_TRY(hipMemsetAsync(s->traces, 0, s->trace_bytes(), s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     s->step = 0;

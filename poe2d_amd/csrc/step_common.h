@@ -1,4 +1,17 @@
 // Device-side building blocks shared by the time-step kernels.
+//
+// Field layout.  Point data live in "groups": one 4-vector of reals per point, stored as an SoA
+// array of 4-vectors with row pitch `pitch` (a multiple of 64 points).  Each group holds the four
+// quantities one stage of the update consumes together, so a wave moves 64 x 16 B (FP32) of
+// consecutive memory per load:
+//   MS  (alpha, M, lambda_c, mu)       coefficients of the stress update
+//   MV  (rho, rho_f, m, b)             coefficients of the velocity update
+//   V   (vx, vy, wx, wy)               velocities, double-buffered
+//   S   (e_xx, e_yy, zeta, sigma_xy)   strains, fluid content, shear stress, double-buffered
+//   I   (p, sigma_xx, sigma_yy, -)     intermediates of the two-pass schedule
+// All groups sit in one arena, `group_stride` points apart; every group carries zero pad rows
+// above/below and pad columns left/right of the (nz+1) x (nx+1) grid, and `origin` is the offset
+// of point (0, 0).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -7,10 +20,22 @@
 
 namespace poe2d {
 
-// C-PML memory variables kept in the x strips (columns with a nonzero x profile), one array each,
-// indexed [z][strip column].  The coefficient set each one uses is noted.
+template <class T>
+struct alignas(4 * sizeof(T)) Vec4 {
+    T x, y, z, w;
+};
+
+enum { G_MS = 0, G_MV = 1, G_V0 = 2, G_S0 = 3, G_V1 = 4, G_S1 = 5, G_I = 6, NGROUPS = 7 };
+__host__ __device__ constexpr int group_v(int buf) { return G_V0 + 2 * buf; }
+__host__ __device__ constexpr int group_s(int buf) { return G_S0 + 2 * buf; }
+
+constexpr int kPadRows = 8;  // zero rows above and below the grid
+constexpr int kPadCols = 4;  // zero columns left of the grid (>= 1 more on the right)
+
+// C-PML memory variables.  Each strip point carries a record of 6 values per state buffer.
+// x strips (columns with a nonzero x profile), record index z * wxs + strip column:
 enum {
-    PX_DVX = 0,   // d vx / dx in the strain update          (half-grid x coefficients)
+    PX_DVX = 0,   // d vx / dx in the strain update          (half-grid x)
     PX_DWX = 1,   // d wx / dx in the fluid-content update   (half-grid x)
     PX_DVY = 2,   // d vy / dx in the shear-stress update    (grid x)
     PX_DSXX = 3,  // d sxx / dx in the vx update             (grid x)
@@ -18,7 +43,7 @@ enum {
     PX_DSXY = 5,  // d sxy / dx in the vy update             (half-grid x)
     NPSI_X = 6
 };
-// ... and in the z strips (rows with a nonzero z profile), indexed [strip row][x].
+// z strips (rows with a nonzero z profile), record index strip row * pitch + x:
 enum {
     PZ_DVZ = 0,   // d vy / dz in the strain update          (grid z)
     PZ_DWZ = 1,   // d wy / dz in the fluid-content update   (grid z)
@@ -28,30 +53,60 @@ enum {
     PZ_DPZ = 5,   // d p / dz in the wy update               (half-grid z)
     NPSI_Z = 6
 };
+constexpr int kMaxPalette = 256;  // distinct (MS, MV) tuples a one-byte material index can select
 
+// 1-D C-PML profiles, one table each, `tn` entries apart.  The K tables hold 1/K (K = 1 outside
+// the layers, and wherever K_max = 1).
+enum { TB_AX, TB_BX, TB_RKX, TB_AXH, TB_BXH, TB_RKXH, TB_AZ, TB_BZ, TB_RKZ, TB_AZH, TB_BZH, TB_RKZH, NTABLES };
+
+// Everything a step kernel needs: the arena of groups, the memory-variable arena (x strips of
+// both buffers, then z strips of both buffers), the profile tables and the strip maps.
 template <class T>
 struct StepArgs {
-    int nx, nz;        // NX, NZ (largest point indices)
-    int pitch;         // elements between rows (>= nx + 2, pad columns zero)
-    int wxs;           // number of x-strip columns
-    T rdx, rdz;        // 1 / (24 dx), 1 / (24 dz)
-    T dt, inv_dt;
-    // state
-    T *vx, *vz, *wx, *wz, *exx, *eyy, *zeta, *sxy;
-    // intermediates of the two-pass schedule
-    T *p, *sxx, *syy;
-    // material
-    const T *rho, *rhof, *m, *mu, *lc, *M, *al, *b;
-    // C-PML memory variables and 1-D profiles
-    T *psx[NPSI_X];
-    T *psz[NPSI_Z];
-    const T *ax, *bx, *Kx, *axh, *bxh, *Kxh;
-    const T *az, *bz, *Kz, *azh, *bzh, *Kzh;
-    const int *xmap, *zmap;
-    // source of this step
+    int nx, nz, pitch, wxs;
+    long long origin;        // offset of point (0,0) inside a group
+    long long group_stride;  // points between consecutive groups
+    long long pxn, pzn;      // x-strip / z-strip records per state buffer
+    int tn;                  // entries per profile table
+    int in, out;             // state buffer read / written by this step
+    T rdx, rdz, dt, inv_dt;  // 1/(24 dx), 1/(24 dz), dt, 1/dt
+    Vec4<T> *grp;            // arena of groups
+    T *psi;                  // memory variables
+    const T *tab;            // profile tables
+    const int *xmap, *zmap;  // strip index of each column / row, -1 outside the strips
+    // material palette: when the grid holds at most kMaxPalette distinct coefficient tuples, a
+    // one-byte index per point (laid out like a group, one byte per point) selects the MS entry
+    // pal[i] and the MV entry pal[kMaxPalette + i]; npal = 0 when the grid has no palette
+    const unsigned char *mid;
+    const Vec4<T> *pal;
+    int npal;
     int src_kind, src_x, src_z;
-    T src_val;
+    T src_val;               // source sample of this step
+    // tiling of the single-pass kernel
+    int th, ntx, ntiles, per_xcd;
 };
+
+template <class T>
+__device__ __forceinline__ const Vec4<T> *group_c(const StepArgs<T> &A, int g)
+{
+    return A.grp + A.origin + (long long)g * A.group_stride;
+}
+template <class T>
+__device__ __forceinline__ Vec4<T> *group_w(const StepArgs<T> &A, int g)
+{
+    return A.grp + A.origin + (long long)g * A.group_stride;
+}
+// Memory-variable record of x-strip record `r` / z-strip record `r` in state buffer `buf`.
+template <class T>
+__device__ __forceinline__ T *psi_x(const StepArgs<T> &A, int buf, long long r)
+{
+    return A.psi + ((long long)buf * A.pxn + r) * NPSI_X;
+}
+template <class T>
+__device__ __forceinline__ T *psi_z(const StepArgs<T> &A, int buf, long long r)
+{
+    return A.psi + 2LL * NPSI_X * A.pxn + ((long long)buf * A.pzn + r) * NPSI_Z;
+}
 
 // Fourth-order staggered difference (27 (f1 - f0) - (f2 - fm)) / (24 h) between f0 and f1, with
 // f2 the far point beyond f1 and fm the far point before f0; rh = 1 / (24 h).
@@ -61,14 +116,20 @@ __device__ __forceinline__ T d4(T f1, T f0, T f2, T fm, T rh)
     return (T(27) * (f1 - f0) - (f2 - fm)) * rh;
 }
 
-// C-PML correction of a derivative d inside a strip: psi <- b psi + a d, returns d / K + psi.
+// C-PML correction of a derivative d: psi_new = b psi_old + a d (stored when `store`), returns
+// d / K + psi_new, with rk = 1/K.
 template <class T>
-__device__ __forceinline__ T cpml(T d, T *psi, T a, T b, T K)
+__device__ __forceinline__ T cpml(T d, const T *psi_o, T *psi_n, bool store, T a, T b, T rk)
 {
-    const T s = b * *psi + a * d;
-    *psi = s;
-    return d / K + s;
+    const T s = b * *psi_o + a * d;
+    if (store) *psi_n = s;
+    return d * rk + s;
 }
+
+// Quotient used by the velocity update: IEEE division in FP64; in FP32 the hardware reciprocal
+// (1 ulp) times the numerator, within 2 ulp of the IEEE quotient.
+__device__ __forceinline__ float quot(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double quot(double a, double b) { return a / b; }
 
 // Biot momentum update of one velocity point: solid velocity v and relative fluid velocity w
 // from the stress divergence `div` and the pressure gradient `grad`,
@@ -84,9 +145,9 @@ __device__ __forceinline__ void biot_velocity(T &v, T &w, T div, T grad, T rho, 
     const T c1 = co + half_rb;
     const T c2 = co - half_rb;
     const T wold = w;
-    const T wnew = (c2 * wold + (-rhof * div - rho * grad)) / c1;
+    const T wnew = quot(c2 * wold + (-rhof * div - rho * grad), c1);
     const T wavg = (wold + wnew) * T(0.5);
-    v = v + (m * div + rhof * grad + rhof * b * wavg) / co;
+    v = v + quot(m * div + rhof * grad + rhof * b * wavg, co);
     w = wnew;
 }
 
@@ -99,38 +160,18 @@ __device__ __forceinline__ T pore_pressure(T exx, T eyy, T zeta, T al, T M)
     return -al * M * (exx + eyy) + M * zeta;
 }
 
-// Arguments of the single-pass kernel: state at step n is read from the `*_o` arrays and the
-// state at n+1 written to the `*_n` arrays (double buffering lets neighbouring tiles recompute
-// halo stresses from unmodified inputs).  Field pointers address point (0, 0); rows are
-// `pitch` apart and the arrays carry zero pad rows/columns around the grid.
-template <class T>
-struct FusedArgs {
-    int nx, nz, pitch, wxs;
-    int th;          // output rows per tile
-    int ntx;         // tiles across x
-    int ntiles;      // tiles in total
-    int per_xcd;     // tiles per XCD slot (grid = 8 * per_xcd blocks)
-    T rdx, rdz, dt, inv_dt;
-    const T *vx_o, *vz_o, *wx_o, *wz_o, *exx_o, *eyy_o, *zeta_o, *sxy_o;
-    T *vx_n, *vz_n, *wx_n, *wz_n, *exx_n, *eyy_n, *zeta_n, *sxy_n;
-    const T *rho, *rhof, *m, *mu, *lc, *M, *al, *b;
-    const T *psx_o[NPSI_X];
-    T *psx_n[NPSI_X];
-    const T *psz_o[NPSI_Z];
-    T *psz_n[NPSI_Z];
-    const T *ax, *bx, *Kx, *axh, *bxh, *Kxh;
-    const T *az, *bz, *Kz, *azh, *bzh, *Kzh;
-    const int *xmap, *zmap;
-    int src_kind, src_x, src_z;
-    T src_val;
-};
+// Stress-update index ranges of the reference kernels (seismic_CPML_2D_poroelastic_withCUDA.cu):
+// stress1 x in [1,NX), z in [2,NZ-1) (:60-62); stress2 x in [2,NX), z in [1,NZ-1) (:212-214);
+// stress3 x in [1,NX-1), z in [2,NZ) (:237-239); vx x in [2,NX), z in [2,NZ) (:289-291);
+// vz x in [1,NX-1), z in [1,NZ-1) (:326-328).
 
 template <class T>
 void launch_twopass(const StepArgs<T> &A, hipStream_t stream, int phase);
 
-// Launches the single-pass step with block width bw (128, 256 or 512 threads).
+// Launches the single-pass step with block width bw (128, 256 or 512 threads); with `palette` the
+// kernel reads the one-byte material index instead of the MS and MV groups (needs A.npal > 0).
 template <class T>
-hipError_t launch_fused(const FusedArgs<T> &A, int bw, hipStream_t stream);
+hipError_t launch_fused(const StepArgs<T> &A, int bw, bool palette, hipStream_t stream);
 
 // Output columns of one tile for block width bw (halo of 4 columns on each side).
 inline int fused_tile_width(int bw) { return bw - 8; }

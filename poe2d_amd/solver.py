@@ -202,9 +202,17 @@ class Solver(HandleSolver):
         self._check(self._lib.poe2d_reset(self._h), "reset")
 
     def set_schedule(self, schedule=0, block_w=256, tile_rows=64):
-        """0: single fused launch per step (tiles of block_w-8 columns x tile_rows rows);
-        1: separate stress and velocity launches."""
+        """0: single fused launch per step (tiles of block_w-8 columns x tile_rows rows), reading
+        the coefficients through the material palette when the grid has one;
+        1: separate stress and velocity launches;
+        2: as 0, always reading the per-point coefficient arrays."""
         self._check(self._lib.poe2d_set_schedule(self._h, schedule, block_w, tile_rows), "set_schedule")
+
+    def material_palette(self):
+        """Distinct coefficient tuples of the grid when at most 256, else 0."""
+        n = ctypes.c_int()
+        self._check(self._lib.poe2d_material_palette(self._h, ctypes.byref(n)), "material_palette")
+        return n.value
 
     def checkpoint(self):
         n = ctypes.c_size_t()

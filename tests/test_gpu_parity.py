@@ -266,6 +266,48 @@ def test_schedules_agree_and_tiles_are_exact():
             assert np.abs(a - b).max() <= 1e-5 * scale, bw
 
 
+def test_palette_and_groups_agree():
+    """A two-material patchy grid is read through the one-byte material palette by default;
+    forcing the per-point coefficient arrays (schedule 2) gives the same seismograms and fields."""
+    from poe2d_amd.material import material_biot
+    n, nt = 300, 500
+    coeffs = material_biot(**biot_fields(n, n, random_patchy_mask(n, n, 0.25, seed=22)))
+    rec = ReceiverSpec(ix=np.arange(25, 276, 10, dtype=np.int32), iz=np.full(26, 200, np.int32))
+    prob = Problem(nx=n, nz=n, dx=0.5, dz=0.5, dt=5e-5, nt=nt, coeffs=coeffs,
+                   source=SourceSpec(kind=_abi.SRC_PRESSURE_DGAUSS, ix=150, iz=140, amp=1e10),
+                   receivers=rec, npml=20, f0=80.0)
+    with Solver(prob) as s:
+        assert s.material_palette() == 2
+    for precision in (F32, F64):
+        say(f"palette vs coefficient groups, precision {precision}")
+        pal = _run_sched(prob, 0, 256, 64, precision)
+        grp = _run_sched(prob, 2, 256, 64, precision)
+        for a, b in zip(pal, grp):
+            scale = np.abs(a).max()
+            say(f"  bitwise equal: {np.array_equal(a, b)}, max diff / scale {np.abs(a - b).max() / scale:.2e}")
+            assert scale > 0 and np.abs(a - b).max() <= 1e-6 * scale
+
+
+def test_parity_graded_medium_without_palette():
+    """A medium whose porosity grades with depth and offset has more than 256 distinct
+    coefficient tuples: the fused step reads the per-point coefficient arrays, and matches the
+    oracle."""
+    from poe2d_amd.material import material_biot
+    n, nt = 240, 1000
+    fields = biot_fields(n, n)
+    ramp = np.linspace(-1, 1, n + 1)
+    fields["phi"] = fields["phi"] * (1.0 + 0.15 * ramp[:, None]) * (1.0 + 0.05 * ramp[None, :])
+    coeffs = material_biot(**fields)
+    rec = ReceiverSpec(ix=np.arange(30, 211, 6, dtype=np.int32), iz=np.full(31, 160, np.int32))
+    prob = Problem(nx=n, nz=n, dx=0.5, dz=0.5, dt=5e-5, nt=nt, coeffs=coeffs,
+                   source=SourceSpec(kind=_abi.SRC_FORCE_Y_RICKER, ix=120, iz=100, amp=1e10),
+                   receivers=rec, npml=20, f0=80.0)
+    with Solver(prob) as s:
+        assert s.material_palette() == 0
+    say(f"graded medium {n + 1}^2 nt={nt}, no palette")
+    assert_parity(run_gpu(prob), prob, "graded medium")
+
+
 @pytest.mark.parametrize("case", ["pressure_patchy", "force_patchy"])
 def test_two_pass_schedule_parity(case):
     from poe2d_amd.material import material_biot
