@@ -4,22 +4,30 @@
 // Work decomposition.  A workgroup of BW threads owns a column strip of W = BW - 8 output columns
 // and th output rows.  Thread `lane` holds column c = x0 - 4 + lane, so the block spans the tile
 // plus a 4-column halo on each side.  The block marches down its rows; at row step s it
-//   1. slides a per-thread 5-row register window of the old velocities (rows s-2 .. s+2),
+//   1. holds the old velocities of rows s-2 .. s+2 in registers (row s+3 is in flight),
 //   2. forms the stresses of row s for every held column (strains, fluid content, shear stress,
 //      pore pressure and normal stresses, with their C-PML memory variables), taking x
-//      neighbours through LDS and z neighbours from the register window,
-//   3. forms the velocities of row j = s-2 from per-thread windows of stress rows and the x
+//      neighbours through LDS and z neighbours from registers,
+//   3. forms the velocities of row j = s-2 from registers holding stress rows s-4 .. s and the x
 //      neighbours of stress row j, which were put in an LDS ring two row steps earlier.
-// Each point's state vectors (V, S) are read with one 16-byte load each, prefetched one row step
-// ahead, and written with one 16-byte store each; every access is a uniform group base plus one
-// 32-bit per-lane offset.  The coefficients come either from the MS and MV groups (one 16-byte
+// Each point's state vectors (V, S) are read with one 16-byte load each, issued one row step
+// before use, and written with one 16-byte store each; every access is a uniform group base plus
+// one 32-bit per-lane offset.  The coefficients come either from the MS and MV groups (one 16-byte
 // load each) or, in palette mode, from a one-byte material index per point that selects an
-// (MS, MV) entry of a palette held in LDS: 1 byte per point instead of 32.  Stresses of the 2 halo columns
-// and 2 halo rows on each side are recomputed from the unmodified old state (double buffering)
-// instead of being exchanged.  One barrier per row step.  Tiles are mapped so that consecutive
-// tiles of a row band land on the same XCD (blocks are dispatched round-robin over the 8 XCDs),
-// keeping halo re-reads inside one L2.
+// (MS, MV) entry of a palette held in LDS: 1 byte per point instead of 32.
+//
+// Register rings.  The row loop is unrolled by U = 6 and every per-row register ring (and LDS
+// ring) has a size dividing U, so inside the unrolled body the slot of each row is a compile-time
+// constant: a load lands directly in the register that later consumes it, with no register
+// rotation at the loop latch (a rotation would force the load to complete within the row step
+// that issued it).
+//
+// Stresses of the 2 halo columns and 2 halo rows on each side are recomputed from the unmodified
+// old state (double buffering) instead of being exchanged.  One barrier per row step.  Tiles are
+// mapped so that consecutive tiles of a row band land on the same XCD (blocks are dispatched
+// round-robin over the 8 XCDs), keeping halo re-reads inside one L2.
 #include <cstdint>
+#include <type_traits>
 
 #include "step_common.h"
 
@@ -72,8 +80,6 @@ __global__ void __launch_bounds__(BW) fused_step(StepArgs<T> A)
             if ((i & (kMaxPalette - 1)) < A.npal) lpal[i] = A.pal[i];
         __syncthreads();
     }
-    auto ms_at = [&](int id) -> Vec4<T> { return lpal[id]; };
-    auto mv_at = [&](int id) -> Vec4<T> { return lpal[kMaxPalette + id]; };
 
     // per-column C-PML profile (x strips only)
     const int sx = (c >= 0 && c <= nx) ? A.xmap[c] : -1;
@@ -90,75 +96,86 @@ __global__ void __launch_bounds__(BW) fused_step(StepArgs<T> A)
     const bool src_p = A.src_kind == POE2D_SRC_PRESSURE_DGAUSS && c == A.src_x;
     const bool src_f = A.src_kind == POE2D_SRC_FORCE_Y_RICKER && c == A.src_x;
 
-    // velocity window: slot k holds row s-2+k as (vx, vy, wx, wy)
-    Vec4<T> vw[5];
-    vw[0] = zero4;
+    // Register rings.  Row r of a ring of size R sits in slot (r - s0) mod R, s0 = zA - 2 being the
+    // first row step, so at phase PH of the unrolled group (s - s0 = PH mod 6) row s + k is in slot
+    // (PH + k) mod R.
+    const int s0 = zA - 2;
+    const int send = zB + 2;
+    Vec4<T> vr[6];              // velocities (vx, vy, wx, wy), rows s-2 .. s+3
+    Vec4<T> sr[2];              // strains, fluid content, shear stress, rows s, s+1
+    Vec4<T> msr[3];             // (alpha, M, lambda_c, mu), rows s, s+1 (and s+2 in flight)
+    Vec4<T> mvr[3];             // (rho, rho_f, m, b), rows s-2, s-1 (and s in flight)
+    int idr[6];                 // palette mode: material index, rows s-1 .. s+2
+    T sxyr[6], syyr[6], pr[6];  // computed sxy rows s-4 .. s; syy and p rows s-3 .. s
 #pragma unroll
-    for (int k = 0; k < 4; ++k) vw[k + 1] = ld(Vo, zA - 4 + k);
-    // stress windows: sxy rows s-4..s, syy and p rows s-3..s
-    T sxyw[5] = {0, 0, 0, 0, 0}, syyw[4] = {0, 0, 0, 0}, pw[4] = {0, 0, 0, 0};
-    Vec4<T> ms_cur, mv_cur = zero4;  // (alpha, M, lambda_c, mu) of row s, (rho, rho_f, m, b) of row j = s-2
-    // prefetched inputs of the next row step
-    Vec4<T> v_pf = ld(Vo, zA);
-    Vec4<T> s_pf = ld(So, zA - 2);
-    Vec4<T> ms_pf, mv_pf;
-    int id_m1 = 0, id_0 = 0, id_p1 = 0, id_pf = 0;  // palette mode: material index of rows s-1, s, s+1
+    for (int k = 0; k < 6; ++k) {
+        sxyr[k] = T(0);
+        syyr[k] = T(0);
+        pr[k] = T(0);
+        idr[k] = 0;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        msr[k] = zero4;
+        mvr[k] = zero4;
+    }
+#pragma unroll
+    for (int k = -2; k <= 2; ++k) vr[(k + 6) % 6] = ld(Vo, s0 + k);
+    sr[0] = ld(So, s0);
+    sr[1] = zero4;
+    vr[3] = zero4;
     if constexpr (PAL) {
-        id_0 = ldm(zA - 3);
-        id_p1 = ldm(zA - 2);
-        id_pf = ldm(zA - 1);
-        ms_cur = ms_at(id_p1);
+        idr[5] = ldm(s0 - 1);
+        idr[0] = ldm(s0);
+        idr[1] = ldm(s0 + 1);
+        msr[0] = lpal[idr[0]];
     } else {
-        ms_cur = ld(MS, zA - 2);
-        ms_pf = ld(MS, zA - 1);
-        mv_pf = ld(MV, zA - 3);
+        msr[0] = ld(MS, s0);
+        msr[1] = ld(MS, s0 + 1);
+        mvr[2] = ld(MV, s0 - 1);
     }
 
-    for (int s = zA - 2; s < zB + 2; ++s) {
-        // ---- inputs of this row step, then prefetch the next one
-#pragma unroll
-        for (int k = 0; k < 4; ++k) vw[k] = vw[k + 1];
-        vw[4] = v_pf;
-        const Vec4<T> so = s_pf;
-        Vec4<T> ms_nxt, mv_nxt;  // coefficients of row s+1 (stress side) and row s-1 = j+1 (velocity side)
+    int s = s0;
+    // One row step at phase PH of the unrolled group.
+    auto row_step = [&](auto phase) __attribute__((always_inline)) {
+        constexpr int PH = decltype(phase)::value;
+        auto V = [&](int k) -> Vec4<T> & { return vr[(PH + k + 12) % 6]; };  // velocity row s+k
+        const Vec4<T> so = sr[PH & 1];
+        // ---- loads for later row steps
+        V(3) = ld(Vo, s + 3);
+        sr[(PH + 1) & 1] = ld(So, s + 1);
+        const Vec4<T> &ms_cur = msr[PH % 3];      // row s
+        Vec4<T> &ms_nxt = msr[(PH + 1) % 3];      // row s+1
+        const Vec4<T> &mv_j = mvr[(PH + 1) % 3];  // row j = s-2
+        Vec4<T> &mv_j1 = mvr[(PH + 2) % 3];       // row j+1 = s-1
         if constexpr (PAL) {
-            id_m1 = id_0;
-            id_0 = id_p1;
-            id_p1 = id_pf;
-            ms_nxt = ms_at(id_p1);
-            mv_nxt = mv_at(id_m1);
+            ms_nxt = lpal[idr[(PH + 1) % 6]];
+            mv_j1 = lpal[kMaxPalette + idr[(PH + 5) % 6]];
+            idr[(PH + 2) % 6] = ldm(s + 2);
         } else {
-            ms_nxt = ms_pf;
-            mv_nxt = mv_pf;
-        }
-        v_pf = ld(Vo, s + 3);
-        s_pf = ld(So, s + 1);
-        if constexpr (PAL) {
-            id_pf = ldm(s + 2);
-        } else {
-            ms_pf = ld(MS, s + 2);
-            mv_pf = ld(MV, s);
+            msr[(PH + 2) % 3] = ld(MS, s + 2);
+            mvr[PH % 3] = ld(MV, s);
         }
 
         // ---- stresses of row s
         const bool srow = s >= zA && s < zB;
         const bool zr1 = s >= 2 && s < nz - 1;
         const bool zr2 = s >= 1 && s < nz - 1;
-        const int szs = (s >= 0 && s <= nz) ? A.zmap[s] : -1;
-        T(*xv)[BW + 4] = lv[s & 1];
-        xv[0][lane + 2] = vw[2].x;
-        xv[1][lane + 2] = vw[2].z;
-        xv[2][lane + 2] = vw[2].y;
+        const int szs = zstrip(A, s);
+        T(*xv)[BW + 4] = lv[PH & 1];
+        xv[0][lane + 2] = V(0).x;
+        xv[1][lane + 2] = V(0).z;
+        xv[2][lane + 2] = V(0).y;
         __syncthreads();
 
         const bool store_s = own && srow;
         T exx_n = so.x, eyy_n = so.y, zeta_n = so.z, sxy_n = so.w;
         T p = 0, sxx = 0, syy = 0;
         if (xr1 && zr1) {
-            T dvx = d4(xv[0][lane + 3], vw[2].x, xv[0][lane + 4], xv[0][lane + 1], rdx);
-            T dwx = d4(xv[1][lane + 3], vw[2].z, xv[1][lane + 4], xv[1][lane + 1], rdx);
-            T dvz = d4(vw[2].y, vw[1].y, vw[3].y, vw[0].y, rdz);
-            T dwz = d4(vw[2].w, vw[1].w, vw[3].w, vw[0].w, rdz);
+            T dvx = d4(xv[0][lane + 3], V(0).x, xv[0][lane + 4], xv[0][lane + 1], rdx);
+            T dwx = d4(xv[1][lane + 3], V(0).z, xv[1][lane + 4], xv[1][lane + 1], rdx);
+            T dvz = d4(V(0).y, V(-1).y, V(1).y, V(-2).y, rdz);
+            T dwz = d4(V(0).w, V(-1).w, V(1).w, V(-2).w, rdz);
             if (sx >= 0) {
                 const long long r = (long long)s * A.wxs + sx;
                 const T *po = psi_x(A, A.in, r);
@@ -187,8 +204,8 @@ __global__ void __launch_bounds__(BW) fused_step(StepArgs<T> A)
             }
         }
         if (xr2 && zr2) {
-            T ddx = d4(vw[2].y, xv[2][lane + 1], xv[2][lane + 3], xv[2][lane], rdx);
-            T ddz = d4(vw[3].x, vw[2].x, vw[4].x, vw[1].x, rdz);
+            T ddx = d4(V(0).y, xv[2][lane + 1], xv[2][lane + 3], xv[2][lane], rdx);
+            T ddz = d4(V(1).x, V(0).x, V(2).x, V(-1).x, rdz);
             if (sx >= 0) {
                 const long long r = (long long)s * A.wxs + sx;
                 ddx = cpml(ddx, psi_x(A, A.in, r) + PX_DVY, psi_x(A, A.out, r) + PX_DVY, store_s, ax, bx, rkx);
@@ -202,36 +219,29 @@ __global__ void __launch_bounds__(BW) fused_step(StepArgs<T> A)
             sxy_n = so.w + ms_nxt.w * (ddx + ddz) * dt;  // mu of point (x, z+1)
         }
         if (store_s) *(Vec4<T> *)(Sn + off(s)) = Vec4<T>{exx_n, eyy_n, zeta_n, sxy_n};
-        ms_cur = ms_nxt;
         {
-            T(*xs)[BW + 4] = ls[(s + 3) % 3];
+            T(*xs)[BW + 4] = ls[PH % 3];
             xs[0][lane + 2] = sxx;
             xs[1][lane + 2] = p;
             xs[2][lane + 2] = sxy_n;
         }
-#pragma unroll
-        for (int k = 0; k < 4; ++k) sxyw[k] = sxyw[k + 1];
-        sxyw[4] = sxy_n;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            syyw[k] = syyw[k + 1];
-            pw[k] = pw[k + 1];
-        }
-        syyw[3] = syy;
-        pw[3] = p;
+        sxyr[PH % 6] = sxy_n;
+        syyr[PH % 6] = syy;
+        pr[PH % 6] = p;
+        auto SXY = [&](int k) -> T { return sxyr[(PH + k + 12) % 6]; };  // sxy of row s+k
+        auto SYY = [&](int k) -> T { return syyr[(PH + k + 12) % 6]; };
+        auto PP = [&](int k) -> T { return pr[(PH + k + 12) % 6]; };
 
         // ---- velocities of row j = s - 2 (its stress row went into the LDS ring two steps ago)
         const int j = s - 2;
-        const Vec4<T> mv_j = mv_cur;  // row j
-        mv_cur = mv_nxt;              // row j+1, and row j of the next step
         if (j >= zA && j < zB && own) {
-            const T(*xs)[BW + 4] = ls[(j + 3) % 3];
-            const int szj = A.zmap[j];
-            Vec4<T> v = vw[0];  // (vx, vy, wx, wy) of row j, step n
+            const T(*xs)[BW + 4] = ls[(PH + 1) % 3];
+            const int szj = zstrip(A, j);
+            Vec4<T> v = V(-2);  // (vx, vy, wx, wy) of row j, step n
             if (xr4 && j >= 2 && j < nz) {
                 T e1 = d4(xs[0][lane + 2], xs[0][lane + 1], xs[0][lane + 3], xs[0][lane], rdx);
                 T e2 = d4(xs[1][lane + 2], xs[1][lane + 1], xs[1][lane + 3], xs[1][lane], rdx);
-                T e3 = d4(sxyw[2], sxyw[1], sxyw[3], sxyw[0], rdz);
+                T e3 = d4(SXY(-2), SXY(-3), SXY(-1), SXY(-4), rdz);
                 if (sx >= 0) {
                     const long long r = (long long)j * A.wxs + sx;
                     const T *po = psi_x(A, A.in, r);
@@ -248,8 +258,8 @@ __global__ void __launch_bounds__(BW) fused_step(StepArgs<T> A)
             }
             if (xr5 && j >= 1 && j < nz - 1) {
                 T e1 = d4(xs[2][lane + 3], xs[2][lane + 2], xs[2][lane + 4], xs[2][lane + 1], rdx);
-                T e2 = d4(syyw[2], syyw[1], syyw[3], syyw[0], rdz);
-                T e3 = d4(pw[2], pw[1], pw[3], pw[0], rdz);
+                T e2 = d4(SYY(-1), SYY(-2), SYY(0), SYY(-3), rdz);
+                T e3 = d4(PP(-1), PP(-2), PP(0), PP(-3), rdz);
                 if (sx >= 0) {
                     const long long r = (long long)j * A.wxs + sx;
                     e1 = cpml(e1, psi_x(A, A.in, r) + PX_DSXY, psi_x(A, A.out, r) + PX_DSXY, true, axh, bxh, rkxh);
@@ -265,10 +275,23 @@ __global__ void __launch_bounds__(BW) fused_step(StepArgs<T> A)
                 }
                 T div = e1 + e2;
                 if (src_f && j == A.src_z) div += A.src_val;
-                biot_velocity(v.y, v.w, div, e3, mv_cur.x, mv_cur.y, mv_cur.z, mv_cur.w, A.inv_dt);
+                biot_velocity(v.y, v.w, div, e3, mv_j1.x, mv_j1.y, mv_j1.z, mv_j1.w, A.inv_dt);
             }
             *(Vec4<T> *)(Vn + off(j)) = v;
         }
+        ++s;
+    };
+
+    // Whole groups of 6 row steps: the last group may run up to 5 steps past zB + 2, over rows
+    // that store nothing (every store is guarded by the tile's row range) and read only zero pad
+    // rows (at most row nz + 10 < nz + 1 + kPadRows).
+    for (int g = (send - s0 + 5) / 6; g > 0; --g) {
+        row_step(std::integral_constant<int, 0>{});
+        row_step(std::integral_constant<int, 1>{});
+        row_step(std::integral_constant<int, 2>{});
+        row_step(std::integral_constant<int, 3>{});
+        row_step(std::integral_constant<int, 4>{});
+        row_step(std::integral_constant<int, 5>{});
     }
 }
 

@@ -56,6 +56,7 @@ struct poe2d_solver {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     size_t pitch = 0, alloc_rows = 0;  // points per row, rows per group (with pads)
     int wxs = 0, wzs = 0;              // x-strip columns, z-strip rows
+    int ztop = 0, zbot = 0;            // z strips: rows [0, ztop) and [zbot, nz]
     int tn = 0;                        // entries per profile table
     double cp_max = 0;
     int cur = 0;
@@ -242,17 +243,27 @@ poe2d_status upload_tables(poe2d_solver *s)
     return POE2D_OK;
 }
 
-// Strip map of one axis: index into the strip arrays where any profile deviates from the
-// no-absorption values (a = 0, K = 1), else -1.  t = {a, b, K, a_half, b_half, K_half}.
-std::vector<int> strip_map(const std::vector<double> *t, int n, int *count)
+// Strip map of one axis: index into the strip arrays of every point inside the absorbing strips,
+// else -1.  A point is active where any profile deviates from the no-absorption values (a = 0,
+// K = 1); each strip is the contiguous hull of the active points of its half of the axis, so the
+// map is [0, lo) -> 0..lo-1 and [hi, n) -> lo..lo+n-hi-1.  Points of the hull that are not active
+// have a = 0 and K = 1: their memory variable stays zero and their derivative is returned
+// unchanged, so the hull only changes which points carry a (zero) memory variable.
+// t = {a, b, K, a_half, b_half, K_half}.
+std::vector<int> strip_map(const std::vector<double> *t, int n, int *count, int *lo, int *hi)
 {
+    auto active = [&](int i) { return t[0][i] != 0.0 || t[3][i] != 0.0 || t[2][i] != 1.0 || t[5][i] != 1.0; };
+    int l = 0, h = n;
+    for (int i = 0; i < n / 2; ++i)
+        if (active(i)) l = i + 1;
+    for (int i = n - 1; i >= n / 2; --i)
+        if (active(i)) h = i;
     std::vector<int> map(n, -1);
-    int c = 0;
-    for (int i = 0; i < n; ++i) {
-        const bool active = t[0][i] != 0.0 || t[3][i] != 0.0 || t[2][i] != 1.0 || t[5][i] != 1.0;
-        if (active) map[i] = c++;
-    }
-    *count = c;
+    for (int i = 0; i < l; ++i) map[i] = i;
+    for (int i = h; i < n; ++i) map[i] = l + (i - h);
+    *count = l + (n - h);
+    *lo = l;
+    *hi = h;
     return map;
 }
 
@@ -281,6 +292,8 @@ StepArgs<T> step_args(poe2d_solver *s, int in, int out)
     A.tab = (const T *)s->tab;
     A.xmap = s->xmap;
     A.zmap = s->zmap;
+    A.ztop = s->ztop;
+    A.zbot = s->zbot;
     A.mid = s->mid;
     A.pal = (const Vec4<T> *)s->pal;
     A.npal = s->npal;
@@ -485,8 +498,9 @@ poe2d_status poe2d_create(const poe2d_grid *g, const poe2d_coeffs *c, poe2d_solv
                                              t[3].data(), t[4].data(), t[5].data());
         if (st != POE2D_OK) return bail(st);
     }
-    std::vector<int> xm = strip_map(s->htab + TB_AX, g->nx + 1, &s->wxs);
-    std::vector<int> zm = strip_map(s->htab + TB_AZ, g->nz + 1, &s->wzs);
+    int xlo = 0, xhi = 0;
+    std::vector<int> xm = strip_map(s->htab + TB_AX, g->nx + 1, &s->wxs, &xlo, &xhi);
+    std::vector<int> zm = strip_map(s->htab + TB_AZ, g->nz + 1, &s->wzs, &s->ztop, &s->zbot);
     s->tn = std::max(g->nx, g->nz) + 1;
 
     s->pitch = (((size_t)kPadCols + g->nx + 2 + 63) / 64) * 64;

@@ -29,7 +29,7 @@ enum { G_MS = 0, G_MV = 1, G_V0 = 2, G_S0 = 3, G_V1 = 4, G_S1 = 5, G_I = 6, NGRO
 __host__ __device__ constexpr int group_v(int buf) { return G_V0 + 2 * buf; }
 __host__ __device__ constexpr int group_s(int buf) { return G_S0 + 2 * buf; }
 
-constexpr int kPadRows = 8;  // zero rows above and below the grid
+constexpr int kPadRows = 12;  // zero rows above and below the grid
 constexpr int kPadCols = 4;  // zero columns left of the grid (>= 1 more on the right)
 
 // C-PML memory variables.  Each strip point carries a record of 6 values per state buffer.
@@ -74,6 +74,7 @@ struct StepArgs {
     T *psi;                  // memory variables
     const T *tab;            // profile tables
     const int *xmap, *zmap;  // strip index of each column / row, -1 outside the strips
+    int ztop, zbot;          // the z strips are rows [0, ztop) and [zbot, nz] (see zstrip)
     // material palette: when the grid holds at most kMaxPalette distinct coefficient tuples, a
     // one-byte index per point (laid out like a group, one byte per point) selects the MS entry
     // pal[i] and the MV entry pal[kMaxPalette + i]; npal = 0 when the grid has no palette
@@ -106,6 +107,13 @@ template <class T>
 __device__ __forceinline__ T *psi_z(const StepArgs<T> &A, int buf, long long r)
 {
     return A.psi + 2LL * NPSI_X * A.pxn + ((long long)buf * A.pzn + r) * NPSI_Z;
+}
+
+// z-strip index of row z (uniform arithmetic, equal to zmap[z]), -1 outside the strips.
+template <class T>
+__device__ __forceinline__ int zstrip(const StepArgs<T> &A, int z)
+{
+    return z < 0 ? -1 : z < A.ztop ? z : z < A.zbot ? -1 : z <= A.nz ? A.ztop + (z - A.zbot) : -1;
 }
 
 // Fourth-order staggered difference (27 (f1 - f0) - (f2 - fm)) / (24 h) between f0 and f1, with
