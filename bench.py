@@ -1,14 +1,22 @@
 """Benchmark: FP32 Biot C-PML stepping on configuration 1 (4001 x 4001 points, patchy gas 0.15
 over brine rock, 6000 steps, 801 receivers x (vx, vy, p)) on one MI355X.
 
-A "step" of this benchmark is one full solve (nt = 6000 time steps) with the inputs resident in
-HBM; set-up, uploads and trace download are outside the timed region.  value = Gcell-updates/s =
-4001^2 * 6000 * K / (time of K solves).  The roofline figure uses the algorithmic traffic of the
-time step: 104 B per point-update (9 state fields read + written, 8 coefficients read, FP32) plus
-48 B per C-PML strip point and 96 B per corner point, divided by the device time per time step
-(hipEvent on the solver's stream).  The CPU baseline is the oracle in its reference-structured
-form (five sweeps plus three full-grid receiver scans per step, one thread) timed on a bounded
-sample of the same model.
+A "step" of this benchmark is one full solve (nt = 6000 time steps, receiver gather included)
+with the inputs resident in HBM; set-up, uploads, the tile autotuning and the trace download are
+outside the timed region.  value = Gcell-updates/s = 4001^2 * 6000 * K / (device time of K solves,
+hipEvents on the solver's stream).
+
+Roofline: the dominant kernel is the single-pass step kernel, one launch per time step.  Its
+algorithmic traffic is 104 B per point-update (9 state fields read + written, 8 coefficients read,
+FP32) plus 48 B per C-PML strip point and 96 B per corner point; `achieved` divides that by the
+kernel's average launch duration, timed with hipEvents over 200 back-to-back launches on the
+solver's stream (a handle without receivers, so the stream holds only step launches).  `traffic`
+is the measured HBM bytes per launch (rocprofv3 FETCH_SIZE and WRITE_SIZE, each scaled by what the
+same counter reads for a plain copy of known size), read from the committed profile summary when
+it was taken with the same tile shape, else null.
+
+The CPU baseline is the oracle in its reference-structured form (five sweeps plus three full-grid
+receiver scans per step, one thread) timed on a bounded sample of the same model.
 """
 import argparse
 import json
@@ -48,6 +56,40 @@ def strip_counts(prob):
         else:
             rows = int(active.sum())
     return cols, rows
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fused_step_4001x4001_traffic.txt")
+
+
+def recorded_traffic(tile, n):
+    """HBM bytes per step launch from the committed PMC summary (lines "key: value"), if it was
+    measured on this grid with this tile shape."""
+    try:
+        kv = dict(line.split(":", 1) for line in open(TRAFFIC_FILE) if ":" in line and not line.startswith("#"))
+    except OSError:
+        return None, None
+    kv = {k.strip(): v.strip() for k, v in kv.items()}
+    try:
+        if int(kv["grid_n"]) != n + 1 or kv["tile"] != f"{tile[0]}x{tile[1]}":
+            return None, None
+        return float(kv["hbm_bytes_per_launch"]), os.path.relpath(TRAFFIC_FILE, ROOT)
+    except (KeyError, ValueError):
+        return None, None
+
+
+def time_step_launches(prob, tile, launches=200):
+    """Average duration of one step launch: hipEvents around `launches` back-to-back launches of
+    the step kernel (no receivers, so nothing else is on the stream)."""
+    import dataclasses
+    from poe2d_amd import F32
+    from poe2d_amd.solver import Solver
+    with Solver(dataclasses.replace(prob, receivers=None), precision=F32) as s:
+        s.set_schedule(0, *tile)
+        s.run(20)
+        s.reset()
+        s.run(launches)
+        ms, n = s.last_run_ms()
+    return ms / n
 
 
 def cpu_baseline(prob, steps):
@@ -100,6 +142,9 @@ def main():
     ap.add_argument("--n", type=int, default=4000, help="NX = NZ (grid has n+1 points per side)")
     ap.add_argument("--nt", type=int, default=6000)
     ap.add_argument("--cpu-steps", type=int, default=12, help="time steps of the CPU baseline sample (0: skip)")
+    ap.add_argument("--tile", default="auto", metavar="BWxTH",
+                    help="tile shape of the step kernel (block width x tile rows); 'auto' runs the "
+                         "library's autotuner first")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="instead of the benchmark: time a plain device copy of MIB MiB (torch), the "
                          "reference point for achievable HBM bandwidth and for scaling the FETCH_SIZE / "
@@ -123,6 +168,14 @@ def main():
     bytes_step = algorithmic_bytes_per_step(prob.nx, prob.nz, cols, rows)
 
     solver = Solver(prob, precision=F32)
+    palette = solver.material_palette()
+    if args.tile == "auto":
+        tile = solver.autotune(10)
+        tile_note = "autotuned"
+    else:
+        tile = tuple(int(v) for v in args.tile.lower().split("x"))
+        solver.set_schedule(0, *tile)
+        tile_note = "fixed"
     for _ in range(args.warmup):
         solver.reset()
         solver.run(args.nt)
@@ -141,8 +194,9 @@ def main():
 
     ms_solve = statistics.median(dev_ms)
     gcell = npts * args.nt / (ms_solve / 1e3) / 1e9
-    step_s = ms_solve / 1e3 / args.nt
-    achieved = bytes_step / step_s / 1e9
+    launch_ms = time_step_launches(prob, tile)
+    achieved = bytes_step / (launch_ms / 1e3) / 1e9
+    traffic, traffic_src = recorded_traffic(tile, args.n)
     out = {
         "metric": "Gcell-updates/s (FP32 Biot C-PML time loop, 4001x4001, 6000 steps)",
         "value": round(gcell * 1.0, 3),
@@ -161,12 +215,17 @@ def main():
                    "grid": [prob.nx + 1, prob.nz + 1], "nt": args.nt, "npml": prob.npml,
                    "gas_fraction": prob.meta.get("gas_fraction_realised"),
                    "step": "one full solve", "wall_ms_median": round(statistics.median(wall_ms), 3),
-                   "traces_finite": finite},
+                   "traces_finite": finite, "tile": f"{tile[0]}x{tile[1]} ({tile_note})",
+                   "material_palette": palette},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None if traffic is None else round(traffic),
+                     "kernel": "fused_step (one launch per time step)",
+                     "launch_us": round(launch_ms * 1e3, 2),
                      "per_launch": f"one time step: {bytes_step / 1e9:.4f} GB algorithmic "
                                    f"(104 B x {npts} points + 48 B x strip points, {cols} strip columns, "
-                                   f"{rows} strip rows)"},
+                                   f"{rows} strip rows)",
+                     "traffic_source": traffic_src},
     }
     if args.cpu_steps > 0:
         from oracle.pyoracle import material_biot

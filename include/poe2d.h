@@ -142,7 +142,7 @@ poe2d_status poe2d_set_receivers(poe2d_solver *s, const poe2d_receivers *rec);
 /* Choose how a time step is executed on this handle (no counterpart in the reference, whose
  * six launches per step are fixed at seismic_CPML_2D_poroelastic_withCUDA.cu:1628-1634).
  *   schedule 0 (default): one fused launch per step; tiles of block_w - 8 columns (block_w =
- *     128, 256 or 512 threads, default 256) by tile_rows rows (default 64).  When the grid holds
+ *     128, 256 or 512 threads, default 256) by tile_rows rows (default 14; see poe2d_autotune).  When the grid holds
  *     at most 256 distinct coefficient tuples (see poe2d_material_palette) the coefficients are
  *     read through a one-byte material index per point.
  *   schedule 1: a stress launch and a velocity launch per step, intermediates through memory.
@@ -187,6 +187,14 @@ poe2d_status poe2d_restore(poe2d_solver *s, const void *blob, size_t bytes);
 /* Zero the wavefield, C-PML memory, traces and step counter.  Replaces zero_kernel
  * (:387-422, called at :1736). */
 poe2d_status poe2d_reset(poe2d_solver *s);
+
+/* Pick the fastest tile shape of the single-pass schedule for this handle's grid: times
+ * nsteps steps (after one warm-up step) for each block width in {128, 256, 512} and tile height
+ * in {8, 14, 20, 26, 32, 44, 62, 98}, keeps the fastest and resets the handle.  Needs step
+ * count 0 (POE2D_E_STATE otherwise); nsteps <= 0 means 10.  The shape changes speed only, never
+ * a result.  No counterpart in the reference, whose launch shape is fixed
+ * (seismic_CPML_2D_poroelastic_withCUDA.cu:1620-1634). */
+poe2d_status poe2d_autotune(poe2d_solver *s, int nsteps, int *block_w, int *tile_rows);
 
 void poe2d_destroy(poe2d_solver *s);
 
@@ -247,6 +255,47 @@ poe2d_status poe2d_patchy_mask(int nx, int nz, double dx, double dz, double gas_
 /* One of the six disk lattices of patchymodel.m:13-42 and the per-type blocks after it
  * (radius r points, rows x cols layer, 1 inside a disk).  type = 1..6. */
 poe2d_status poe2d_patchy_lattice(int type, int rows, int cols, int r, uint8_t *layer);
+
+/* ---- Reference-format input and output (host only; no device needed) ---- */
+
+/* Layout of a float32 model grid file: x fastest (arr2bin.m:4-9 applied to a z-by-x matrix, the
+ * order the reference's fread loops expect, seismic_CPML_2D_poroelastic_withCUDA.cu:603-640), or
+ * z fastest (patchymodel.m:48-50, which reshapes the matrix column-major before arr2bin). */
+#define POE2D_X_FASTEST 0
+#define POE2D_Z_FASTEST 1
+
+/* input1.txt: NSTEP, ISOURCE, JSOURCE, a0_pos, a1_pos as whitespace-separated integers, read in
+ * that order (seismic_CPML_2D_poroelastic_withCUDA.cu:472-477, 525, 752-762).  POE2D_E_IO when
+ * the file is missing or short. */
+poe2d_status poe2d_read_input1(const char *path, int *nstep, int *isource, int *jsource,
+                               int *a0_pos, int *a1_pos);
+
+/* input2.txt: the source frequency f0, read as a float (:479-484, 827-836). */
+poe2d_status poe2d_read_input2(const char *path, double *f0);
+
+/* An (nx1 = NX+1) x (nz1 = NZ+1) float32 little-endian grid into out (x fastest).  The file must
+ * hold exactly nx1 * nz1 values (the reference freads that many without checking, :603-640). */
+poe2d_status poe2d_read_grid_f32(const char *path, int nx1, int nz1, int layout, float *out);
+
+/* A full-grid float32 snapshot, x fastest: the vz%05i.vzo / vx%05i.vxo files (:1648-1661,
+ * 1700-1713). */
+poe2d_status poe2d_write_grid_f32(const char *path, int nx1, int nz1, const float *data);
+
+/* ASCII traces as the reference writes them with default stream formatting (6 significant
+ * digits), each value followed by a space: a table of nrows lines of ncols values each ending in
+ * a newline (.vsp / .hor, :1749-1805), or with one_line a single line without newline (.ats /
+ * .bwh / .src, :1846-1860). */
+poe2d_status poe2d_write_traces(const char *path, int nrows, int ncols, const float *data, int one_line);
+
+/* The reference's receiver layout (save1_kernel, :114-201): the VSP line at column ISOURCE, rows
+ * y = m (NZ+1)/panj - 1, m = 1..panj (n_vsp positions, ending at the grid's last row); the
+ * horizontal line at row 10, columns x = m (NX+1)/panj2 - 1, m = 1..panj2 (n_hor); and the point
+ * traces at column (NX+1)/2 - 1, rows a0_pos, a1_pos, JSOURCE (n_pts = 3), written to ix/iz in
+ * that order.  cap = 0 only returns the counts.  The reference records v_z and v_x on both lines
+ * and v_z at the points, sample k holding the state at the start of step k. */
+poe2d_status poe2d_reference_receivers(int nx, int nz, int isource, int jsource, int a0_pos,
+                                       int a1_pos, int panj, int panj2, int cap, int *ix, int *iz,
+                                       int *n_vsp, int *n_hor, int *n_pts);
 
 #ifdef __cplusplus
 }

@@ -111,6 +111,13 @@ def _declare(lib):
         "poe2d_run": ([vp, i], i),
         "poe2d_set_schedule": ([vp, i, i, i], i),
         "poe2d_material_palette": ([vp, c_int_p], i),
+        "poe2d_autotune": ([vp, i, c_int_p, c_int_p], i),
+        "poe2d_read_input1": ([ctypes.c_char_p, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p], i),
+        "poe2d_read_input2": ([ctypes.c_char_p, c_double_p], i),
+        "poe2d_read_grid_f32": ([ctypes.c_char_p, i, i, i, c_float_p], i),
+        "poe2d_write_grid_f32": ([ctypes.c_char_p, i, i, c_float_p], i),
+        "poe2d_write_traces": ([ctypes.c_char_p, i, i, c_float_p, i], i),
+        "poe2d_reference_receivers": ([i, i, i, i, i, i, i, i, i, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p], i),
         "poe2d_last_run_ms": ([vp, c_double_p, c_int_p], i),
         "poe2d_step_count": ([vp, c_int_p], i),
         "poe2d_seismograms": ([vp, vp, sz], i),

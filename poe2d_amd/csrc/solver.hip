@@ -73,7 +73,7 @@ struct poe2d_solver {
     std::vector<double> htab[NTABLES];
     // schedule
     int schedule = SCHED_FUSED;
-    int block_w = 256, tile_rows = 64;
+    int block_w = 256, tile_rows = 14;
     // source
     bool have_src = false;
     poe2d_source src{};
@@ -751,6 +751,40 @@ poe2d_status poe2d_reset(poe2d_solver *s)
     HIP_TRY(hipStreamSynchronize(s->stream));
     s->step = 0;
     s->cur = 0;
+    return POE2D_OK;
+}
+
+poe2d_status poe2d_autotune(poe2d_solver *s, int nsteps, int *block_w, int *tile_rows)
+{
+    poe2d_status st = check_handle(s);
+    if (st) return st;
+    if (s->schedule == SCHED_TWOPASS) return fail(POE2D_E_INVALID, "autotune: the two-pass schedule has no tile shape");
+    if (s->step != 0) return fail(POE2D_E_STATE, "autotune: needs a handle at step 0 (call poe2d_reset first)");
+    if (nsteps <= 0) nsteps = 10;
+    // Tile heights of 6k + 2 rows make whole groups of the kernel's 6-step row loop.
+    static const int widths[] = {128, 256, 512};
+    static const int heights[] = {8, 14, 20, 26, 32, 44, 62, 98};
+    double best = 0;
+    int best_w = s->block_w, best_h = s->tile_rows;
+    for (int w : widths)
+        for (int h : heights) {
+            if (h > s->grid.nz + 1 && h != heights[0]) continue;
+            s->block_w = w;
+            s->tile_rows = h;
+            if ((st = poe2d_run(s, 1)) || (st = poe2d_run(s, nsteps))) break;
+            if (best == 0 || s->last_ms < best) {
+                best = s->last_ms;
+                best_w = w;
+                best_h = h;
+            }
+        }
+    s->block_w = best_w;
+    s->tile_rows = best_h;
+    const poe2d_status rst = poe2d_reset(s);
+    if (st) return st;
+    if (rst) return rst;
+    if (block_w) *block_w = best_w;
+    if (tile_rows) *tile_rows = best_h;
     return POE2D_OK;
 }
 

@@ -201,12 +201,19 @@ class Solver(HandleSolver):
     def reset(self):
         self._check(self._lib.poe2d_reset(self._h), "reset")
 
-    def set_schedule(self, schedule=0, block_w=256, tile_rows=64):
+    def set_schedule(self, schedule=0, block_w=256, tile_rows=14):
         """0: single fused launch per step (tiles of block_w-8 columns x tile_rows rows), reading
         the coefficients through the material palette when the grid has one;
         1: separate stress and velocity launches;
         2: as 0, always reading the per-point coefficient arrays."""
         self._check(self._lib.poe2d_set_schedule(self._h, schedule, block_w, tile_rows), "set_schedule")
+
+    def autotune(self, nsteps=10):
+        """Time the single-pass tile shapes on this grid, keep the fastest, reset the handle;
+        returns (block_w, tile_rows)."""
+        w, h = ctypes.c_int(), ctypes.c_int()
+        self._check(self._lib.poe2d_autotune(self._h, nsteps, ctypes.byref(w), ctypes.byref(h)), "autotune")
+        return w.value, h.value
 
     def material_palette(self):
         """Distinct coefficient tuples of the grid when at most 256, else 0."""

@@ -308,6 +308,30 @@ def test_parity_graded_medium_without_palette():
     assert_parity(run_gpu(prob), prob, "graded medium")
 
 
+def test_autotune_picks_a_shape_and_resets():
+    """autotune leaves the handle at step 0 with a valid shape, whose results equal a run with the
+    same shape set by hand; on a handle past step 0 it refuses."""
+    from poe2d_amd.material import material_biot
+    n, nt = 200, 300
+    coeffs = material_biot(**biot_fields(n, n, random_patchy_mask(n, n, 0.2, seed=9)))
+    rec = ReceiverSpec(ix=np.arange(30, 171, 7, dtype=np.int32), iz=np.full(21, 140, np.int32))
+    prob = Problem(nx=n, nz=n, dx=0.5, dz=0.5, dt=5e-5, nt=nt, coeffs=coeffs,
+                   source=SourceSpec(kind=_abi.SRC_FORCE_Y_RICKER, ix=100, iz=90, amp=1e10),
+                   receivers=rec, npml=20, f0=80.0)
+    with Solver(prob) as s:
+        bw, th = s.autotune(3)
+        say(f"autotune {n + 1}^2: block {bw}, tile rows {th}")
+        assert bw in (128, 256, 512) and th in (8, 14, 20, 26, 32, 44, 62, 98)
+        assert s.step_count() == 0
+        s.run()
+        tuned = s.seismograms()
+        with pytest.raises(_abi.Poe2dError) as ei:
+            s.autotune(3)
+        assert ei.value.status == _abi.POE2D_E_STATE
+    manual = _run_sched(prob, 0, bw, th)[0]
+    assert np.array_equal(tuned, manual)
+
+
 @pytest.mark.parametrize("case", ["pressure_patchy", "force_patchy"])
 def test_two_pass_schedule_parity(case):
     from poe2d_amd.material import material_biot
