@@ -62,17 +62,18 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fused_step_4001x4001_traffic.txt"
 
 
 def recorded_traffic(tile, n):
-    """HBM bytes per step launch from the committed PMC summary (lines "key: value"), if it was
-    measured on this grid with this tile shape."""
+    """HBM bytes per step launch from the committed PMC summary ("grid_n: N" and
+    "tile BWxTH: bytes" lines), if it was measured on this grid with this tile shape."""
     try:
-        kv = dict(line.split(":", 1) for line in open(TRAFFIC_FILE) if ":" in line and not line.startswith("#"))
+        lines = [ln.strip() for ln in open(TRAFFIC_FILE) if ln.strip() and not ln.startswith("#")]
     except OSError:
         return None, None
+    kv = dict(ln.split(":", 1) for ln in lines if ":" in ln)
     kv = {k.strip(): v.strip() for k, v in kv.items()}
     try:
-        if int(kv["grid_n"]) != n + 1 or kv["tile"] != f"{tile[0]}x{tile[1]}":
+        if int(kv.get("grid_n", -1)) != n + 1:
             return None, None
-        return float(kv["hbm_bytes_per_launch"]), os.path.relpath(TRAFFIC_FILE, ROOT)
+        return float(kv[f"tile {tile[0]}x{tile[1]}"]), os.path.relpath(TRAFFIC_FILE, ROOT)
     except (KeyError, ValueError):
         return None, None
 
@@ -130,8 +131,16 @@ def copy_probe(mib, reps=20):
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / reps
     moved = 2 * n * 4
+    del src, dst
+    import ctypes
+    from poe2d_amd import _abi
+    lib = _abi.load_library()
+    best = [ctypes.c_double() for _ in range(3)]
+    _abi.check(lib, lib.poe2d_stream_probe(mib << 20, 10, *[ctypes.byref(b) for b in best]), "stream_probe")
     print(json.dumps({"copy_probe_mib": mib, "bytes_read_per_copy": n * 4, "bytes_written_per_copy": n * 4,
-                      "ms_per_copy": round(ms, 4), "GBps_read_plus_write": round(moved / ms / 1e6, 1)}), flush=True)
+                      "ms_per_copy": round(ms, 4), "GBps_read_plus_write": round(moved / ms / 1e6, 1),
+                      "stream_probe_GBps": {"copy": round(best[0].value, 1), "read": round(best[1].value, 1),
+                                            "write": round(best[2].value, 1)}}), flush=True)
 
 
 def main():
@@ -146,9 +155,9 @@ def main():
                     help="tile shape of the step kernel (block width x tile rows); 'auto' runs the "
                          "library's autotuner first")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
-                    help="instead of the benchmark: time a plain device copy of MIB MiB (torch), the "
-                         "reference point for achievable HBM bandwidth and for scaling the FETCH_SIZE / "
-                         "WRITE_SIZE counters")
+                    help="instead of the benchmark: time a plain device copy of MIB MiB (torch; the copy "
+                         "that scales the FETCH_SIZE / WRITE_SIZE counters), then the library's stream "
+                         "probe (best copy / read / write rates over MIB MiB buffers)")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)

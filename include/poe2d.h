@@ -256,6 +256,13 @@ poe2d_status poe2d_patchy_mask(int nx, int nz, double dx, double dz, double gas_
  * (radius r points, rows x cols layer, 1 inside a disk).  type = 1..6. */
 poe2d_status poe2d_patchy_lattice(int type, int rows, int cols, int r, uint8_t *layer);
 
+/* Achievable bandwidth of plain streams on the current device (a measurement helper with no
+ * counterpart in the reference): grid-stride float4 copy, read and write kernels over buffers of
+ * `bytes`, best of several grid sizes, each timed over `reps` launches; GB/s counts bytes read
+ * plus bytes written. */
+poe2d_status poe2d_stream_probe(size_t bytes, int reps, double *copy_gbps, double *read_gbps,
+                                double *write_gbps);
+
 /* ---- Reference-format input and output (host only; no device needed) ---- */
 
 /* Layout of a float32 model grid file: x fastest (arr2bin.m:4-9 applied to a z-by-x matrix, the

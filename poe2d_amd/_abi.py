@@ -112,6 +112,7 @@ def _declare(lib):
         "poe2d_set_schedule": ([vp, i, i, i], i),
         "poe2d_material_palette": ([vp, c_int_p], i),
         "poe2d_autotune": ([vp, i, c_int_p, c_int_p], i),
+        "poe2d_stream_probe": ([sz, i, c_double_p, c_double_p, c_double_p], i),
         "poe2d_read_input1": ([ctypes.c_char_p, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p], i),
         "poe2d_read_input2": ([ctypes.c_char_p, c_double_p], i),
         "poe2d_read_grid_f32": ([ctypes.c_char_p, i, i, i, c_float_p], i),
