@@ -117,6 +117,7 @@ enum {
 };
 
 typedef struct poe2d_solver poe2d_solver;
+typedef struct poe2d_batch poe2d_batch;
 
 /* Library version / device probe.  Returns POE2D_OK when a gfx950 device is usable. */
 int poe2d_abi_version(void);
@@ -195,6 +196,35 @@ poe2d_status poe2d_reset(poe2d_solver *s);
  * a result.  No counterpart in the reference, whose launch shape is fixed
  * (seismic_CPML_2D_poroelastic_withCUDA.cu:1620-1634). */
 poe2d_status poe2d_autotune(poe2d_solver *s, int nsteps, int *block_w, int *tile_rows);
+
+/* Launch mode of the single-pass schedule: with graphs on (the default) a run captures 32 steps
+ * (the step kernel and the receiver gather, which also advances the device step counter) in a
+ * HIP graph once and replays it; off,
+ * every step is launched directly.  Results are identical either way.  No counterpart in the
+ * reference, which launches six kernels per step (.cu:1628-1634). */
+poe2d_status poe2d_set_graphs(poe2d_solver *s, int enable);
+
+/* A new handle on the same grid and coefficients as s (shared, read-only), with its own
+ * wavefield, memory variables, source and receivers (none set), starting at step 0 with s's
+ * schedule.  For shots over one model: the reference re-runs its whole program per shot position
+ * (.cu:1615-1736 inside the shot loop), re-reading and re-uploading the model. */
+poe2d_status poe2d_clone(poe2d_solver *s, poe2d_solver **out);
+
+/* ---- Batches: several handles advanced in lock-step by one launch per time step ----
+ * A batch is a view over existing handles (distinct handles, same device and precision,
+ * single-pass schedules, ragged grids allowed); they keep their own results, read with the
+ * handle calls above.  One step of the batch is one launch over a work list of (handle, tile)
+ * items plus one gather launch; with graphs on, 32 steps are captured once and replayed.  Each
+ * member's results equal a run of that handle alone.  No counterpart in the reference, which runs
+ * one model and one shot per process.  batch_run needs every member at the same step. */
+poe2d_status poe2d_batch_create(int n, poe2d_solver *const *members, poe2d_batch **out);
+poe2d_status poe2d_batch_set_schedule(poe2d_batch *b, int block_w, int tile_rows);
+poe2d_status poe2d_batch_set_graphs(poe2d_batch *b, int enable);
+poe2d_status poe2d_batch_run(poe2d_batch *b, int nsteps);
+poe2d_status poe2d_batch_last_run_ms(poe2d_batch *b, double *ms, int *nsteps);
+/* As poe2d_autotune, over the whole batch (every member at step 0; all are reset after). */
+poe2d_status poe2d_batch_autotune(poe2d_batch *b, int nsteps, int *block_w, int *tile_rows);
+void poe2d_batch_destroy(poe2d_batch *b);
 
 void poe2d_destroy(poe2d_solver *s);
 

@@ -112,6 +112,15 @@ def _declare(lib):
         "poe2d_set_schedule": ([vp, i, i, i], i),
         "poe2d_material_palette": ([vp, c_int_p], i),
         "poe2d_autotune": ([vp, i, c_int_p, c_int_p], i),
+        "poe2d_set_graphs": ([vp, i], i),
+        "poe2d_clone": ([vp, ctypes.POINTER(vp)], i),
+        "poe2d_batch_create": ([i, ctypes.POINTER(vp), ctypes.POINTER(vp)], i),
+        "poe2d_batch_set_schedule": ([vp, i, i], i),
+        "poe2d_batch_set_graphs": ([vp, i], i),
+        "poe2d_batch_run": ([vp, i], i),
+        "poe2d_batch_last_run_ms": ([vp, c_double_p, c_int_p], i),
+        "poe2d_batch_autotune": ([vp, i, c_int_p, c_int_p], i),
+        "poe2d_batch_destroy": ([vp], None),
         "poe2d_stream_probe": ([sz, i, c_double_p, c_double_p, c_double_p], i),
         "poe2d_read_input1": ([ctypes.c_char_p, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p], i),
         "poe2d_read_input2": ([ctypes.c_char_p, c_double_p], i),

@@ -1,5 +1,5 @@
-// Small kernels around the time step: receiver gather, derived-field recompute and component
-// extraction for snapshots, and the stability reduction.
+// Small kernels around the time step: the receiver gather of the two-pass schedule, derived-field
+// recompute and component extraction for snapshots, and the stability reduction.
 #include "aux_kernels.h"
 
 namespace poe2d {
@@ -23,11 +23,11 @@ __global__ void gather_kernel(StepArgs<T> A, int vbuf, int sbuf, Receivers R, T 
     if (r >= R.n) return;
     const int x = R.ix[r], z = R.iz[r];
     const long long o = (long long)z * A.pitch + x;
-    const Vec4<T> v = group_c(A, group_v(vbuf))[o];
+    const Vec4<T> v = state_group(A, group_v(vbuf))[o];
     int c = 0;
     if (R.mask & POE2D_VX) out[(size_t)(c++) * R.n + r] = v.x;
     if (R.mask & POE2D_VY) out[(size_t)(c++) * R.n + r] = v.y;
-    if (R.mask & POE2D_P) out[(size_t)c * R.n + r] = pressure_at(A, group_c(A, group_s(sbuf)), group_c(A, G_MS), x, z, o);
+    if (R.mask & POE2D_P) out[(size_t)c * R.n + r] = pressure_at(A, state_group(A, group_s(sbuf)), mat_group(A, G_MS), x, z, o);
 }
 
 template <class T>
@@ -37,8 +37,8 @@ __global__ void derived_kernel(StepArgs<T> A, int buf, int what, T *out)
     const int z = blockIdx.y;
     if (x > A.nx || z > A.nz) return;
     const long long o = (long long)z * A.pitch + x;
-    const Vec4<T> *S = group_c(A, group_s(buf));
-    const Vec4<T> *MS = group_c(A, G_MS);
+    const Vec4<T> *S = state_group(A, group_s(buf));
+    const Vec4<T> *MS = mat_group(A, G_MS);
     const T p = pressure_at(A, S, MS, x, z, o);
     T v = p;
     if (what != 0) {
@@ -58,7 +58,7 @@ __global__ void extract_kernel(StepArgs<T> A, int g, int comp, T *out)
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int z = blockIdx.y;
     if (x > A.nx || z > A.nz) return;
-    const Vec4<T> v = group_c(A, g)[(long long)z * A.pitch + x];
+    const Vec4<T> v = state_group(A, g)[(long long)z * A.pitch + x];
     out[(size_t)z * (A.nx + 1) + x] = comp == 0 ? v.x : comp == 1 ? v.y : comp == 2 ? v.z : v.w;
 }
 
@@ -66,7 +66,7 @@ template <class T>
 __global__ void __launch_bounds__(256) absmax_blocks(StepArgs<T> A, int buf, float *partial)
 {
     __shared__ float red[256];
-    const Vec4<T> *V = group_c(A, group_v(buf));
+    const Vec4<T> *V = state_group(A, group_v(buf));
     const long long npts = (long long)(A.nz + 1) * A.pitch;
     float m = 0.f;
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < npts; i += (long long)gridDim.x * blockDim.x) {

@@ -9,7 +9,9 @@
 //      pore pressure and normal stresses, with their C-PML memory variables), taking x
 //      neighbours through LDS and z neighbours from registers,
 //   3. forms the velocities of row j = s-2 from registers holding stress rows s-4 .. s and the x
-//      neighbours of stress row j, which were put in an LDS ring two row steps earlier.
+//      neighbours of stress row j, which were put in an LDS ring two row steps earlier;
+//   4. on recorded steps, gathers the receivers of row s-1 from the LDS rings, which still hold
+//      that row's old velocities and new pore pressure.
 // Each point's state vectors (V, S) are read with one 16-byte load each, issued one row step
 // before use, and written with one 16-byte store each; every access is a uniform group base plus
 // one 32-bit per-lane offset.  The coefficients come either from the MS and MV groups (one 16-byte
@@ -23,9 +25,13 @@
 // that issued it).
 //
 // Stresses of the 2 halo columns and 2 halo rows on each side are recomputed from the unmodified
-// old state (double buffering) instead of being exchanged.  One barrier per row step.  Tiles are
-// mapped so that consecutive tiles of a row band land on the same XCD (blocks are dispatched
-// round-robin over the 8 XCDs), keeping halo re-reads inside one L2.
+// old state (double buffering) instead of being exchanged.  One barrier per row step.
+//
+// Work list.  One launch advances every model of a batch (a single model is a batch of one): each
+// block takes a (model, tile) item, reads that model's arguments from device memory and the step
+// index from a device counter, so a captured sequence of launches replays for any step.  Items are
+// placed so that consecutive items (neighbouring tiles of a row band) land on the same XCD (blocks
+// are dispatched round-robin over the 8 XCDs), keeping halo re-reads inside one L2.
 #include <cstdint>
 #include <type_traits>
 
@@ -34,16 +40,25 @@
 namespace poe2d {
 
 template <class T, int BW, bool PAL>
-__global__ void __launch_bounds__(BW) fused_step(StepArgs<T> A)
+__global__ void __launch_bounds__(BW)
+    fused_step(const StepArgs<T> *__restrict__ models, const WorkItem *__restrict__ work, int nwork, int per_xcd,
+               const long long *__restrict__ base, int k)
 {
     constexpr int W = BW - 8;
     constexpr uint32_t VB = sizeof(Vec4<T>);
-    __shared__ T lv[2][3][BW + 4];  // ring of 2 velocity rows: vx, wx, vy   (entry lane + 2)
+    typedef __attribute__((address_space(1))) nvec4<T> GN;
+    typedef __attribute__((address_space(1))) T GT;
+    __shared__ T lv[3][3][BW + 4];  // ring of 3 velocity rows: vx, wx, vy   (entry lane + 2)
     __shared__ T ls[3][3][BW + 4];  // ring of 3 stress rows:   sxx, p, sxy
     __shared__ Vec4<T> lpal[PAL ? 2 * kMaxPalette : 1];  // MS entries, then MV entries
 
-    const int tile = (int)(blockIdx.x & 7) * A.per_xcd + (int)(blockIdx.x >> 3);
-    if (tile >= A.ntiles) return;
+    const int wi = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+    if (wi >= nwork) return;
+    const WorkItem item = work[wi];
+    StepArgs<T> A = models[item.model];
+    const long long n = *(const kcll *)base + k;
+    at_step(A, n);
+    const int tile = item.tile;
     const int tx = tile % A.ntx;
     const int tz = tile / A.ntx;
     const int lane = threadIdx.x;
@@ -59,34 +74,50 @@ __global__ void __launch_bounds__(BW) fused_step(StepArgs<T> A)
     // at ((z + kPadRows) * P + x + kPadCols) * VB.  Columns left of the grid (c >= -4) are zero pads
     // in every group, and so is column nx+1; lanes right of it read column nx+1, so every load is
     // unconditional.
-    const char *const gb = (const char *)A.grp;
+    gchar *const gb = (gchar *)A.grp;
+    const gchar *const mb = (const gchar *)A.mat;
     const long long gsb = A.group_stride * (long long)VB;
-    const char *MS = gb + G_MS * gsb;
-    const char *MV = gb + G_MV * gsb;
-    const char *Vo = gb + group_v(A.in) * gsb;
-    const char *So = gb + group_s(A.in) * gsb;
-    char *Vn = (char *)gb + group_v(A.out) * gsb;
-    char *Sn = (char *)gb + group_s(A.out) * gsb;
+    const gchar *MS = mb + G_MS * gsb;
+    const gchar *MV = mb + G_MV * gsb;
+    const gchar *Vo = gb + group_v(A.in) * gsb;
+    const gchar *So = gb + group_s(A.in) * gsb;
+    gchar *Vn = gb + group_v(A.out) * gsb;
+    gchar *Sn = gb + group_s(A.out) * gsb;
+    const GT *const tab = (const GT *)A.tab;
+    GT *const psi = (GT *)A.psi;
+    // memory-variable records of x-strip / z-strip record r in state buffer b
+    auto px = [&](int b, long long r) -> GT * { return psi + ((long long)b * A.pxn + r) * NPSI_X; };
+    auto pz = [&](int b, long long r) -> GT * {
+        return psi + 2LL * NPSI_X * A.pxn + ((long long)b * A.pzn + r) * NPSI_Z;
+    };
     const uint32_t colb = (uint32_t)(min(c, nx + 1) + kPadCols) * VB;
     const uint32_t rowb = (uint32_t)P * VB;
     auto off = [&](int row) -> uint32_t { return colb + (uint32_t)(row + kPadRows) * rowb; };
-    auto ld = [&](const char *g, int row) -> Vec4<T> { return *(const Vec4<T> *)(g + off(row)); };
+    auto ld = [&](const gchar *g, int row) -> Vec4<T> {
+        const nvec4<T> v = *(const GN *)(g + off(row));
+        return Vec4<T>{v.x, v.y, v.z, v.w};
+    };
     const Vec4<T> zero4 = {T(0), T(0), T(0), T(0)};
     // palette mode: material index of a row (same pads as the groups, one byte per point)
     const uint32_t colm = (uint32_t)(min(c, nx + 1) + kPadCols);
-    auto ldm = [&](int row) -> int { return A.mid[colm + (uint32_t)(row + kPadRows) * (uint32_t)P]; };
+    gcuchar *const mid = (gcuchar *)A.mid;
+    auto ldm = [&](int row) -> int { return mid[colm + (uint32_t)(row + kPadRows) * (uint32_t)P]; };
     if constexpr (PAL) {
+        const GN *const pal = (const GN *)A.pal;
         for (int i = lane; i < 2 * kMaxPalette; i += BW)
-            if ((i & (kMaxPalette - 1)) < A.npal) lpal[i] = A.pal[i];
+            if ((i & (kMaxPalette - 1)) < A.npal) {
+                const nvec4<T> e = pal[i];
+                lpal[i] = Vec4<T>{e.x, e.y, e.z, e.w};
+            }
         __syncthreads();
     }
 
     // per-column C-PML profile (x strips only)
-    const int sx = (c >= 0 && c <= nx) ? A.xmap[c] : -1;
+    const int sx = (c >= 0 && c <= nx) ? ((gcint *)A.xmap)[c] : -1;
     T ax = 0, bx = 1, rkx = 1, axh = 0, bxh = 1, rkxh = 1;
     if (sx >= 0) {
-        ax = A.tab[TB_AX * A.tn + c]; bx = A.tab[TB_BX * A.tn + c]; rkx = A.tab[TB_RKX * A.tn + c];
-        axh = A.tab[TB_AXH * A.tn + c]; bxh = A.tab[TB_BXH * A.tn + c]; rkxh = A.tab[TB_RKXH * A.tn + c];
+        ax = tab[TB_AX * A.tn + c]; bx = tab[TB_BX * A.tn + c]; rkx = tab[TB_RKX * A.tn + c];
+        axh = tab[TB_AXH * A.tn + c]; bxh = tab[TB_BXH * A.tn + c]; rkxh = tab[TB_RKXH * A.tn + c];
     }
     const bool xr1 = c >= 1 && c < nx;      // stress1
     const bool xr2 = c >= 2 && c < nx;      // stress2
@@ -94,6 +125,14 @@ __global__ void __launch_bounds__(BW) fused_step(StepArgs<T> A)
     const bool xr4 = c >= 2 && c < nx;      // vx
     const bool xr5 = c >= 1 && c < nx - 1;  // vz
     const bool src_p = A.src_kind == POE2D_SRC_PRESSURE_DGAUSS && c == A.src_x;
+    // receivers of this step: slot and the tile's entries per row
+    const int n32 = (int)n;
+    kcint *const rec_ptr = (kcint *)A.rec_tile_ptr + (long long)tx * (nz + 1);  // offsets of this tile column
+    const bool rec_on = A.rec_n > 0 && n32 % A.rec_dec == 0 && n32 / A.rec_dec < A.rec_nsamp &&
+                        rec_ptr[zA] < rec_ptr[zB];  // receivers in this tile
+    GT *const trace = (GT *)A.traces + (rec_on ? (long long)(n32 / A.rec_dec) * A.rec_ncomp * A.rec_n : 0);
+    typedef int nint2 __attribute__((ext_vector_type(2)));
+    const __attribute__((address_space(1))) nint2 *const rec_ent = (const __attribute__((address_space(1))) nint2 *)A.rec_entries;
     const bool src_f = A.src_kind == POE2D_SRC_FORCE_Y_RICKER && c == A.src_x;
 
     // Register rings.  Row r of a ring of size R sits in slot (r - s0) mod R, s0 = zA - 2 being the
@@ -162,11 +201,26 @@ __global__ void __launch_bounds__(BW) fused_step(StepArgs<T> A)
         const bool zr1 = s >= 2 && s < nz - 1;
         const bool zr2 = s >= 1 && s < nz - 1;
         const int szs = zstrip(A, s);
-        T(*xv)[BW + 4] = lv[PH & 1];
+        T(*xv)[BW + 4] = lv[PH % 3];
         xv[0][lane + 2] = V(0).x;
         xv[1][lane + 2] = V(0).z;
         xv[2][lane + 2] = V(0).y;
         __syncthreads();
+
+        // ---- receivers of row s-1: its old velocities and new pore pressure are in the ring
+        // entries written one row step ago (visible after this step's barrier; the velocity ring
+        // has 3 entries, so the next step's write goes to row s-2's entry, not this one)
+        if (rec_on && s - 1 >= zA && s - 1 < zB) {
+            const int lo = rec_ptr[s - 1], hi = rec_ptr[s];
+            for (int e = lo + lane; e < hi; e += BW) {
+                const nint2 en = rec_ent[e];
+                const int li = en.x - tx * W + 6;  // LDS entry of column en.x
+                int q = 0;
+                if (A.rec_mask & POE2D_VX) trace[(long long)(q++) * A.rec_n + en.y] = lv[(PH + 2) % 3][0][li];
+                if (A.rec_mask & POE2D_VY) trace[(long long)(q++) * A.rec_n + en.y] = lv[(PH + 2) % 3][2][li];
+                if (A.rec_mask & POE2D_P) trace[(long long)q * A.rec_n + en.y] = ls[(PH + 2) % 3][1][li];
+            }
+        }
 
         const bool store_s = own && srow;
         T exx_n = so.x, eyy_n = so.y, zeta_n = so.z, sxy_n = so.w;
@@ -178,16 +232,16 @@ __global__ void __launch_bounds__(BW) fused_step(StepArgs<T> A)
             T dwz = d4(V(0).w, V(-1).w, V(1).w, V(-2).w, rdz);
             if (sx >= 0) {
                 const long long r = (long long)s * A.wxs + sx;
-                const T *po = psi_x(A, A.in, r);
-                T *pn = psi_x(A, A.out, r);
+                const GT *po = px(A.in, r);
+                GT *pn = px(A.out, r);
                 dvx = cpml(dvx, po + PX_DVX, pn + PX_DVX, store_s, axh, bxh, rkxh);
                 dwx = cpml(dwx, po + PX_DWX, pn + PX_DWX, store_s, axh, bxh, rkxh);
             }
             if (szs >= 0) {
                 const long long r = (long long)szs * P + c;
-                const T *po = psi_z(A, A.in, r);
-                T *pn = psi_z(A, A.out, r);
-                const T az = A.tab[TB_AZ * A.tn + s], bz = A.tab[TB_BZ * A.tn + s], rkz = A.tab[TB_RKZ * A.tn + s];
+                const GT *po = pz(A.in, r);
+                GT *pn = pz(A.out, r);
+                const T az = tab[TB_AZ * A.tn + s], bz = tab[TB_BZ * A.tn + s], rkz = tab[TB_RKZ * A.tn + s];
                 dvz = cpml(dvz, po + PZ_DVZ, pn + PZ_DVZ, store_s, az, bz, rkz);
                 dwz = cpml(dwz, po + PZ_DWZ, pn + PZ_DWZ, store_s, az, bz, rkz);
             }
@@ -208,17 +262,17 @@ __global__ void __launch_bounds__(BW) fused_step(StepArgs<T> A)
             T ddz = d4(V(1).x, V(0).x, V(2).x, V(-1).x, rdz);
             if (sx >= 0) {
                 const long long r = (long long)s * A.wxs + sx;
-                ddx = cpml(ddx, psi_x(A, A.in, r) + PX_DVY, psi_x(A, A.out, r) + PX_DVY, store_s, ax, bx, rkx);
+                ddx = cpml(ddx, px(A.in, r) + PX_DVY, px(A.out, r) + PX_DVY, store_s, ax, bx, rkx);
             }
             if (szs >= 0) {
                 // half-grid memory coefficients, full-grid K_z divisor (reference :224-226)
                 const long long r = (long long)szs * P + c;
-                ddz = cpml(ddz, psi_z(A, A.in, r) + PZ_DVX, psi_z(A, A.out, r) + PZ_DVX, store_s,
-                           A.tab[TB_AZH * A.tn + s], A.tab[TB_BZH * A.tn + s], A.tab[TB_RKZ * A.tn + s]);
+                ddz = cpml(ddz, pz(A.in, r) + PZ_DVX, pz(A.out, r) + PZ_DVX, store_s,
+                           tab[TB_AZH * A.tn + s], tab[TB_BZH * A.tn + s], tab[TB_RKZ * A.tn + s]);
             }
             sxy_n = so.w + ms_nxt.w * (ddx + ddz) * dt;  // mu of point (x, z+1)
         }
-        if (store_s) *(Vec4<T> *)(Sn + off(s)) = Vec4<T>{exx_n, eyy_n, zeta_n, sxy_n};
+        if (store_s) *(GN *)(Sn + off(s)) = nvec4<T>{exx_n, eyy_n, zeta_n, sxy_n};
         {
             T(*xs)[BW + 4] = ls[PH % 3];
             xs[0][lane + 2] = sxx;
@@ -244,15 +298,15 @@ __global__ void __launch_bounds__(BW) fused_step(StepArgs<T> A)
                 T e3 = d4(SXY(-2), SXY(-3), SXY(-1), SXY(-4), rdz);
                 if (sx >= 0) {
                     const long long r = (long long)j * A.wxs + sx;
-                    const T *po = psi_x(A, A.in, r);
-                    T *pn = psi_x(A, A.out, r);
+                    const GT *po = px(A.in, r);
+                    GT *pn = px(A.out, r);
                     e1 = cpml(e1, po + PX_DSXX, pn + PX_DSXX, true, ax, bx, rkx);
                     e2 = cpml(e2, po + PX_DPX, pn + PX_DPX, true, ax, bx, rkx);
                 }
                 if (szj >= 0) {
                     const long long r = (long long)szj * P + c;
-                    e3 = cpml(e3, psi_z(A, A.in, r) + PZ_DSXY, psi_z(A, A.out, r) + PZ_DSXY, true,
-                              A.tab[TB_AZ * A.tn + j], A.tab[TB_BZ * A.tn + j], A.tab[TB_RKZ * A.tn + j]);
+                    e3 = cpml(e3, pz(A.in, r) + PZ_DSXY, pz(A.out, r) + PZ_DSXY, true,
+                              tab[TB_AZ * A.tn + j], tab[TB_BZ * A.tn + j], tab[TB_RKZ * A.tn + j]);
                 }
                 biot_velocity(v.x, v.z, e1 + e3, e2, mv_j.x, mv_j.y, mv_j.z, mv_j.w, A.inv_dt);
             }
@@ -262,14 +316,14 @@ __global__ void __launch_bounds__(BW) fused_step(StepArgs<T> A)
                 T e3 = d4(PP(-1), PP(-2), PP(0), PP(-3), rdz);
                 if (sx >= 0) {
                     const long long r = (long long)j * A.wxs + sx;
-                    e1 = cpml(e1, psi_x(A, A.in, r) + PX_DSXY, psi_x(A, A.out, r) + PX_DSXY, true, axh, bxh, rkxh);
+                    e1 = cpml(e1, px(A.in, r) + PX_DSXY, px(A.out, r) + PX_DSXY, true, axh, bxh, rkxh);
                 }
                 if (szj >= 0) {
                     const long long r = (long long)szj * P + c;
-                    const T *po = psi_z(A, A.in, r);
-                    T *pn = psi_z(A, A.out, r);
-                    const T azh = A.tab[TB_AZH * A.tn + j], bzh = A.tab[TB_BZH * A.tn + j];
-                    const T rkzh = A.tab[TB_RKZH * A.tn + j];
+                    const GT *po = pz(A.in, r);
+                    GT *pn = pz(A.out, r);
+                    const T azh = tab[TB_AZH * A.tn + j], bzh = tab[TB_BZH * A.tn + j];
+                    const T rkzh = tab[TB_RKZH * A.tn + j];
                     e2 = cpml(e2, po + PZ_DSYY, pn + PZ_DSYY, true, azh, bzh, rkzh);
                     e3 = cpml(e3, po + PZ_DPZ, pn + PZ_DPZ, true, azh, bzh, rkzh);
                 }
@@ -277,7 +331,7 @@ __global__ void __launch_bounds__(BW) fused_step(StepArgs<T> A)
                 if (src_f && j == A.src_z) div += A.src_val;
                 biot_velocity(v.y, v.w, div, e3, mv_j1.x, mv_j1.y, mv_j1.z, mv_j1.w, A.inv_dt);
             }
-            *(Vec4<T> *)(Vn + off(j)) = v;
+            *(GN *)(Vn + off(j)) = nvec4<T>{v.x, v.y, v.z, v.w};
         }
         ++s;
     };
@@ -296,29 +350,41 @@ __global__ void __launch_bounds__(BW) fused_step(StepArgs<T> A)
 }
 
 template <class T, bool PAL>
-hipError_t launch_fused_bw(const StepArgs<T> &A, int bw, hipStream_t stream)
+hipError_t launch_fused_bw(const StepArgs<T> *models, const WorkItem *work, int nwork, int per_xcd, int bw,
+                           const long long *base, int k, hipStream_t stream)
 {
-    const dim3 grid(8 * A.per_xcd);
+    const dim3 grid(8 * per_xcd);
     switch (bw) {
-    case 128: fused_step<T, 128, PAL><<<grid, 128, 0, stream>>>(A); break;
-    case 256: fused_step<T, 256, PAL><<<grid, 256, 0, stream>>>(A); break;
-    case 512: fused_step<T, 512, PAL><<<grid, 512, 0, stream>>>(A); break;
+    case 128: fused_step<T, 128, PAL><<<grid, 128, 0, stream>>>(models, work, nwork, per_xcd, base, k); break;
+    case 256: fused_step<T, 256, PAL><<<grid, 256, 0, stream>>>(models, work, nwork, per_xcd, base, k); break;
+    case 512: fused_step<T, 512, PAL><<<grid, 512, 0, stream>>>(models, work, nwork, per_xcd, base, k); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
 template <class T>
-hipError_t launch_fused(const StepArgs<T> &A, int bw, bool palette, hipStream_t stream)
+hipError_t launch_fused(const StepArgs<T> *models, const WorkItem *work, int nwork, int per_xcd, int bw,
+                        bool palette, const long long *base, int k, hipStream_t stream)
 {
-    if (palette) {
-        if (A.npal < 1 || A.npal > kMaxPalette || !A.mid || !A.pal) return hipErrorInvalidValue;
-        return launch_fused_bw<T, true>(A, bw, stream);
-    }
-    return launch_fused_bw<T, false>(A, bw, stream);
+    if (nwork < 1 || per_xcd < 1 || 8LL * per_xcd < nwork) return hipErrorInvalidValue;
+    return palette ? launch_fused_bw<T, true>(models, work, nwork, per_xcd, bw, base, k, stream)
+                   : launch_fused_bw<T, false>(models, work, nwork, per_xcd, bw, base, k, stream);
 }
 
-template hipError_t launch_fused<float>(const StepArgs<float> &, int, bool, hipStream_t);
-template hipError_t launch_fused<double>(const StepArgs<double> &, int, bool, hipStream_t);
+template hipError_t launch_fused<float>(const StepArgs<float> *, const WorkItem *, int, int, int, bool,
+                                        const long long *, int, hipStream_t);
+template hipError_t launch_fused<double>(const StepArgs<double> *, const WorkItem *, int, int, int, bool,
+                                         const long long *, int, hipStream_t);
+
+namespace {
+__global__ void base_advance(long long *base, int delta) { base[0] += delta; }
+}  // namespace
+
+hipError_t launch_base_advance(long long *base, int delta, hipStream_t stream)
+{
+    base_advance<<<1, 1, 0, stream>>>(base, delta);
+    return hipGetLastError();
+}
 
 }  // namespace poe2d

@@ -20,10 +20,10 @@ __global__ void __launch_bounds__(256) stress_pass(StepArgs<T> A)
     if (x > A.nx || z > A.nz) return;
     const long long P = A.pitch;
     const long long o = (long long)z * P + x;
-    const Vec4<T> *V = group_c(A, group_v(A.in));
-    Vec4<T> *S = group_w(A, group_s(A.in));
-    Vec4<T> *I = group_w(A, G_I);
-    const Vec4<T> *MS = group_c(A, G_MS);
+    const Vec4<T> *V = state_group(A, group_v(A.in));
+    Vec4<T> *S = state_group(A, group_s(A.in));
+    Vec4<T> *I = state_group(A, G_I);
+    const Vec4<T> *MS = mat_group(A, G_MS);
     const int sx = A.xmap[x];
     const int sz = A.zmap[z];
     const int b = A.in;
@@ -86,10 +86,10 @@ __global__ void __launch_bounds__(256) velocity_pass(StepArgs<T> A)
     if (x > A.nx || z > A.nz) return;
     const long long P = A.pitch;
     const long long o = (long long)z * P + x;
-    Vec4<T> *V = group_w(A, group_v(A.in));
-    const Vec4<T> *S = group_c(A, group_s(A.in));
-    const Vec4<T> *I = group_c(A, G_I);
-    const Vec4<T> *MV = group_c(A, G_MV);
+    Vec4<T> *V = state_group(A, group_v(A.in));
+    const Vec4<T> *S = state_group(A, group_s(A.in));
+    const Vec4<T> *I = state_group(A, G_I);
+    const Vec4<T> *MV = mat_group(A, G_MV);
     const int sx = A.xmap[x];
     const int sz = A.zmap[z];
     const int b = A.in;

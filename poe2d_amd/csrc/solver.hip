@@ -1,18 +1,26 @@
-// Solver handle behind the C ABI of include/poe2d.h: device memory, the time loop, gather,
-// snapshots, checkpoint/restore and the stability check.
+// Solver handles and batches behind the C ABI of include/poe2d.h: device memory, the time loop,
+// gather, snapshots, checkpoint/restore and the stability check.
 //
-// Device memory of a handle (field layout described in step_common.h):
-//   arena   NGROUPS groups of 4-vectors (coefficients MS, MV; state V, S for buffers 0 and 1;
-//           two-pass intermediates I), each (nz+1+2*kPadRows) rows x pitch points, zero pads;
-//   mid/pal when the grid holds at most kMaxPalette distinct coefficient tuples: a one-byte
-//           material index per point (group layout, pads select entry 0) and the palette of
-//           (MS, MV) entries, which the single-pass kernel reads instead of the MS/MV groups;
-//   psi     C-PML memory variables: 6 x-strip and 6 z-strip arrays per state buffer;
-//   tables  the 12 1-D C-PML profiles;
-//   maps    strip index of each column and row.
+// Device memory (field layout described in step_common.h):
+//   model   (shared by a handle and its clones, read-only while stepping)
+//           material arena: MS and MV groups; when the grid holds at most kMaxPalette distinct
+//           coefficient tuples, a one-byte material index per point (group layout, pads select
+//           entry 0) and the palette of (MS, MV) entries; the 12 C-PML profiles (1/K for the K
+//           tables); the strip maps;
+//   handle  state arena: V and S groups for state buffers 0 and 1 and the two-pass
+//           intermediates I; C-PML memory variables (6 x-strip and 6 z-strip arrays per state
+//           buffer); the source series of the current run; traces; the step counter and the
+//           work list of its launches.
 // A single-pass step reads state buffer `cur` and writes buffer 1-cur.  The reference's stress1
 // reads column nx+1 at x = nx-1, which in its flat array is column 0 of the next row and always
 // zero; the zero pad column nx+1 gives the same value.
+//
+// Time loop.  Single-pass steps are launched over a work list of (model, tile) items that reads
+// each model's arguments from device memory and the step index from a device counter (a single
+// handle is a batch of one); the kernel also gathers the receivers of recorded steps.  Launch k
+// of a chunk computes step *base + k; a chunk of kGraphSteps launches and the advance of base is
+// captured once in a HIP graph and replayed, the remainder is launched directly.  The two-pass schedule is launched
+// directly with host-side arguments.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -34,6 +42,7 @@ namespace {
 
 const float kRefPiF = 3.1415926535897932f;  // the reference's float PI (seismic_CPML_2d_poroelastic.cpp:9)
 const double kStabilityThreshold = 1.0e25;  // STABILITY_THRESHOLD of the reference (declared, unused there)
+const int kGraphSteps = 32;                 // steps per captured graph
 
 #define HIP_TRY(expr)                                                                              \
     do {                                                                                           \
@@ -45,35 +54,157 @@ const double kStabilityThreshold = 1.0e25;  // STABILITY_THRESHOLD of the refere
 // single-pass (palette when the grid has one), two-pass, single-pass always on the MS/MV groups
 enum { SCHED_FUSED = 0, SCHED_TWOPASS = 1, SCHED_FUSED_GROUPS = 2 };
 
-}  // namespace
+poe2d_status dev_alloc(void **p, size_t bytes)
+{
+    hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 1));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(e == hipErrorOutOfMemory ? POE2D_E_NOMEM : POE2D_E_HIP,
+                    std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    }
+    return POE2D_OK;
+}
 
-struct poe2d_solver {
-    int precision = POE2D_F32;
-    size_t rs = 4;  // sizeof(real)
-    poe2d_grid grid{};
+poe2d_status dev_alloc_zero(void **p, size_t bytes, hipStream_t stream)
+{
+    poe2d_status st = dev_alloc(p, bytes);
+    if (st) return st;
+    HIP_TRY(hipMemsetAsync(*p, 0, std::max<size_t>(bytes, 1), stream));
+    return POE2D_OK;
+}
+
+void dev_free(void *&p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+// Grid- and material-derived device data: shared by a handle and its clones.
+struct Model {
+    int refs = 1;
     int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int precision = POE2D_F32;
+    size_t rs = 4;
+    poe2d_grid grid{};
     size_t pitch = 0, alloc_rows = 0;  // points per row, rows per group (with pads)
     int wxs = 0, wzs = 0;              // x-strip columns, z-strip rows
     int ztop = 0, zbot = 0;            // z strips: rows [0, ztop) and [zbot, nz]
     int tn = 0;                        // entries per profile table
     double cp_max = 0;
-    int cur = 0;
-    void *arena = nullptr;
-    void *psi = nullptr;
+    void *mat = nullptr;               // MS, MV groups
+    unsigned char *mid = nullptr;      // material index per point (palette mode)
+    void *pal = nullptr;               // 2 * kMaxPalette Vec4 entries
+    int npal = 0;                      // palette entries, 0 when the grid has more than kMaxPalette
     void *tab = nullptr;
     int *xmap = nullptr, *zmap = nullptr;
-    unsigned char *mid = nullptr;  // material index per point (palette mode)
-    void *pal = nullptr;           // 2 * kMaxPalette Vec4 entries
-    int npal = 0;                  // palette entries, 0 when the grid has more than kMaxPalette
-    float *red = nullptr;  // reduction scratch: [0] result, [1..] per-block partials
+    std::vector<double> htab[NTABLES];
+
+    size_t group_points() const { return pitch * alloc_rows; }
+    size_t group_bytes() const { return group_points() * 4 * rs; }
+    size_t origin() const { return (size_t)kPadRows * pitch + kPadCols; }
+    size_t pxn() const { return (size_t)std::max(wxs, 1) * (grid.nz + 1); }
+    size_t pzn() const { return (size_t)std::max(wzs, 1) * pitch; }
+    size_t psi_half_elems() const { return NPSI_X * pxn() + NPSI_Z * pzn(); }  // one buffer's worth
+
+    void release()
+    {
+        if (--refs > 0) return;
+        (void)hipSetDevice(device);
+        dev_free(mat);
+        void *q = mid; dev_free(q); mid = nullptr;
+        dev_free(pal);
+        dev_free(tab);
+        q = xmap; dev_free(q); xmap = nullptr;
+        q = zmap; dev_free(q); zmap = nullptr;
+        delete this;
+    }
+};
+
+// Device arrays of one work-list launch sequence (a handle's own or a batch's) and its captured
+// graph.  Arrays only grow, so their addresses stay put while the sizes allow, and a captured
+// graph (which holds those addresses, the item count and the kernel choice) is reused across runs;
+// the per-model arguments it points to are rewritten before every run.
+struct StepList {
+    void *models = nullptr;      // StepArgs<T>[nmodels]
+    WorkItem *work = nullptr;    // [nwork]
+    int *rec_ptr = nullptr;      // receiver offsets per (row, tile column), all models
+    int2 *rec_ent = nullptr;     // receiver entries, all models
+    long long *base = nullptr;   // step of launch 0 of the current chunk
+    long long *h_base = nullptr; // pinned host staging of base
+    size_t cap_models = 0, cap_work = 0, cap_ptr = 0, cap_ent = 0;
+    int nmodels = 0, nwork = 0, per_xcd = 0, bw = 0;
+    bool palette = false;
+    int precision = POE2D_F32;
+    // graph of kGraphSteps launches plus the base advance, and the launch it was captured for
+    hipGraphExec_t exec = nullptr;
+    struct Sig {
+        void *models, *work, *base;
+        int nwork, per_xcd, bw, palette, precision;
+        bool operator==(const Sig &o) const
+        {
+            return models == o.models && work == o.work && base == o.base && nwork == o.nwork &&
+                   per_xcd == o.per_xcd && bw == o.bw && palette == o.palette && precision == o.precision;
+        }
+    } sig{};
+
+    Sig signature() const
+    {
+        Sig g{};
+        g.models = models; g.work = work; g.base = base;
+        g.nwork = nwork; g.per_xcd = per_xcd; g.bw = bw; g.palette = palette; g.precision = precision;
+        return g;
+    }
+
+    void release()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        exec = nullptr;
+        dev_free(models);
+        void *q = work; dev_free(q); work = nullptr;
+        q = rec_ptr; dev_free(q); rec_ptr = nullptr;
+        q = rec_ent; dev_free(q); rec_ent = nullptr;
+        q = base; dev_free(q); base = nullptr;
+        if (h_base) (void)hipHostFree(h_base);
+        h_base = nullptr;
+        cap_models = cap_work = cap_ptr = cap_ent = 0;
+    }
+};
+
+poe2d_status grow(void **p, size_t *cap, size_t bytes)
+{
+    if (bytes <= *cap && *p) return POE2D_OK;
+    dev_free(*p);
+    *cap = 0;
+    poe2d_status st = dev_alloc(p, bytes);
+    if (st) return st;
+    *cap = bytes;
+    return POE2D_OK;
+}
+
+}  // namespace
+
+struct poe2d_solver {
+    Model *m = nullptr;
+    int device = 0;
+    int precision = POE2D_F32;
+    size_t rs = 4;
+    poe2d_grid grid{};
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int cur = 0;
+    void *arena = nullptr;  // state groups
+    void *psi = nullptr;
+    float *red = nullptr;   // reduction scratch: [0] result, [1..] per-block partials
     int red_blocks = 1024;
     void *dense = nullptr;  // (nz+1) x (nx+1) scratch for snapshots, made on first use
-    std::vector<double> htab[NTABLES];
+    void *src_dev = nullptr;  // source series of the current run
+    size_t src_cap = 0;
+    long long src_n0 = 0;
+    int src_len = 0;
     // schedule
     int schedule = SCHED_FUSED;
     int block_w = 256, tile_rows = 14;
+    bool graphs = true;
     // source
     bool have_src = false;
     poe2d_source src{};
@@ -88,79 +219,78 @@ struct poe2d_solver {
     int step = 0;
     double last_ms = 0;
     int last_steps = 0;
+    StepList list;
 
-    size_t group_points() const { return pitch * alloc_rows; }
-    size_t group_bytes() const { return group_points() * 4 * rs; }
-    size_t origin() const { return (size_t)kPadRows * pitch + kPadCols; }
-    size_t pxn() const { return (size_t)std::max(wxs, 1) * (grid.nz + 1); }
-    size_t pzn() const { return (size_t)std::max(wzs, 1) * pitch; }
-    size_t psi_half_elems() const { return NPSI_X * pxn() + NPSI_Z * pzn(); }  // one buffer's worth
+    size_t group_bytes() const { return m->group_bytes(); }
     size_t trace_bytes() const { return (size_t)nsamp * ncomp * nrec * rs; }
     unsigned char *group_ptr(int g) const { return (unsigned char *)arena + (size_t)g * group_bytes(); }
 
-    double source_value(int n) const
+    double source_value(long long n) const
     {
         if (!have_src) return 0.0;
         const double t = n * grid.dt;
         if (src.t_cut > 0 && t >= src.t_cut) return 0.0;
-        if (have_user_w) return n < (int)user_w.size() ? src.amp * user_w[n] : 0.0;
+        if (have_user_w) return n < (long long)user_w.size() ? src.amp * user_w[n] : 0.0;
         return src.amp * wavelet_shape(src.kind, grid.f0, t0, t);
     }
 };
 
+struct poe2d_batch {
+    std::vector<poe2d_solver *> members;
+    int device = 0;
+    int precision = POE2D_F32;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int block_w = 256, tile_rows = 14;
+    bool graphs = true;
+    double last_ms = 0;
+    int last_steps = 0;
+    StepList list;
+};
+
 namespace {
 
-void free_all(poe2d_solver *s)
+poe2d_status check_handle(poe2d_solver *s)
 {
-    auto f = [](void *&p) {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-    };
-    f(s->arena);
-    f(s->psi);
-    f(s->tab);
-    f(s->pal);
-    void *q = s->mid; f(q); s->mid = nullptr;
-    q = s->xmap; f(q); s->xmap = nullptr;
-    q = s->zmap; f(q); s->zmap = nullptr;
-    q = s->red; f(q); s->red = nullptr;
-    q = s->rix; f(q); s->rix = nullptr;
-    q = s->riz; f(q); s->riz = nullptr;
-    f(s->traces);
-    f(s->dense);
+    if (!s) return fail(POE2D_E_INVALID, "null solver handle");
+    HIP_TRY(hipSetDevice(s->device));
+    return POE2D_OK;
+}
+
+void free_handle(poe2d_solver *s)
+{
+    s->list.release();
+    dev_free(s->arena);
+    dev_free(s->psi);
+    void *q = s->red; dev_free(q); s->red = nullptr;
+    q = s->rix; dev_free(q); s->rix = nullptr;
+    q = s->riz; dev_free(q); s->riz = nullptr;
+    dev_free(s->traces);
+    dev_free(s->dense);
+    dev_free(s->src_dev);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     s->ev0 = s->ev1 = nullptr;
     s->stream = nullptr;
-}
-
-poe2d_status alloc_zero(poe2d_solver *s, void **p, size_t bytes)
-{
-    hipError_t e = hipMalloc(p, bytes);
-    if (e != hipSuccess) {
-        *p = nullptr;
-        return fail(e == hipErrorOutOfMemory ? POE2D_E_NOMEM : POE2D_E_HIP,
-                    std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
-    }
-    HIP_TRY(hipMemsetAsync(*p, 0, bytes, s->stream));
-    return POE2D_OK;
+    if (s->m) s->m->release();
+    s->m = nullptr;
 }
 
 // Coefficients into the MS (alpha, M, lambda_c, mu) and MV (rho, rho_f, m, b) groups, and, when
 // the grid holds at most kMaxPalette distinct tuples (bitwise, after rounding to T), into the
 // palette and the one-byte material index.
 template <class T>
-poe2d_status upload_coeffs(poe2d_solver *s, const poe2d_coeffs *c)
+poe2d_status upload_coeffs(Model *md, const poe2d_coeffs *c)
 {
-    const poe2d_grid &g = s->grid;
+    const poe2d_grid &g = md->grid;
     const size_t cpitch = c->pitch ? c->pitch : (size_t)g.nx + 1;
     const void *ms_src[4] = {c->alpha, c->M, c->lambdac, c->mu};
     const void *mv_src[4] = {c->rho, c->rhof, c->m, c->b};
     auto at = [&](const void *a, size_t i) {
         return c->dtype == POE2D_F64 ? ((const double *)a)[i] : (double)((const float *)a)[i];
     };
-    std::vector<Vec4<T>> host(s->group_points(), Vec4<T>{T(0), T(0), T(0), T(0)});
+    std::vector<Vec4<T>> host(md->group_points(), Vec4<T>{T(0), T(0), T(0), T(0)});
     for (int grp = 0; grp < 2; ++grp) {
         const void **src = grp == 0 ? ms_src : mv_src;
         for (int z = 0; z <= g.nz; ++z)
@@ -178,9 +308,10 @@ poe2d_status upload_coeffs(poe2d_solver *s, const poe2d_coeffs *c)
                         return fail(POE2D_E_INVALID, "create: coefficients need rho > 0, rho m > rho_f^2 and "
                                                      "b >= 0 (point " + std::to_string(x) + ", " + std::to_string(z) + ")");
                 }
-                host[s->origin() + (size_t)z * s->pitch + x] = Vec4<T>{(T)v[0], (T)v[1], (T)v[2], (T)v[3]};
+                host[md->origin() + (size_t)z * md->pitch + x] = Vec4<T>{(T)v[0], (T)v[1], (T)v[2], (T)v[3]};
             }
-        HIP_TRY(hipMemcpy(s->group_ptr(grp == 0 ? G_MS : G_MV), host.data(), s->group_bytes(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy((unsigned char *)md->mat + (size_t)(grp == 0 ? G_MS : G_MV) * md->group_bytes(), host.data(),
+                          md->group_bytes(), hipMemcpyHostToDevice));
     }
 
     // palette: distinct 8-tuples in order of first appearance
@@ -188,7 +319,7 @@ poe2d_status upload_coeffs(poe2d_solver *s, const poe2d_coeffs *c)
     auto less = [](const Key &a, const Key &b) { return std::memcmp(a.data(), b.data(), sizeof(Key)) < 0; };
     std::map<Key, int, decltype(less)> seen(less);
     std::vector<Key> pal;
-    std::vector<unsigned char> idx(s->group_points(), 0);
+    std::vector<unsigned char> idx(md->group_points(), 0);
     Key last{};
     int last_id = -1;
     for (int z = 0; z <= g.nz && pal.size() <= (size_t)kMaxPalette; ++z)
@@ -209,7 +340,7 @@ poe2d_status upload_coeffs(poe2d_solver *s, const poe2d_coeffs *c)
                 last = k;
                 last_id = it->second;
             }
-            idx[s->origin() + (size_t)z * s->pitch + x] = (unsigned char)last_id;
+            idx[md->origin() + (size_t)z * md->pitch + x] = (unsigned char)last_id;
         }
     if (pal.size() > (size_t)kMaxPalette) return POE2D_OK;  // no palette: the groups serve every schedule
     std::vector<Vec4<T>> hp(2 * kMaxPalette, Vec4<T>{T(0), T(0), T(0), T(0)});
@@ -218,28 +349,27 @@ poe2d_status upload_coeffs(poe2d_solver *s, const poe2d_coeffs *c)
         hp[kMaxPalette + e] = Vec4<T>{pal[e][4], pal[e][5], pal[e][6], pal[e][7]};
     }
     void *q = nullptr;
-    poe2d_status st = alloc_zero(s, &q, s->group_points());
+    poe2d_status st = dev_alloc(&q, md->group_points());
     if (st) return st;
-    s->mid = (unsigned char *)q;
-    if ((st = alloc_zero(s, &s->pal, hp.size() * sizeof(Vec4<T>)))) return st;
-    HIP_TRY(hipStreamSynchronize(s->stream));
-    HIP_TRY(hipMemcpy(s->mid, idx.data(), idx.size(), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(s->pal, hp.data(), hp.size() * sizeof(Vec4<T>), hipMemcpyHostToDevice));
-    s->npal = (int)pal.size();
+    md->mid = (unsigned char *)q;
+    if ((st = dev_alloc(&md->pal, hp.size() * sizeof(Vec4<T>)))) return st;
+    HIP_TRY(hipMemcpy(md->mid, idx.data(), idx.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(md->pal, hp.data(), hp.size() * sizeof(Vec4<T>), hipMemcpyHostToDevice));
+    md->npal = (int)pal.size();
     return POE2D_OK;
 }
 
 template <class T>
-poe2d_status upload_tables(poe2d_solver *s)
+poe2d_status upload_tables(Model *md)
 {
     // the K tables go to the device as 1/K, so the kernels multiply
-    std::vector<T> h((size_t)NTABLES * s->tn, T(0));
+    std::vector<T> h((size_t)NTABLES * md->tn, T(0));
     for (int k = 0; k < NTABLES; ++k) {
         const bool recip = k == TB_RKX || k == TB_RKXH || k == TB_RKZ || k == TB_RKZH;
-        for (size_t i = 0; i < s->htab[k].size(); ++i)
-            h[(size_t)k * s->tn + i] = (T)(recip ? 1.0 / s->htab[k][i] : s->htab[k][i]);
+        for (size_t i = 0; i < md->htab[k].size(); ++i)
+            h[(size_t)k * md->tn + i] = (T)(recip ? 1.0 / md->htab[k][i] : md->htab[k][i]);
     }
-    HIP_TRY(hipMemcpy(s->tab, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(md->tab, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
     return POE2D_OK;
 }
 
@@ -267,20 +397,23 @@ std::vector<int> strip_map(const std::vector<double> *t, int n, int *count, int 
     return map;
 }
 
+// Arguments of the step kernels for handle s, reading state buffer `in` and writing `out`, with
+// tiles of bw - 8 columns by th rows.
 template <class T>
-StepArgs<T> step_args(poe2d_solver *s, int in, int out)
+StepArgs<T> step_args(const poe2d_solver *s, int in, int out, int bw, int th)
 {
     StepArgs<T> A{};
-    const poe2d_grid &g = s->grid;
+    const Model *md = s->m;
+    const poe2d_grid &g = md->grid;
     A.nx = g.nx;
     A.nz = g.nz;
-    A.pitch = (int)s->pitch;
-    A.wxs = s->wxs;
-    A.origin = (long long)s->origin();
-    A.group_stride = (long long)s->group_points();
-    A.pxn = (long long)s->pxn();
-    A.pzn = (long long)s->pzn();
-    A.tn = s->tn;
+    A.pitch = (int)md->pitch;
+    A.wxs = md->wxs;
+    A.origin = (long long)md->origin();
+    A.group_stride = (long long)md->group_points();
+    A.pxn = (long long)md->pxn();
+    A.pzn = (long long)md->pzn();
+    A.tn = md->tn;
     A.in = in;
     A.out = out;
     A.rdx = T(1.0 / (24.0 * g.dx));
@@ -288,35 +421,210 @@ StepArgs<T> step_args(poe2d_solver *s, int in, int out)
     A.dt = T(g.dt);
     A.inv_dt = T(1.0 / g.dt);
     A.grp = (Vec4<T> *)s->arena;
+    A.mat = (const Vec4<T> *)md->mat;
     A.psi = (T *)s->psi;
-    A.tab = (const T *)s->tab;
-    A.xmap = s->xmap;
-    A.zmap = s->zmap;
-    A.ztop = s->ztop;
-    A.zbot = s->zbot;
-    A.mid = s->mid;
-    A.pal = (const Vec4<T> *)s->pal;
-    A.npal = s->npal;
+    A.tab = (const T *)md->tab;
+    A.xmap = md->xmap;
+    A.zmap = md->zmap;
+    A.ztop = md->ztop;
+    A.zbot = md->zbot;
+    A.mid = md->mid;
+    A.pal = (const Vec4<T> *)md->pal;
+    A.npal = md->npal;
     A.src_kind = s->have_src ? s->src.kind : -1;
     A.src_x = s->have_src ? s->src.ix : -1;
     A.src_z = s->have_src ? s->src.iz : -1;
     A.src_val = T(0);
-    const int W = fused_tile_width(s->block_w);
-    A.th = s->tile_rows;
-    A.ntx = (g.nx + 1 + W - 1) / W;
-    const int ntz = (g.nz + 1 + A.th - 1) / A.th;
-    A.ntiles = A.ntx * ntz;
+    A.src_series = (const T *)s->src_dev;
+    A.src_n0 = s->src_n0;
+    A.src_len = s->src_len;
+    // step n = s->step reads state buffer cur
+    A.par = (int)(((long long)s->cur - s->step) & 1);
+    A.th = th;
+    A.ntx = (g.nx + 1 + fused_tile_width(bw) - 1) / fused_tile_width(bw);
+    A.ntiles = A.ntx * ((g.nz + 1 + th - 1) / th);
     A.per_xcd = (A.ntiles + 7) / 8;
     return A;
 }
 
 template <class T>
+StepArgs<T> step_args(const poe2d_solver *s, int in, int out)
+{
+    return step_args<T>(s, in, out, s->block_w, s->tile_rows);
+}
+
+// Source samples of steps [n0, n0 + k) into the handle's device series.
+template <class T>
+poe2d_status upload_source(poe2d_solver *s, long long n0, int k)
+{
+    poe2d_status st = grow(&s->src_dev, &s->src_cap, (size_t)std::max(k, 1) * sizeof(T));
+    if (st) return st;
+    std::vector<T> h(std::max(k, 1), T(0));
+    for (int i = 0; i < k; ++i) h[i] = (T)s->source_value(n0 + i);
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipMemcpy(s->src_dev, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    s->src_n0 = n0;
+    s->src_len = k;
+    return POE2D_OK;
+}
+
+// Fill `L` for the members (all at the same step, current source series uploaded): per-member
+// arguments, the (member, tile) work list with block width bw and tile height th, and each
+// member's receivers sorted by row and column with their offsets per (row, tile column).  The
+// uploads wait for `stream` first.
+template <class T>
+poe2d_status build_list(StepList &L, const std::vector<poe2d_solver *> &members, int bw, int th, bool palette,
+                        hipStream_t stream)
+{
+    const int nm = (int)members.size();
+    std::vector<StepArgs<T>> args(nm);
+    std::vector<WorkItem> work;
+    std::vector<int> ptr;
+    std::vector<int2> ent;
+    std::vector<size_t> ptr_off(nm), ent_off(nm);
+    const int W = fused_tile_width(bw);
+    for (int i = 0; i < nm; ++i) {
+        const poe2d_solver *s = members[i];
+        args[i] = step_args<T>(s, 0, 1, bw, th);
+        if ((long long)work.size() + args[i].ntiles > (1LL << 30))
+            return fail(POE2D_E_INVALID, "run: too many tiles in one launch");
+        for (int t = 0; t < args[i].ntiles; ++t) work.push_back(WorkItem{i, t});
+        // receivers sorted by (tile column, row, column), with offsets per (tile column, row)
+        const int ntx = args[i].ntx, nrow = s->grid.nz + 1;
+        std::vector<int> order(s->nrec);
+        for (int r = 0; r < s->nrec; ++r) order[r] = r;
+        std::vector<int> hx(s->nrec), hz(s->nrec);
+        if (s->nrec > 0) {
+            HIP_TRY(hipMemcpy(hx.data(), s->rix, s->nrec * sizeof(int), hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(hz.data(), s->riz, s->nrec * sizeof(int), hipMemcpyDeviceToHost));
+        }
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+            if (hx[a] / W != hx[b] / W) return hx[a] / W < hx[b] / W;
+            return hz[a] != hz[b] ? hz[a] < hz[b] : hx[a] < hx[b];
+        });
+        ptr_off[i] = ptr.size();
+        ent_off[i] = ent.size();
+        std::vector<int> cnt((size_t)ntx * nrow + 1, 0);
+        for (int r : order) cnt[(size_t)(hx[r] / W) * nrow + hz[r] + 1]++;
+        for (size_t q = 1; q < cnt.size(); ++q) cnt[q] += cnt[q - 1];
+        ptr.insert(ptr.end(), cnt.begin(), cnt.end());
+        for (int r : order) ent.push_back(int2{hx[r], r});
+        args[i].rec_n = s->nrec;
+        args[i].rec_ncomp = s->ncomp;
+        args[i].rec_dec = s->dec;
+        args[i].rec_nsamp = s->nsamp;
+        args[i].rec_mask = s->mask;
+        args[i].traces = (T *)s->traces;
+    }
+    poe2d_status st;
+    if ((st = grow(&L.models, &L.cap_models, args.size() * sizeof(StepArgs<T>)))) return st;
+    void *w = L.work;
+    st = grow(&w, &L.cap_work, work.size() * sizeof(WorkItem));
+    L.work = (WorkItem *)w;
+    if (st) return st;
+    void *pp = L.rec_ptr;
+    st = grow(&pp, &L.cap_ptr, ptr.size() * sizeof(int));
+    L.rec_ptr = (int *)pp;
+    if (st) return st;
+    void *pe = L.rec_ent;
+    st = grow(&pe, &L.cap_ent, std::max<size_t>(ent.size(), 1) * sizeof(int2));
+    L.rec_ent = (int2 *)pe;
+    if (st) return st;
+    for (int i = 0; i < nm; ++i) {
+        args[i].rec_tile_ptr = L.rec_ptr + ptr_off[i];
+        args[i].rec_entries = L.rec_ent + ent_off[i];
+    }
+    if (!L.base) {
+        void *q = nullptr;
+        if ((st = dev_alloc(&q, sizeof(long long)))) return st;
+        L.base = (long long *)q;
+    }
+    if (!L.h_base) HIP_TRY(hipHostMalloc((void **)&L.h_base, sizeof(long long), hipHostMallocDefault));
+    HIP_TRY(hipStreamSynchronize(stream));
+    HIP_TRY(hipMemcpy(L.models, args.data(), args.size() * sizeof(StepArgs<T>), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(L.work, work.data(), work.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(L.rec_ptr, ptr.data(), ptr.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (!ent.empty()) HIP_TRY(hipMemcpy(L.rec_ent, ent.data(), ent.size() * sizeof(int2), hipMemcpyHostToDevice));
+    L.nmodels = nm;
+    L.nwork = (int)work.size();
+    L.per_xcd = (L.nwork + 7) / 8;
+    L.bw = bw;
+    L.palette = palette;
+    L.precision = sizeof(T) == 4 ? POE2D_F32 : POE2D_F64;
+    return POE2D_OK;
+}
+
+// Launches k = 0 .. count-1 of list L on `stream` (steps *base + k), then *base += count.
+template <class T>
+hipError_t launch_list_chunk(const StepList &L, int count, hipStream_t stream)
+{
+    for (int k = 0; k < count; ++k) {
+        hipError_t e = launch_fused<T>((const StepArgs<T> *)L.models, L.work, L.nwork, L.per_xcd, L.bw, L.palette,
+                                       L.base, k, stream);
+        if (e != hipSuccess) return e;
+    }
+    return launch_base_advance(L.base, count, stream);
+}
+
+// k steps of list L from step n0 on `stream`: graph replays of kGraphSteps steps, the remainder
+// launched directly.
+template <class T>
+poe2d_status run_list(StepList &L, long long n0, int k, bool graphs, hipStream_t stream)
+{
+    *L.h_base = n0;
+    HIP_TRY(hipMemcpyAsync(L.base, L.h_base, sizeof(long long), hipMemcpyHostToDevice, stream));
+    int done = 0;
+    if (graphs && k >= kGraphSteps) {
+        const StepList::Sig sig = L.signature();
+        if (!L.exec || !(L.sig == sig)) {
+            if (L.exec) (void)hipGraphExecDestroy(L.exec);
+            L.exec = nullptr;
+            hipGraph_t graph = nullptr;
+            HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            const hipError_t e = launch_list_chunk<T>(L, kGraphSteps, stream);
+            const hipError_t e2 = hipStreamEndCapture(stream, &graph);
+            if (e != hipSuccess || e2 != hipSuccess) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return fail(POE2D_E_HIP, std::string("run: graph capture failed: ") +
+                                             hipGetErrorString(e != hipSuccess ? e : e2));
+            }
+            const hipError_t e3 = hipGraphInstantiate(&L.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e3 != hipSuccess) {
+                L.exec = nullptr;
+                return fail(POE2D_E_HIP, std::string("run: graph instantiation failed: ") + hipGetErrorString(e3));
+            }
+            L.sig = sig;
+        }
+        for (; done + kGraphSteps <= k; done += kGraphSteps) HIP_TRY(hipGraphLaunch(L.exec, stream));
+    }
+    if (done < k) HIP_TRY(launch_list_chunk<T>(L, k - done, stream));
+    return POE2D_OK;
+}
+
+// Largest |v| of the handle's current state; POE2D_E_UNSTABLE above the reference's threshold.
+template <class T>
+poe2d_status stability_check(poe2d_solver *s)
+{
+    StepArgs<T> A = step_args<T>(s, s->cur, s->cur);
+    launch_absmax<T>(A, s->cur, s->red + 1, s->red_blocks, s->red, s->stream);
+    float vmax = 0.f;
+    HIP_TRY(hipMemcpyAsync(&vmax, s->red, sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (!(vmax <= kStabilityThreshold))
+        return fail(POE2D_E_UNSTABLE, "run: wavefield became unstable (max |v| = " + std::to_string(vmax) +
+                                          ") by step " + std::to_string(s->step));
+    return POE2D_OK;
+}
+
+template <class T>
 poe2d_status run_steps(poe2d_solver *s, int nsteps)
 {
-    Receivers R{s->nrec, s->mask, s->rix, s->riz};
-    HIP_TRY(hipEventRecord(s->ev0, s->stream));
+    poe2d_status st;
     if (s->schedule == SCHED_TWOPASS) {
+        Receivers R{s->nrec, s->mask, s->rix, s->riz};
         StepArgs<T> A = step_args<T>(s, s->cur, s->cur);
+        HIP_TRY(hipEventRecord(s->ev0, s->stream));
         for (int k = 0; k < nsteps; ++k) {
             const int n = s->step + k;
             A.src_val = (T)s->source_value(n);
@@ -327,36 +635,24 @@ poe2d_status run_steps(poe2d_solver *s, int nsteps)
                                  s->stream);
             launch_twopass<T>(A, s->stream, 1);
         }
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(s->ev1, s->stream));
     } else {
-        StepArgs<T> F[2] = {step_args<T>(s, 0, 1), step_args<T>(s, 1, 0)};
-        for (int k = 0; k < nsteps; ++k) {
-            const int n = s->step + k;
-            StepArgs<T> &A = F[s->cur];
-            A.src_val = (T)s->source_value(n);
-            HIP_TRY(launch_fused<T>(A, s->block_w, s->schedule == SCHED_FUSED && s->npal > 0, s->stream));
-            if (s->nrec > 0 && n % s->dec == 0 && n / s->dec < s->nsamp)
-                launch_gather<T>(A, s->cur, 1 - s->cur, R, (T *)s->traces + (size_t)(n / s->dec) * s->ncomp * s->nrec,
-                                 s->stream);
-            s->cur = 1 - s->cur;
-        }
+        if ((st = upload_source<T>(s, s->step, nsteps))) return st;
+        const bool palette = s->schedule == SCHED_FUSED && s->m->npal > 0;
+        if ((st = build_list<T>(s->list, {s}, s->block_w, s->tile_rows, palette, s->stream))) return st;
+        HIP_TRY(hipEventRecord(s->ev0, s->stream));
+        if ((st = run_list<T>(s->list, s->step, nsteps, s->graphs, s->stream))) return st;
+        HIP_TRY(hipEventRecord(s->ev1, s->stream));
+        s->cur = (s->cur + nsteps) & 1;
     }
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(s->ev1, s->stream));
     HIP_TRY(hipEventSynchronize(s->ev1));
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, s->ev0, s->ev1));
     s->last_ms = ms;
     s->last_steps = nsteps;
     s->step += nsteps;
-    // stability check: largest |v| over the grid
-    launch_absmax<T>(step_args<T>(s, s->cur, s->cur), s->cur, s->red + 1, s->red_blocks, s->red, s->stream);
-    float vmax = 0.f;
-    HIP_TRY(hipMemcpyAsync(&vmax, s->red, sizeof(float), hipMemcpyDeviceToHost, s->stream));
-    HIP_TRY(hipStreamSynchronize(s->stream));
-    if (!(vmax <= kStabilityThreshold))
-        return fail(POE2D_E_UNSTABLE, "run: wavefield became unstable (max |v| = " + std::to_string(vmax) +
-                                          ") by step " + std::to_string(s->step));
-    return POE2D_OK;
+    return stability_check<T>(s);
 }
 
 // One field of the current state (or p / sxx / syy recomputed from it) to the host.
@@ -365,7 +661,7 @@ poe2d_status field_to_host(poe2d_solver *s, int field, void *out)
 {
     const size_t bytes = (size_t)(s->grid.nx + 1) * (s->grid.nz + 1) * sizeof(T);
     if (!s->dense) {
-        poe2d_status st = alloc_zero(s, &s->dense, bytes);
+        poe2d_status st = dev_alloc_zero(&s->dense, bytes, s->stream);
         if (st) return st;
     }
     StepArgs<T> A = step_args<T>(s, s->cur, s->cur);
@@ -386,17 +682,10 @@ poe2d_status field_to_host(poe2d_solver *s, int field, void *out)
     return POE2D_OK;
 }
 
-poe2d_status check_handle(poe2d_solver *s)
-{
-    if (!s) return fail(POE2D_E_INVALID, "null solver handle");
-    HIP_TRY(hipSetDevice(s->device));
-    return POE2D_OK;
-}
-
 struct CkptHeader {
     char magic[8];
     int version, precision, nx, nz, step, nrec, ncomp, nsamp, wxs, wzs;
-    unsigned long long pitch;
+    unsigned long long pitch, alloc_rows;
 };
 
 // The buffers that make up the state of step `step`: V and S groups of buffer b, its memory
@@ -404,12 +693,87 @@ struct CkptHeader {
 template <class F>
 void for_each_buffer(poe2d_solver *s, int b, F f)
 {
+    const Model *md = s->m;
     f(s->group_ptr(group_v(b)), s->group_bytes());
     f(s->group_ptr(group_s(b)), s->group_bytes());
     unsigned char *psi = (unsigned char *)s->psi;
-    f(psi + (size_t)b * NPSI_X * s->pxn() * s->rs, NPSI_X * s->pxn() * s->rs);
-    f(psi + 2 * NPSI_X * s->pxn() * s->rs + (size_t)b * NPSI_Z * s->pzn() * s->rs, NPSI_Z * s->pzn() * s->rs);
+    f(psi + (size_t)b * NPSI_X * md->pxn() * s->rs, NPSI_X * md->pxn() * s->rs);
+    f(psi + 2 * NPSI_X * md->pxn() * s->rs + (size_t)b * NPSI_Z * md->pzn() * s->rs, NPSI_Z * md->pzn() * s->rs);
     if (s->traces) f(s->traces, s->trace_bytes());
+}
+
+// A handle on model md (whose reference the caller hands over): stream, events, state arena,
+// memory variables, reduction scratch.  On failure the reference is released.
+poe2d_solver *new_handle(Model *md, poe2d_status *st)
+{
+    poe2d_solver *s = new (std::nothrow) poe2d_solver;
+    if (!s) {
+        md->release();
+        *st = fail(POE2D_E_NOMEM, "create: out of host memory");
+        return nullptr;
+    }
+    s->m = md;
+    s->device = md->device;
+    s->precision = md->precision;
+    s->rs = md->rs;
+    s->grid = md->grid;
+    auto bail = [&](poe2d_status e) -> poe2d_solver * {
+        *st = e;
+        free_handle(s);
+        delete s;
+        return nullptr;
+    };
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess)
+        return bail(fail(POE2D_E_HIP, "create: stream/event creation failed"));
+    poe2d_status e;
+    if ((e = dev_alloc_zero(&s->arena, NSTATE_GROUPS * md->group_bytes(), s->stream))) return bail(e);
+    if ((e = dev_alloc_zero(&s->psi, 2 * md->psi_half_elems() * s->rs, s->stream))) return bail(e);
+    void *q = nullptr;
+    if ((e = dev_alloc_zero(&q, (s->red_blocks + 1) * sizeof(float), s->stream))) return bail(e);
+    s->red = (float *)q;
+    if (hipStreamSynchronize(s->stream) != hipSuccess) return bail(fail(POE2D_E_HIP, "create: synchronisation failed"));
+    *st = POE2D_OK;
+    return s;
+}
+
+poe2d_status check_batch(poe2d_batch *b)
+{
+    if (!b) return fail(POE2D_E_INVALID, "null batch handle");
+    HIP_TRY(hipSetDevice(b->device));
+    return POE2D_OK;
+}
+
+template <class T>
+poe2d_status batch_run_steps(poe2d_batch *b, int nsteps)
+{
+    const long long n0 = b->members[0]->step;
+    bool palette = true;
+    for (poe2d_solver *s : b->members) {
+        poe2d_status st = upload_source<T>(s, n0, nsteps);
+        if (st) return st;
+        palette = palette && s->schedule == SCHED_FUSED && s->m->npal > 0;
+    }
+    poe2d_status st = build_list<T>(b->list, b->members, b->block_w, b->tile_rows, palette, b->stream);
+    if (st) return st;
+    HIP_TRY(hipEventRecord(b->ev0, b->stream));
+    if ((st = run_list<T>(b->list, n0, nsteps, b->graphs, b->stream))) return st;
+    HIP_TRY(hipEventRecord(b->ev1, b->stream));
+    HIP_TRY(hipEventSynchronize(b->ev1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    b->last_ms = ms;
+    b->last_steps = nsteps;
+    poe2d_status first = POE2D_OK;
+    for (poe2d_solver *s : b->members) {
+        s->cur = (s->cur + nsteps) & 1;
+        s->step += nsteps;
+        s->last_ms = ms;
+        s->last_steps = nsteps;
+        const poe2d_status c = stability_check<T>(s);
+        if (c && !first) first = c;
+    }
+    return first;
 }
 
 }  // namespace
@@ -451,87 +815,86 @@ poe2d_status poe2d_create(const poe2d_grid *g, const poe2d_coeffs *c, poe2d_solv
     if (c->pitch && c->pitch < (size_t)g->nx + 1) return fail(POE2D_E_INVALID, "create: coefficient pitch < nx+1");
     if (!(g->f0 > 0)) return fail(POE2D_E_INVALID, "create: f0 must be > 0");
 
-    poe2d_solver *s = new (std::nothrow) poe2d_solver;
-    if (!s) return fail(POE2D_E_NOMEM, "create: out of host memory");
-    s->grid = *g;
-    s->precision = g->precision;
-    s->rs = g->precision == POE2D_F32 ? 4 : 8;
+    Model *md = new (std::nothrow) Model;
+    if (!md) return fail(POE2D_E_NOMEM, "create: out of host memory");
+    md->grid = *g;
+    md->precision = g->precision;
+    md->rs = g->precision == POE2D_F32 ? 4 : 8;
     auto bail = [&](poe2d_status st) {
-        free_all(s);
-        delete s;
+        md->release();
         return st;
     };
-    hipError_t e = hipGetDevice(&s->device);
-    if (e != hipSuccess) {
-        fail(POE2D_E_HIP, std::string("create: no HIP device: ") + hipGetErrorString(e));
-        return bail(POE2D_E_HIP);
-    }
-    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess) {
-        fail(POE2D_E_HIP, "create: stream/event creation failed");
-        return bail(POE2D_E_HIP);
-    }
+    hipError_t e = hipGetDevice(&md->device);
+    if (e != hipSuccess) return bail(fail(POE2D_E_HIP, std::string("create: no HIP device: ") + hipGetErrorString(e)));
 
     // stability and C-PML set-up
-    s->cp_max = g->cp_max > 0 ? g->cp_max : cp_max_from_coeffs(*g, *c);
-    if (!(s->cp_max > 0) || !std::isfinite(s->cp_max)) {
-        fail(POE2D_E_INVALID, "create: coefficients give no positive P velocity");
-        return bail(POE2D_E_INVALID);
-    }
-    const double courant = s->cp_max * g->dt / std::min(g->dx, g->dz);
-    if (g->check_cfl && courant > 6.0 / 7.0 / std::sqrt(2.0)) {
-        fail(POE2D_E_CFL, "create: time step is too large, simulation will be unstable (Courant " +
-                              std::to_string(courant) + " > 6/7/sqrt(2))");
-        return bail(POE2D_E_CFL);
-    }
+    md->cp_max = g->cp_max > 0 ? g->cp_max : cp_max_from_coeffs(*g, *c);
+    if (!(md->cp_max > 0) || !std::isfinite(md->cp_max))
+        return bail(fail(POE2D_E_INVALID, "create: coefficients give no positive P velocity"));
+    const double courant = md->cp_max * g->dt / std::min(g->dx, g->dz);
+    if (g->check_cfl && courant > 6.0 / 7.0 / std::sqrt(2.0))
+        return bail(fail(POE2D_E_CFL, "create: time step is too large, simulation will be unstable (Courant " +
+                                          std::to_string(courant) + " > 6/7/sqrt(2))"));
     double amax = g->alpha_max;
     if (amax <= 0)
         amax = g->precision == POE2D_F32 ? (double)(float)(2.e0 * kRefPiF * ((float)g->f0 / 2.e0))
                                          : 3.14159265358979323846 * g->f0;
     for (int axis = 0; axis < 2; ++axis) {
         const int n = (axis == 0 ? g->nx : g->nz) + 1;
-        std::vector<double> *t = s->htab + (axis == 0 ? TB_AX : TB_AZ);
+        std::vector<double> *t = md->htab + (axis == 0 ? TB_AX : TB_AZ);
         for (int k = 0; k < 6; ++k) t[k].assign(n, 0.0);
         poe2d_status st = poe2d_cpml_profile(axis, n, axis == 0 ? g->dx : g->dz, g->dt, g->npml,
-                                             g->npower, g->rcoef, g->kmax, amax, s->cp_max,
+                                             g->npower, g->rcoef, g->kmax, amax, md->cp_max,
                                              g->precision, t[0].data(), t[1].data(), t[2].data(),
                                              t[3].data(), t[4].data(), t[5].data());
         if (st != POE2D_OK) return bail(st);
     }
     int xlo = 0, xhi = 0;
-    std::vector<int> xm = strip_map(s->htab + TB_AX, g->nx + 1, &s->wxs, &xlo, &xhi);
-    std::vector<int> zm = strip_map(s->htab + TB_AZ, g->nz + 1, &s->wzs, &s->ztop, &s->zbot);
-    s->tn = std::max(g->nx, g->nz) + 1;
+    std::vector<int> xm = strip_map(md->htab + TB_AX, g->nx + 1, &md->wxs, &xlo, &xhi);
+    std::vector<int> zm = strip_map(md->htab + TB_AZ, g->nz + 1, &md->wzs, &md->ztop, &md->zbot);
+    md->tn = std::max(g->nx, g->nz) + 1;
 
-    s->pitch = (((size_t)kPadCols + g->nx + 2 + 63) / 64) * 64;
-    s->alloc_rows = (size_t)g->nz + 1 + 2 * kPadRows;
-    if (s->group_bytes() >= (1ull << 32)) {
-        // the single-pass kernel addresses a group with 32-bit byte offsets
-        fail(POE2D_E_INVALID, "create: grid too large (one field group exceeds 4 GiB)");
-        return bail(POE2D_E_INVALID);
-    }
+    md->pitch = (((size_t)kPadCols + g->nx + 2 + 63) / 64) * 64;
+    md->alloc_rows = (size_t)g->nz + 1 + 2 * kPadRows;
+    // the single-pass kernel addresses a group with 32-bit byte offsets
+    if (md->group_bytes() >= (1ull << 32))
+        return bail(fail(POE2D_E_INVALID, "create: grid too large (one field group exceeds 4 GiB)"));
     poe2d_status st = POE2D_OK;
-    if ((st = alloc_zero(s, &s->arena, NGROUPS * s->group_bytes()))) return bail(st);
-    if ((st = alloc_zero(s, &s->psi, 2 * s->psi_half_elems() * s->rs))) return bail(st);
-    if ((st = alloc_zero(s, &s->tab, (size_t)NTABLES * s->tn * s->rs))) return bail(st);
+    if ((st = dev_alloc_zero(&md->mat, NMAT_GROUPS * md->group_bytes(), nullptr))) return bail(st);
+    if ((st = dev_alloc_zero(&md->tab, (size_t)NTABLES * md->tn * md->rs, nullptr))) return bail(st);
     void *q = nullptr;
-    if ((st = alloc_zero(s, &q, xm.size() * sizeof(int)))) return bail(st);
-    s->xmap = (int *)q;
-    if ((st = alloc_zero(s, &q, zm.size() * sizeof(int)))) return bail(st);
-    s->zmap = (int *)q;
-    if ((st = alloc_zero(s, &q, (s->red_blocks + 1) * sizeof(float)))) return bail(st);
-    s->red = (float *)q;
-    if (hipStreamSynchronize(s->stream) != hipSuccess ||
-        hipMemcpy(s->xmap, xm.data(), xm.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(s->zmap, zm.data(), zm.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
-        fail(POE2D_E_HIP, "create: map upload failed");
-        return bail(POE2D_E_HIP);
-    }
-    st = s->precision == POE2D_F32 ? upload_tables<float>(s) : upload_tables<double>(s);
+    if ((st = dev_alloc(&q, xm.size() * sizeof(int)))) return bail(st);
+    md->xmap = (int *)q;
+    if ((st = dev_alloc(&q, zm.size() * sizeof(int)))) return bail(st);
+    md->zmap = (int *)q;
+    if (hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(md->xmap, xm.data(), xm.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(md->zmap, zm.data(), zm.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+        return bail(fail(POE2D_E_HIP, "create: map upload failed"));
+    st = md->precision == POE2D_F32 ? upload_tables<float>(md) : upload_tables<double>(md);
     if (st) return bail(st);
-    st = s->precision == POE2D_F32 ? upload_coeffs<float>(s, c) : upload_coeffs<double>(s, c);
+    st = md->precision == POE2D_F32 ? upload_coeffs<float>(md, c) : upload_coeffs<double>(md, c);
     if (st) return bail(st);
+    poe2d_solver *s = new_handle(md, &st);  // hands the model reference over
+    if (!s) return st;
     *out = s;
+    return POE2D_OK;
+}
+
+poe2d_status poe2d_clone(poe2d_solver *s, poe2d_solver **out)
+{
+    if (!out) return fail(POE2D_E_INVALID, "clone: null output pointer");
+    *out = nullptr;
+    poe2d_status st = check_handle(s);
+    if (st) return st;
+    s->m->refs++;
+    poe2d_solver *c = new_handle(s->m, &st);
+    if (!c) return st;
+    c->schedule = s->schedule;
+    c->block_w = s->block_w;
+    c->tile_rows = s->tile_rows;
+    c->graphs = s->graphs;
+    *out = c;
     return POE2D_OK;
 }
 
@@ -551,12 +914,20 @@ poe2d_status poe2d_set_schedule(poe2d_solver *s, int schedule, int block_w, int 
     return POE2D_OK;
 }
 
+poe2d_status poe2d_set_graphs(poe2d_solver *s, int enable)
+{
+    poe2d_status st = check_handle(s);
+    if (st) return st;
+    s->graphs = enable != 0;
+    return POE2D_OK;
+}
+
 poe2d_status poe2d_material_palette(poe2d_solver *s, int *entries)
 {
     poe2d_status st = check_handle(s);
     if (st) return st;
     if (!entries) return fail(POE2D_E_INVALID, "material_palette: null output");
-    *entries = s->npal;
+    *entries = s->m->npal;
     return POE2D_OK;
 }
 
@@ -604,21 +975,20 @@ poe2d_status poe2d_set_receivers(poe2d_solver *s, const poe2d_receivers *r)
         if (r->ix[k] < 0 || r->ix[k] > s->grid.nx || r->iz[k] < 0 || r->iz[k] > s->grid.nz)
             return fail(POE2D_E_INVALID, "set_receivers: receiver " + std::to_string(k) + " outside the grid");
     HIP_TRY(hipStreamSynchronize(s->stream));
-    void *q = s->rix; if (q) (void)hipFree(q); s->rix = nullptr;
-    q = s->riz; if (q) (void)hipFree(q); s->riz = nullptr;
-    if (s->traces) (void)hipFree(s->traces);
-    s->traces = nullptr;
+    void *q = s->rix; dev_free(q); s->rix = nullptr;
+    q = s->riz; dev_free(q); s->riz = nullptr;
+    dev_free(s->traces);
     s->nrec = r->n;
     s->mask = r->comp_mask;
     s->ncomp = ((r->comp_mask & 1) != 0) + ((r->comp_mask & 2) != 0) + ((r->comp_mask & 4) != 0);
     s->dec = r->decimate;
     s->nsamp = r->nsamp;
     if (s->nrec == 0) return POE2D_OK;
-    if ((st = alloc_zero(s, &q, (size_t)r->n * sizeof(int)))) return st;
+    if ((st = dev_alloc(&q, (size_t)r->n * sizeof(int)))) return st;
     s->rix = (int *)q;
-    if ((st = alloc_zero(s, &q, (size_t)r->n * sizeof(int)))) return st;
+    if ((st = dev_alloc(&q, (size_t)r->n * sizeof(int)))) return st;
     s->riz = (int *)q;
-    if ((st = alloc_zero(s, &s->traces, std::max<size_t>(s->trace_bytes(), 1)))) return st;
+    if ((st = dev_alloc_zero(&s->traces, s->trace_bytes(), s->stream))) return st;
     HIP_TRY(hipStreamSynchronize(s->stream));
     HIP_TRY(hipMemcpy(s->rix, r->ix, (size_t)r->n * sizeof(int), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(s->riz, r->iz, (size_t)r->n * sizeof(int), hipMemcpyHostToDevice));
@@ -700,9 +1070,9 @@ poe2d_status poe2d_checkpoint(poe2d_solver *s, void *blob, size_t bytes)
     if (!blob || bytes < need) return fail(POE2D_E_INVALID, "checkpoint: buffer too small");
     CkptHeader h{};
     std::memcpy(h.magic, "POE2DCK", 8);
-    h.version = 2; h.precision = s->precision; h.nx = s->grid.nx; h.nz = s->grid.nz;
+    h.version = 3; h.precision = s->precision; h.nx = s->grid.nx; h.nz = s->grid.nz;
     h.step = s->step; h.nrec = s->nrec; h.ncomp = s->ncomp; h.nsamp = s->nsamp;
-    h.wxs = s->wxs; h.wzs = s->wzs; h.pitch = s->pitch;
+    h.wxs = s->m->wxs; h.wzs = s->m->wzs; h.pitch = s->m->pitch; h.alloc_rows = s->m->alloc_rows;
     std::memcpy(blob, &h, sizeof h);
     HIP_TRY(hipStreamSynchronize(s->stream));
     unsigned char *p = (unsigned char *)blob + sizeof h;
@@ -724,9 +1094,10 @@ poe2d_status poe2d_restore(poe2d_solver *s, const void *blob, size_t bytes)
     if (!blob || bytes < need) return fail(POE2D_E_INVALID, "restore: blob too small");
     CkptHeader h;
     std::memcpy(&h, blob, sizeof h);
-    if (std::memcmp(h.magic, "POE2DCK", 8) || h.version != 2 || h.precision != s->precision ||
+    if (std::memcmp(h.magic, "POE2DCK", 8) || h.version != 3 || h.precision != s->precision ||
         h.nx != s->grid.nx || h.nz != s->grid.nz || h.nrec != s->nrec || h.ncomp != s->ncomp ||
-        h.nsamp != s->nsamp || h.wxs != s->wxs || h.wzs != s->wzs || h.pitch != s->pitch)
+        h.nsamp != s->nsamp || h.wxs != s->m->wxs || h.wzs != s->m->wzs || h.pitch != s->m->pitch ||
+        h.alloc_rows != s->m->alloc_rows)
         return fail(POE2D_E_INVALID, "restore: checkpoint does not match this solver's geometry");
     HIP_TRY(hipStreamSynchronize(s->stream));
     const unsigned char *p = (const unsigned char *)blob + sizeof h;
@@ -744,9 +1115,8 @@ poe2d_status poe2d_reset(poe2d_solver *s)
 {
     poe2d_status st = check_handle(s);
     if (st) return st;
-    // state groups V0, S0, V1, S1 and the intermediates are contiguous after MS, MV
-    HIP_TRY(hipMemsetAsync(s->group_ptr(G_V0), 0, (NGROUPS - G_V0) * s->group_bytes(), s->stream));
-    HIP_TRY(hipMemsetAsync(s->psi, 0, 2 * s->psi_half_elems() * s->rs, s->stream));
+    HIP_TRY(hipMemsetAsync(s->arena, 0, NSTATE_GROUPS * s->group_bytes(), s->stream));
+    HIP_TRY(hipMemsetAsync(s->psi, 0, 2 * s->m->psi_half_elems() * s->rs, s->stream));
     if (s->traces) HIP_TRY(hipMemsetAsync(s->traces, 0, s->trace_bytes(), s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     s->step = 0;
@@ -793,8 +1163,135 @@ void poe2d_destroy(poe2d_solver *s)
     if (!s) return;
     (void)hipSetDevice(s->device);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
-    free_all(s);
+    free_handle(s);
     delete s;
+}
+
+// ---- batches
+
+poe2d_status poe2d_batch_create(int n, poe2d_solver *const *members, poe2d_batch **out)
+{
+    if (!out) return fail(POE2D_E_INVALID, "batch_create: null output pointer");
+    *out = nullptr;
+    if (n < 1 || !members) return fail(POE2D_E_INVALID, "batch_create: needs at least one member");
+    for (int i = 0; i < n; ++i) {
+        poe2d_solver *s = members[i];
+        if (!s) return fail(POE2D_E_INVALID, "batch_create: null member");
+        if (s->device != members[0]->device || s->precision != members[0]->precision)
+            return fail(POE2D_E_INVALID, "batch_create: members must share device and precision");
+        if (s->schedule == SCHED_TWOPASS)
+            return fail(POE2D_E_INVALID, "batch_create: members must use a single-pass schedule");
+        for (int j = 0; j < i; ++j)
+            if (members[j] == s) return fail(POE2D_E_INVALID, "batch_create: a handle appears twice");
+    }
+    poe2d_batch *b = new (std::nothrow) poe2d_batch;
+    if (!b) return fail(POE2D_E_NOMEM, "batch_create: out of host memory");
+    b->members.assign(members, members + n);
+    b->device = members[0]->device;
+    b->precision = members[0]->precision;
+    b->block_w = members[0]->block_w;
+    b->tile_rows = members[0]->tile_rows;
+    if (hipSetDevice(b->device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
+        poe2d_batch_destroy(b);
+        return fail(POE2D_E_HIP, "batch_create: stream/event creation failed");
+    }
+    *out = b;
+    return POE2D_OK;
+}
+
+poe2d_status poe2d_batch_set_schedule(poe2d_batch *b, int block_w, int tile_rows)
+{
+    poe2d_status st = check_batch(b);
+    if (st) return st;
+    if (block_w != 128 && block_w != 256 && block_w != 512)
+        return fail(POE2D_E_INVALID, "batch_set_schedule: block width must be 128, 256 or 512");
+    if (tile_rows < 4 || tile_rows > 1 << 20) return fail(POE2D_E_INVALID, "batch_set_schedule: tile rows out of range");
+    b->block_w = block_w;
+    b->tile_rows = tile_rows;
+    return POE2D_OK;
+}
+
+poe2d_status poe2d_batch_set_graphs(poe2d_batch *b, int enable)
+{
+    poe2d_status st = check_batch(b);
+    if (st) return st;
+    b->graphs = enable != 0;
+    return POE2D_OK;
+}
+
+poe2d_status poe2d_batch_run(poe2d_batch *b, int nsteps)
+{
+    poe2d_status st = check_batch(b);
+    if (st) return st;
+    if (nsteps < 0) return fail(POE2D_E_INVALID, "batch_run: negative step count");
+    for (poe2d_solver *s : b->members) {
+        if (s->step != b->members[0]->step)
+            return fail(POE2D_E_STATE, "batch_run: members are at different steps");
+        if (s->schedule == SCHED_TWOPASS)
+            return fail(POE2D_E_INVALID, "batch_run: members must use a single-pass schedule");
+    }
+    if (nsteps == 0) return POE2D_OK;
+    return b->precision == POE2D_F32 ? batch_run_steps<float>(b, nsteps) : batch_run_steps<double>(b, nsteps);
+}
+
+poe2d_status poe2d_batch_last_run_ms(poe2d_batch *b, double *ms, int *nsteps)
+{
+    if (!b || !ms || !nsteps) return fail(POE2D_E_INVALID, "batch_last_run_ms: null argument");
+    *ms = b->last_ms;
+    *nsteps = b->last_steps;
+    return POE2D_OK;
+}
+
+poe2d_status poe2d_batch_autotune(poe2d_batch *b, int nsteps, int *block_w, int *tile_rows)
+{
+    poe2d_status st = check_batch(b);
+    if (st) return st;
+    for (poe2d_solver *s : b->members)
+        if (s->step != 0) return fail(POE2D_E_STATE, "batch_autotune: needs every member at step 0");
+    if (nsteps <= 0) nsteps = 10;
+    static const int widths[] = {128, 256, 512};
+    static const int heights[] = {8, 14, 20, 26, 32, 44, 62, 98};
+    int nzmax = 0;
+    for (poe2d_solver *s : b->members) nzmax = std::max(nzmax, s->grid.nz);
+    double best = 0;
+    int best_w = b->block_w, best_h = b->tile_rows;
+    for (int w : widths)
+        for (int h : heights) {
+            if (h > nzmax + 1 && h != heights[0]) continue;
+            b->block_w = w;
+            b->tile_rows = h;
+            if ((st = poe2d_batch_run(b, 1)) || (st = poe2d_batch_run(b, nsteps))) break;
+            if (best == 0 || b->last_ms < best) {
+                best = b->last_ms;
+                best_w = w;
+                best_h = h;
+            }
+        }
+    b->block_w = best_w;
+    b->tile_rows = best_h;
+    poe2d_status rst = POE2D_OK;
+    for (poe2d_solver *s : b->members) {
+        const poe2d_status r = poe2d_reset(s);
+        if (r && !rst) rst = r;
+    }
+    if (st) return st;
+    if (rst) return rst;
+    if (block_w) *block_w = best_w;
+    if (tile_rows) *tile_rows = best_h;
+    return POE2D_OK;
+}
+
+void poe2d_batch_destroy(poe2d_batch *b)
+{
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    b->list.release();
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
 }
 
 }  // extern "C"

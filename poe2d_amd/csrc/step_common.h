@@ -4,12 +4,12 @@
 // array of 4-vectors with row pitch `pitch` (a multiple of 64 points).  Each group holds the four
 // quantities one stage of the update consumes together, so a wave moves 64 x 16 B (FP32) of
 // consecutive memory per load:
-//   MS  (alpha, M, lambda_c, mu)       coefficients of the stress update
-//   MV  (rho, rho_f, m, b)             coefficients of the velocity update
-//   V   (vx, vy, wx, wy)               velocities, double-buffered
-//   S   (e_xx, e_yy, zeta, sigma_xy)   strains, fluid content, shear stress, double-buffered
-//   I   (p, sigma_xx, sigma_yy, -)     intermediates of the two-pass schedule
-// All groups sit in one arena, `group_stride` points apart; every group carries zero pad rows
+//   MS  (alpha, M, lambda_c, mu)       coefficients of the stress update     } material arena,
+//   MV  (rho, rho_f, m, b)             coefficients of the velocity update   } shared by clones
+//   V   (vx, vy, wx, wy)               velocities, double-buffered           } state arena,
+//   S   (e_xx, e_yy, zeta, sigma_xy)   strains, fluid content, shear stress  } one per handle
+//   I   (p, sigma_xx, sigma_yy, -)     intermediates of the two-pass schedule }
+// The groups of an arena sit `group_stride` points apart; every group carries zero pad rows
 // above/below and pad columns left/right of the (nz+1) x (nx+1) grid, and `origin` is the offset
 // of point (0, 0).
 #pragma once
@@ -25,7 +25,27 @@ struct alignas(4 * sizeof(T)) Vec4 {
     T x, y, z, w;
 };
 
-enum { G_MS = 0, G_MV = 1, G_V0 = 2, G_S0 = 3, G_V1 = 4, G_S1 = 5, G_I = 6, NGROUPS = 7 };
+template <class T>
+struct NativeVec4 {
+    typedef T type __attribute__((ext_vector_type(4)));
+};
+template <class T>
+using nvec4 = typename NativeVec4<T>::type;
+
+// Global-address-space views of device pointers.  Pointers read from device memory (the
+// work-list arguments) carry no address space, and loads through them would be flat loads
+// (counted together with LDS traffic, no scalar base register); a kernel casts them once.
+// Global loads and stores of a Vec4 go through the native 4-wide vector type nvec4<T>.
+typedef __attribute__((address_space(1))) char gchar;
+typedef __attribute__((address_space(1))) const unsigned char gcuchar;
+typedef __attribute__((address_space(1))) const int gcint;
+// The constant address space: loads through it of values uniform across a block (tables that no
+// kernel of the launch writes) become scalar loads.
+typedef __attribute__((address_space(4))) const int kcint;
+typedef __attribute__((address_space(4))) const long long kcll;
+
+enum { G_MS = 0, G_MV = 1, NMAT_GROUPS = 2 };                              // material arena
+enum { G_V0 = 0, G_S0 = 1, G_V1 = 2, G_S1 = 3, G_I = 4, NSTATE_GROUPS = 5 };  // state arena
 __host__ __device__ constexpr int group_v(int buf) { return G_V0 + 2 * buf; }
 __host__ __device__ constexpr int group_s(int buf) { return G_S0 + 2 * buf; }
 
@@ -65,12 +85,13 @@ template <class T>
 struct StepArgs {
     int nx, nz, pitch, wxs;
     long long origin;        // offset of point (0,0) inside a group
-    long long group_stride;  // points between consecutive groups
+    long long group_stride;  // points between consecutive groups of an arena
     long long pxn, pzn;      // x-strip / z-strip records per state buffer
     int tn;                  // entries per profile table
     int in, out;             // state buffer read / written by this step
     T rdx, rdz, dt, inv_dt;  // 1/(24 dx), 1/(24 dz), dt, 1/dt
-    Vec4<T> *grp;            // arena of groups
+    Vec4<T> *grp;            // state arena (V, S of both buffers, I)
+    const Vec4<T> *mat;      // material arena (MS, MV)
     T *psi;                  // memory variables
     const T *tab;            // profile tables
     const int *xmap, *zmap;  // strip index of each column / row, -1 outside the strips
@@ -83,19 +104,47 @@ struct StepArgs {
     int npal;
     int src_kind, src_x, src_z;
     T src_val;               // source sample of this step
+    // Work-list launches take the step index n = base + k (base in device memory, k a launch
+    // argument): the state buffer read is (n + par) & 1, and the source sample is
+    // src_series[n - src_n0] for n in [src_n0, src_n0 + src_len), else 0.
+    const T *src_series;
+    long long src_n0;
+    int src_len, par;
     // tiling of the single-pass kernel
     int th, ntx, ntiles, per_xcd;
+    // receivers, gathered by the single-pass kernel: rec_entries holds (x, receiver index) sorted
+    // by tile column, row and x; the entries of row z in tile column tx are
+    // [rec_tile_ptr[tx * (nz+1) + z], rec_tile_ptr[tx * (nz+1) + z + 1]), so the rows of one tile
+    // are contiguous.  Steps n with n % rec_dec == 0 and n / rec_dec < rec_nsamp write slot
+    // n / rec_dec of traces ([slot][comp][receiver], components of rec_mask).
+    const int *rec_tile_ptr;
+    const int2 *rec_entries;
+    T *traces;
+    int rec_n, rec_ncomp, rec_dec, rec_nsamp;
+    unsigned rec_mask;
 };
 
+// Step-dependent fields of a work-list launch at step n.
 template <class T>
-__device__ __forceinline__ const Vec4<T> *group_c(const StepArgs<T> &A, int g)
+__device__ __forceinline__ void at_step(StepArgs<T> &A, long long n)
+{
+    A.in = (int)((n + A.par) & 1);
+    A.out = A.in ^ 1;
+    const long long k = n - A.src_n0;
+    typedef __attribute__((address_space(1))) const T GCT;
+    A.src_val = (A.src_series && k >= 0 && k < A.src_len) ? ((GCT *)A.src_series)[k] : T(0);
+}
+
+// Point (0, 0) of state group g / material group g.
+template <class T>
+__device__ __forceinline__ Vec4<T> *state_group(const StepArgs<T> &A, int g)
 {
     return A.grp + A.origin + (long long)g * A.group_stride;
 }
 template <class T>
-__device__ __forceinline__ Vec4<T> *group_w(const StepArgs<T> &A, int g)
+__device__ __forceinline__ const Vec4<T> *mat_group(const StepArgs<T> &A, int g)
 {
-    return A.grp + A.origin + (long long)g * A.group_stride;
+    return A.mat + A.origin + (long long)g * A.group_stride;
 }
 // Memory-variable record of x-strip record `r` / z-strip record `r` in state buffer `buf`.
 template <class T>
@@ -126,8 +175,8 @@ __device__ __forceinline__ T d4(T f1, T f0, T f2, T fm, T rh)
 
 // C-PML correction of a derivative d: psi_new = b psi_old + a d (stored when `store`), returns
 // d / K + psi_new, with rk = 1/K.
-template <class T>
-__device__ __forceinline__ T cpml(T d, const T *psi_o, T *psi_n, bool store, T a, T b, T rk)
+template <class T, class PO, class PN>
+__device__ __forceinline__ T cpml(T d, PO psi_o, PN psi_n, bool store, T a, T b, T rk)
 {
     const T s = b * *psi_o + a * d;
     if (store) *psi_n = s;
@@ -176,10 +225,21 @@ __device__ __forceinline__ T pore_pressure(T exx, T eyy, T zeta, T al, T M)
 template <class T>
 void launch_twopass(const StepArgs<T> &A, hipStream_t stream, int phase);
 
-// Launches the single-pass step with block width bw (128, 256 or 512 threads); with `palette` the
-// kernel reads the one-byte material index instead of the MS and MV groups (needs A.npal > 0).
+// One work item of the single-pass step: tile `tile` of model `model`.
+struct WorkItem {
+    int model, tile;
+};
+
+// Launches the single-pass step over a work list: block width bw (128, 256 or 512 threads), step
+// index *base + k, model arguments from models[] (each with its own tiling and receivers; in, out
+// and the source sample are set from the step).  With `palette` every model is read through its
+// one-byte material index (needs npal > 0 for each).  nwork items, placed per_xcd to an XCD.
 template <class T>
-hipError_t launch_fused(const StepArgs<T> &A, int bw, bool palette, hipStream_t stream);
+hipError_t launch_fused(const StepArgs<T> *models, const WorkItem *work, int nwork, int per_xcd, int bw,
+                        bool palette, const long long *base, int k, hipStream_t stream);
+
+// *base += delta (one thread).
+hipError_t launch_base_advance(long long *base, int delta, hipStream_t stream);
 
 // Output columns of one tile for block width bw (halo of 4 columns on each side).
 inline int fused_tile_width(int bw) { return bw - 8; }

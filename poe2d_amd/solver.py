@@ -7,6 +7,7 @@ traces and snapshots as numpy arrays.  The same call sequence the reference's ma
 seismograms.
 """
 import ctypes
+import dataclasses
 from dataclasses import dataclass, field
 from typing import Optional
 
@@ -208,6 +209,28 @@ class Solver(HandleSolver):
         2: as 0, always reading the per-point coefficient arrays."""
         self._check(self._lib.poe2d_set_schedule(self._h, schedule, block_w, tile_rows), "set_schedule")
 
+    def set_graphs(self, on=True):
+        """Replay captured 32-step graphs (default) or launch every step directly."""
+        self._check(self._lib.poe2d_set_graphs(self._h, int(bool(on))), "set_graphs")
+
+    def clone(self, source=None, receivers=None, nt=None):
+        """A handle on the same grid and coefficients (shared on the device) with its own
+        wavefield, and `source` / `receivers` set on it."""
+        h = ctypes.c_void_p()
+        self._check(self._lib.poe2d_clone(self._h, ctypes.byref(h)), "clone")
+        c = Solver.__new__(Solver)
+        c._lib = self._lib
+        c.problem = dataclasses.replace(self.problem, source=source, receivers=receivers, wavelet=None,
+                                        nt=nt or self.problem.nt)
+        c.precision = self.precision
+        c.real = self.real
+        c._h = h
+        if source is not None:
+            c.set_source(source)
+        if receivers is not None:
+            c.set_receivers(receivers, c.problem.nt)
+        return c
+
     def autotune(self, nsteps=10):
         """Time the single-pass tile shapes on this grid, keep the fastest, reset the handle;
         returns (block_w, tile_rows)."""
@@ -231,3 +254,54 @@ class Solver(HandleSolver):
     def restore(self, blob):
         blob = np.ascontiguousarray(blob, dtype=np.uint8)
         self._check(self._lib.poe2d_restore(self._h, blob.ctypes.data, blob.nbytes), "restore")
+
+
+class Batch:
+    """Several handles advanced in lock-step, one launch per time step over all of them (ragged
+    grids allowed); each keeps its own results."""
+
+    def __init__(self, solvers):
+        self._lib = _abi.load_library()
+        self.members = list(solvers)
+        arr = (ctypes.c_void_p * len(self.members))(*[m._h.value for m in self.members])
+        h = ctypes.c_void_p()
+        _abi.check(self._lib, self._lib.poe2d_batch_create(len(self.members), arr, ctypes.byref(h)), "batch_create")
+        self._h = h
+
+    def set_schedule(self, block_w=256, tile_rows=14):
+        _abi.check(self._lib, self._lib.poe2d_batch_set_schedule(self._h, block_w, tile_rows), "batch_set_schedule")
+
+    def set_graphs(self, on=True):
+        _abi.check(self._lib, self._lib.poe2d_batch_set_graphs(self._h, int(bool(on))), "batch_set_graphs")
+
+    def run(self, nsteps):
+        _abi.check(self._lib, self._lib.poe2d_batch_run(self._h, int(nsteps)), "batch_run")
+
+    def last_run_ms(self):
+        ms, n = ctypes.c_double(), ctypes.c_int()
+        _abi.check(self._lib, self._lib.poe2d_batch_last_run_ms(self._h, ctypes.byref(ms), ctypes.byref(n)),
+                   "batch_last_run_ms")
+        return ms.value, n.value
+
+    def autotune(self, nsteps=10):
+        w, t = ctypes.c_int(), ctypes.c_int()
+        _abi.check(self._lib, self._lib.poe2d_batch_autotune(self._h, nsteps, ctypes.byref(w), ctypes.byref(t)),
+                   "batch_autotune")
+        return w.value, t.value
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.poe2d_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -1,0 +1,119 @@
+"""Batched launches, graph replays and clones on the GPU.
+
+- Graph replays equal direct launches bitwise, run after run, with snapshots (eager work) and
+  synchronisations between the replays.
+- A batch of ragged models and clones advanced in lock-step equals each member run alone with the
+  same kernels, bitwise (seismograms and wavefields).
+- A clone shares the model and equals its parent for the same source.
+"""
+import numpy as np
+import pytest
+
+from poe2d_amd import _abi
+from poe2d_amd.configs import biot_fields
+from poe2d_amd.material import material_biot
+from poe2d_amd.solver import Batch, Problem, ReceiverSpec, Solver, SourceSpec
+
+from .helpers import random_patchy_mask
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[gpu-test] {msg}", flush=True)
+
+
+def make_problem(nx, nz, seed, src_x=None, kind=_abi.SRC_FORCE_Y_RICKER, nt=250, dec=1):
+    mask = random_patchy_mask(nx, nz, 0.2, seed=seed)
+    coeffs = material_biot(**biot_fields(nx, nz, mask))
+    nrec = 20
+    rec = ReceiverSpec(ix=np.linspace(25, nx - 25, nrec).astype(np.int32), iz=np.full(nrec, nz // 2 + 30, np.int32),
+                       decimate=dec)
+    sx = nx // 2 if src_x is None else src_x
+    return Problem(nx=nx, nz=nz, dx=0.5, dz=0.5, dt=5e-5, nt=nt, coeffs=coeffs,
+                   source=SourceSpec(kind=kind, ix=sx, iz=nz // 2, amp=1e10),
+                   receivers=rec, npml=20, f0=80.0)
+
+
+def state(s):
+    return s.seismograms(), s.snapshot("vy"), s.snapshot("sxy"), s.snapshot("p")
+
+
+def test_graph_replays_equal_direct_launches():
+    prob = make_problem(200, 180, seed=41, kind=_abi.SRC_PRESSURE_DGAUSS, nt=320, dec=3)
+    with Solver(prob) as g, Solver(prob) as d:
+        d.set_graphs(False)
+        for k in (40, 7, 70, 100, 64, 39):
+            g.run(k)
+            d.run(k)
+            a, b = state(g), state(d)
+            say(f"after {g.step_count()} steps: bitwise {all(np.array_equal(x, y) for x, y in zip(a, b))}")
+            for x, y in zip(a, b):
+                assert np.array_equal(x, y)
+        assert np.abs(a[0]).max() > 0
+
+
+def test_batch_equals_single_runs():
+    probs = [make_problem(160, 200, seed=1), make_problem(230, 170, seed=2, kind=_abi.SRC_PRESSURE_DGAUSS),
+             make_problem(199, 199, seed=3, dec=2)]
+    base = [Solver(p) for p in probs]
+    # two shots over the first model, sharing its coefficients on the device
+    shots = [SourceSpec(kind=_abi.SRC_FORCE_Y_RICKER, ix=x, iz=100, amp=1e10) for x in (50, 110)]
+    clones = [base[0].clone(source=src, receivers=probs[0].receivers) for src in shots]
+    members = base + clones
+    with Batch(members) as b:
+        b.set_schedule(256, 14)
+        b.run(100)
+        b.run(150)
+        ms, n = b.last_run_ms()
+        say(f"batch of {len(members)}: {n} steps in {ms:.2f} ms")
+    singles = probs + [Problem(**{**probs[0].__dict__, "source": src}) for src in shots]
+    for i, (m, p) in enumerate(zip(members, singles)):
+        assert m.step_count() == 250
+        with Solver(p) as s:
+            s.set_schedule(0, 256, 26)
+            s.run(250)
+            a, b_ = state(m), state(s)
+            same = all(np.array_equal(x, y) for x, y in zip(a, b_))
+            say(f"member {i} ({p.nx + 1}x{p.nz + 1}): equal to the single run bitwise: {same}")
+            assert np.abs(a[0]).max() > 0
+            for x, y in zip(a, b_):
+                assert np.array_equal(x, y)
+    for m in members:
+        m.close()
+
+
+def test_clone_shares_the_model():
+    prob = make_problem(200, 200, seed=5, nt=200)
+    with Solver(prob) as s:
+        c = s.clone(source=prob.source, receivers=prob.receivers)
+        assert c.material_palette() == s.material_palette() == 2
+        s.run()
+        c.run()
+        for x, y in zip(state(s), state(c)):
+            assert np.array_equal(x, y)
+        c.close()
+
+
+def test_batch_errors_and_autotune():
+    p1, p2 = make_problem(150, 150, seed=7, nt=60), make_problem(170, 140, seed=8, nt=60)
+    a, b = Solver(p1), Solver(p2)
+    with pytest.raises(_abi.Poe2dError):
+        Batch([a, a])
+    b.set_schedule(1)
+    with pytest.raises(_abi.Poe2dError):
+        Batch([a, b])
+    b.set_schedule(0)
+    with Batch([a, b]) as bt:
+        bw, th = bt.autotune(2)
+        say(f"batch autotune: {bw}x{th}")
+        assert a.step_count() == 0 and b.step_count() == 0
+        a.run(3)
+        with pytest.raises(_abi.Poe2dError) as ei:
+            bt.run(5)
+        assert ei.value.status == _abi.POE2D_E_STATE
+        b.run(3)
+        bt.run(10)
+        assert a.step_count() == b.step_count() == 13
+    a.close()
+    b.close()
