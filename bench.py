@@ -6,11 +6,12 @@ with the inputs resident in HBM; set-up, uploads, the tile autotuning and the tr
 outside the timed region.  value = Gcell-updates/s = 4001^2 * 6000 * K / (device time of K solves,
 hipEvents on the solver's stream).
 
-Roofline: the dominant kernel is the single-pass step kernel, one launch per time step.  Its
-algorithmic traffic is 104 B per point-update (9 state fields read + written, 8 coefficients read,
-FP32) plus 48 B per C-PML strip point and 96 B per corner point; `achieved` divides that by the
-kernel's average launch duration, timed with hipEvents over 200 back-to-back launches on the
-solver's stream (a handle without receivers, so the stream holds only step launches).  `traffic`
+Roofline: the dominant kernel is the single-pass step kernel, one launch per time step (it also
+records the receivers; the device step counter advances once per 32 steps).  Its algorithmic
+traffic is 104 B per point-update (9 state fields read + written, 8 coefficients read, FP32) plus
+48 B per C-PML strip point and 96 B per corner point; `achieved` divides that by the time per step
+of the timed solves (hipEvents on the solver's stream around the whole run, so launch gaps count
+against the kernel).  `traffic`
 is the measured HBM bytes per launch (rocprofv3 FETCH_SIZE and WRITE_SIZE, each scaled by what the
 same counter reads for a plain copy of known size), read from the committed profile summary when
 it was taken with the same tile shape, else null.
@@ -78,21 +79,6 @@ def recorded_traffic(tile, n):
         return None, None
 
 
-def time_step_launches(prob, tile, launches=200):
-    """Average duration of one step launch: hipEvents around `launches` back-to-back launches of
-    the step kernel (no receivers, so nothing else is on the stream)."""
-    import dataclasses
-    from poe2d_amd import F32
-    from poe2d_amd.solver import Solver
-    with Solver(dataclasses.replace(prob, receivers=None), precision=F32) as s:
-        s.set_schedule(0, *tile)
-        s.run(20)
-        s.reset()
-        s.run(launches)
-        ms, n = s.last_run_ms()
-    return ms / n
-
-
 def cpu_baseline(prob, steps):
     from oracle.pyoracle import OracleSolver
     s = OracleSolver(prob, threads=1)
@@ -143,13 +129,71 @@ def copy_probe(mib, reps=20):
                                             "write": round(best[2].value, 1)}}), flush=True)
 
 
+def batch_bench(args):
+    """Configuration 2: batch A (64 Ricker shots over one 1201^2 patchy model, clones sharing the
+    model) or batch B (32 patchy models, sides uniform in [300, 1500] points), FP32, advanced in
+    lock-step by one launch per time step with the loop in HIP graphs; tile shape autotuned.
+    value = sum over members of points x nt / device time of one batch run."""
+    from poe2d_amd.configs import config2a, config2b
+    from poe2d_amd.solver import Batch, Solver
+
+    nt = args.nt if args.nt_given else 3000
+    if args.workload == "batchA":
+        base, shots, rec = config2a(nt=nt)
+        first = Solver(base)
+        members = [first] + [first.clone(source=sh, receivers=rec, nt=nt) for sh in shots[1:]]
+        workload = base.meta["workload"] + f", {nt} steps, 101 receivers x (vx, vy, p) per shot"
+        # material amortised over the lock-step shots: 4 B x (2 x 9 + 8 / shots)
+        bytes_per_point = 4.0 * (18 + 8.0 / len(members))
+    else:
+        probs = config2b(nt=nt)
+        members = [Solver(p) for p in probs]
+        workload = f"biot-cpml {len(probs)} patchy models, sides U[300,1500], gas U[0,0.3], {nt} steps, 101 receivers each"
+        bytes_per_point = BYTES_PER_POINT
+    npts = sum((m.problem.nx + 1) * (m.problem.nz + 1) for m in members)
+    with Batch(members) as b:
+        tile = b.autotune(5)
+
+        def one_run():
+            for m in members:
+                m.reset()
+            b.run(nt)
+            return b.last_run_ms()[0]
+
+        for _ in range(args.warmup):
+            one_run()
+        dev_ms = [one_run() for _ in range(args.steps)]
+        finite = all(bool(np.isfinite(m.seismograms()).all()) for m in members)
+    for m in members:
+        m.close()
+    ms = statistics.median(dev_ms)
+    gcell = npts * nt / (ms / 1e3) / 1e9
+    achieved = npts * bytes_per_point / (ms / 1e3 / nt) / 1e9
+    print(json.dumps({
+        "metric": f"Gcell-updates/s (FP32 Biot C-PML, configuration 2 {args.workload}, batched)",
+        "value": round(gcell, 3), "unit": "Gcell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": workload, "members": len(members), "points": npts, "nt": nt,
+                   "tile": f"{tile[0]}x{tile[1]} (autotuned)", "step": "one batch run of nt steps",
+                   "launch": "one work-list launch per time step, 32-step HIP graphs", "traces_finite": finite},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "per_launch": f"one time step of the batch: {bytes_per_point:.1f} B x {npts} points "
+                                   "(algorithmic, strips not counted)"},
+    }), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3, help="timed full solves")
     ap.add_argument("--warmup", type=int, default=1, help="untimed full solves")
     ap.add_argument("--n", type=int, default=4000, help="NX = NZ (grid has n+1 points per side)")
-    ap.add_argument("--nt", type=int, default=6000)
+    ap.add_argument("--nt", type=int, default=None, help="time steps per solve (6000; 3000 for the batches)")
+    ap.add_argument("--workload", default="config1", choices=("config1", "batchA", "batchB"),
+                    help="config1: the 4001^2 single shot (the headline line); batchA / batchB: "
+                         "configuration 2's batches")
     ap.add_argument("--cpu-steps", type=int, default=12, help="time steps of the CPU baseline sample (0: skip)")
     ap.add_argument("--tile", default="auto", metavar="BWxTH",
                     help="tile shape of the step kernel (block width x tile rows); 'auto' runs the "
@@ -159,6 +203,9 @@ def main():
                          "that scales the FETCH_SIZE / WRITE_SIZE counters), then the library's stream "
                          "probe (best copy / read / write rates over MIB MiB buffers)")
     args = ap.parse_args()
+    args.nt_given = args.nt is not None
+    if args.nt is None:
+        args.nt = 6000
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return
@@ -166,6 +213,9 @@ def main():
         print(f"bench.py runs on one GPU only (--gpus {args.gpus} requested): every box here has one "
               "GPU; the multi-GPU slab decomposition is a design, not a measured path.", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "config1":
+        batch_bench(args)
+        return
 
     from poe2d_amd import F32
     from poe2d_amd.configs import config1
@@ -203,8 +253,8 @@ def main():
 
     ms_solve = statistics.median(dev_ms)
     gcell = npts * args.nt / (ms_solve / 1e3) / 1e9
-    launch_ms = time_step_launches(prob, tile)
-    achieved = bytes_step / (launch_ms / 1e3) / 1e9
+    step_ms = ms_solve / args.nt
+    achieved = bytes_step / (step_ms / 1e3) / 1e9
     traffic, traffic_src = recorded_traffic(tile, args.n)
     out = {
         "metric": "Gcell-updates/s (FP32 Biot C-PML time loop, 4001x4001, 6000 steps)",
@@ -229,8 +279,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None if traffic is None else round(traffic),
-                     "kernel": "fused_step (one launch per time step)",
-                     "launch_us": round(launch_ms * 1e3, 2),
+                     "kernel": "fused_step (one launch per time step, 32-step HIP graphs)",
+                     "step_us": round(step_ms * 1e3, 2),
                      "per_launch": f"one time step: {bytes_step / 1e9:.4f} GB algorithmic "
                                    f"(104 B x {npts} points + 48 B x strip points, {cols} strip columns, "
                                    f"{rows} strip rows)",

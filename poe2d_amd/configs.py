@@ -73,3 +73,48 @@ def config1(nt=6000, material_fn=None, n=4000, gas_fraction=0.15, seed=1, mask=N
                    receivers=line_receivers(x0, 4, nrec, c + 40), npml=20, f0=80.0,
                    meta={"config": 1, "workload": f"biot-cpml {n + 1}x{n + 1} patchy gas {gas_fraction}",
                          "gas_fraction_realised": float(np.mean(mask))})
+
+
+def config2a(nt=3000, material_fn=None, n=1200, nshots=64, gas_fraction=0.15, seed=1):
+    """Configuration 2, batch A: `nshots` Ricker force shots over one (n+1)^2 patchy model.
+
+    Returns (problem of the model with shot 0, list of SourceSpec, ReceiverSpec per shot).  The
+    configuration's 10 m spacing (20 points) does not fit 64 shots into the 1201-point model's
+    interior between the C-PML strips, so the shots are 9 m (18 points) apart, centred, on the first
+    row below the top strip; each shot records 101 receivers two rows below, spread over the
+    interior."""
+    base = config1(nt=nt, material_fn=material_fn, n=n, gas_fraction=gas_fraction, seed=seed)
+    npml = base.npml
+    lo, hi = npml + 3, n - npml - 2  # interior columns
+    span = 18 * (nshots - 1)
+    x0 = lo + max(0, (hi - lo - span) // 2)
+    zs = npml + 4
+    shots = [SourceSpec(kind=_abi.SRC_FORCE_Y_RICKER, ix=int(x0 + 18 * k), iz=zs, amp=1.0e10) for k in range(nshots)]
+    rec = line_receivers(lo + 10, (hi - lo - 20) // 100, 101, zs + 2)
+    base.source = shots[0]
+    base.receivers = rec
+    base.meta.update(config="2A", workload=f"biot-cpml {nshots} shots over {n + 1}x{n + 1} patchy gas {gas_fraction}")
+    return base, shots, rec
+
+
+def config2b(nt=3000, material_fn=None, nmodels=32, nmin=300, nmax=1500, seed=1):
+    """Configuration 2, batch B: `nmodels` independent patchy models, nx and nz uniform in
+    [nmin, nmax] points, gas fraction uniform in [0, 0.3]; Ricker force at the centre, 101
+    receivers 40 points below it.  Model k uses seed `seed + k` for its shape and patches."""
+    from .patchy import patchy_mask
+    probs = []
+    for k in range(nmodels):
+        rng = np.random.default_rng(seed + k)
+        nx, nz = (int(v) for v in rng.integers(nmin, nmax + 1, size=2))
+        frac = float(rng.uniform(0.0, 0.3))
+        mask = patchy_mask(nx, nz, 0.5, 0.5, frac, seed + k, 2.0, 20.0) if frac > 0 else None
+        coeffs = coefficients(biot_fields(nx, nz, mask), material_fn)
+        cx, cz = nx // 2, nz // 2
+        nrec = 101
+        step = max(1, (nx - 60) // nrec)
+        x0 = cx - step * (nrec // 2)
+        probs.append(Problem(nx=nx, nz=nz, dx=0.5, dz=0.5, dt=5.0e-5, nt=nt, coeffs=coeffs,
+                             source=SourceSpec(kind=_abi.SRC_FORCE_Y_RICKER, ix=cx, iz=cz, amp=1.0e10),
+                             receivers=line_receivers(x0, step, nrec, min(cz + 40, nz - 25)), npml=20, f0=80.0,
+                             meta={"config": "2B", "gas_fraction": frac, "seed": seed + k}))
+    return probs
