@@ -293,6 +293,42 @@ poe2d_status poe2d_patchy_lattice(int type, int rows, int cols, int r, uint8_t *
 poe2d_status poe2d_stream_probe(size_t bytes, int reps, double *copy_gbps, double *read_gbps,
                                 double *write_gbps);
 
+/* ---- Surrogate network (no counterpart in the reference) ----
+ * An ensemble of fully connected networks sizes[0] -> ... -> sizes[nlayers] with GELU (erf form)
+ * between layers, bf16 weights and activations, FP32 accumulation on the matrix cores, FP32 master
+ * weights and Adam (beta 0.9 / 0.999, eps 1e-8), mean-squared-error loss.  It maps patchy-model
+ * parameters to (decimated, normalised) receiver gathers produced by the solver; the ensemble's
+ * spread decides which predictions the solver re-checks.  Batches are at most max_batch rows
+ * (a multiple of 32); training batches are multiples of 32.  Arrays are host, row-major FP32;
+ * weights are [out][in]. */
+typedef struct poe2d_mlp poe2d_mlp;
+poe2d_status poe2d_mlp_create(int nlayers, const int *sizes, int members, int max_batch, uint64_t seed,
+                              poe2d_mlp **out);
+poe2d_status poe2d_mlp_get_params(poe2d_mlp *m, int member, int layer, float *w, float *b);
+poe2d_status poe2d_mlp_set_params(poe2d_mlp *m, int member, int layer, const float *w, const float *b);
+/* y[n][sizes[nlayers]] = network `member` applied to x[n][sizes[0]]. */
+poe2d_status poe2d_mlp_forward(poe2d_mlp *m, int member, int n, const float *x, float *y);
+/* Loss and gradients (gw[l]: [out][in], gb[l]: [out]; null entries skipped) of member on a batch
+ * of n rows (n a multiple of 32), without updating the weights. */
+poe2d_status poe2d_mlp_gradients(poe2d_mlp *m, int member, int n, const float *x, const float *t, float *loss,
+                                 float *const *gw, float *const *gb);
+/* Upload a training set x[n][sizes[0]], t[n][sizes[nlayers]] (kept on the device). */
+poe2d_status poe2d_mlp_set_dataset(poe2d_mlp *m, int n, const float *x, const float *t);
+/* nsteps Adam steps of member on batches of the dataset rows indices[step * batch + i];
+ * losses[step] (may be null) gets each batch's loss before its update. */
+poe2d_status poe2d_mlp_train(poe2d_mlp *m, int member, int nsteps, int batch, const int *indices, float lr,
+                             float *losses);
+/* Ensemble prediction of n rows: mean[n][sizes[nlayers]] (may be null) and spread[n] (may be
+ * null) = ||standard deviation over members|| / ||mean|| per row. */
+poe2d_status poe2d_mlp_predict(poe2d_mlp *m, int n, const float *x, float *mean, float *spread);
+/* Device time of one forward pass and of one training step (forward, backward, Adam) of member
+ * 0 on a batch of `batch` rows, averaged over iters (a measurement helper). */
+poe2d_status poe2d_mlp_time(poe2d_mlp *m, int batch, int iters, double *forward_ms, double *train_ms);
+/* C[M][N] = A[M][K] B[N][K]^T with A and B rounded to bf16 (round to nearest even) and FP32
+ * accumulation on the matrix cores, ksplit >= 1 K partitions (a test and measurement helper). */
+poe2d_status poe2d_gemm_bf16(int M, int N, int K, const float *A, const float *B, float *C, int ksplit);
+void poe2d_mlp_destroy(poe2d_mlp *m);
+
 /* ---- Reference-format input and output (host only; no device needed) ---- */
 
 /* Layout of a float32 model grid file: x fastest (arr2bin.m:4-9 applied to a z-by-x matrix, the

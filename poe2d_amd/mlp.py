@@ -1,0 +1,121 @@
+"""The surrogate network through the C ABI (poe2d_mlp_*): an ensemble of bf16 fully connected
+networks with GELU, trained with Adam on the GPU's matrix cores."""
+import ctypes
+
+import numpy as np
+
+from . import _abi
+
+
+def _f(a):
+    return a.ctypes.data_as(_abi.c_float_p)
+
+
+class Network:
+    def __init__(self, sizes, members=1, max_batch=256, seed=1):
+        self._lib = _abi.load_library()
+        self.sizes = [int(s) for s in sizes]
+        self.members = int(members)
+        self.max_batch = int(max_batch)
+        arr = (ctypes.c_int * len(self.sizes))(*self.sizes)
+        h = ctypes.c_void_p()
+        _abi.check(self._lib, self._lib.poe2d_mlp_create(len(self.sizes) - 1, arr, self.members, self.max_batch,
+                                                         ctypes.c_uint64(seed), ctypes.byref(h)), "mlp_create")
+        self._h = h
+
+    @property
+    def nlayers(self):
+        return len(self.sizes) - 1
+
+    def get_params(self, member=0):
+        out = []
+        for l in range(self.nlayers):
+            w = np.zeros((self.sizes[l + 1], self.sizes[l]), np.float32)
+            b = np.zeros(self.sizes[l + 1], np.float32)
+            _abi.check(self._lib, self._lib.poe2d_mlp_get_params(self._h, member, l, _f(w), _f(b)), "mlp_get_params")
+            out.append((w, b))
+        return out
+
+    def set_params(self, params, member=0):
+        for l, (w, b) in enumerate(params):
+            w = np.ascontiguousarray(w, np.float32)
+            b = np.ascontiguousarray(b, np.float32)
+            _abi.check(self._lib, self._lib.poe2d_mlp_set_params(self._h, member, l, _f(w), _f(b)), "mlp_set_params")
+
+    def forward(self, x, member=0):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.zeros((x.shape[0], self.sizes[-1]), np.float32)
+        _abi.check(self._lib, self._lib.poe2d_mlp_forward(self._h, member, x.shape[0], _f(x), _f(y)), "mlp_forward")
+        return y
+
+    def gradients(self, x, t, member=0):
+        """(loss, [(dW, db) per layer]) without updating the weights."""
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        gw = [np.zeros((self.sizes[l + 1], self.sizes[l]), np.float32) for l in range(self.nlayers)]
+        gb = [np.zeros(self.sizes[l + 1], np.float32) for l in range(self.nlayers)]
+        pw = (_abi.c_float_p * self.nlayers)(*[_f(g) for g in gw])
+        pb = (_abi.c_float_p * self.nlayers)(*[_f(g) for g in gb])
+        loss = ctypes.c_float()
+        _abi.check(self._lib, self._lib.poe2d_mlp_gradients(self._h, member, x.shape[0], _f(x), _f(t),
+                                                            ctypes.byref(loss), pw, pb), "mlp_gradients")
+        return loss.value, list(zip(gw, gb))
+
+    def set_dataset(self, x, t):
+        self._dx = np.ascontiguousarray(x, np.float32)
+        self._dt = np.ascontiguousarray(t, np.float32)
+        _abi.check(self._lib, self._lib.poe2d_mlp_set_dataset(self._h, self._dx.shape[0], _f(self._dx), _f(self._dt)),
+                   "mlp_set_dataset")
+
+    def train(self, indices, batch, lr, member=0):
+        """Adam steps on dataset rows indices.reshape(-1, batch); returns the losses."""
+        idx = np.ascontiguousarray(indices, np.int32).ravel()
+        nsteps = idx.size // batch
+        losses = np.zeros(nsteps, np.float32)
+        _abi.check(self._lib, self._lib.poe2d_mlp_train(self._h, member, nsteps, batch,
+                                                        idx.ctypes.data_as(_abi.c_int_p), ctypes.c_float(lr),
+                                                        _f(losses)), "mlp_train")
+        return losses
+
+    def predict(self, x, want_mean=True):
+        """(ensemble mean or None, spread per row)."""
+        x = np.ascontiguousarray(x, np.float32)
+        n = x.shape[0]
+        mean = np.zeros((n, self.sizes[-1]), np.float32) if want_mean else None
+        spread = np.zeros(n, np.float32)
+        _abi.check(self._lib, self._lib.poe2d_mlp_predict(self._h, n, _f(x), _f(mean) if want_mean else None,
+                                                          _f(spread)), "mlp_predict")
+        return mean, spread
+
+    def time(self, batch, iters=5):
+        f, t = ctypes.c_double(), ctypes.c_double()
+        _abi.check(self._lib, self._lib.poe2d_mlp_time(self._h, batch, iters, ctypes.byref(f), ctypes.byref(t)),
+                   "mlp_time")
+        return f.value, t.value
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.poe2d_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def gemm_bf16(a, b, ksplit=1):
+    """a [M][K] @ b[N][K]^T with bf16 operands and FP32 accumulation on the matrix cores."""
+    lib = _abi.load_library()
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    c = np.zeros((a.shape[0], b.shape[0]), np.float32)
+    _abi.check(lib, lib.poe2d_gemm_bf16(a.shape[0], b.shape[0], a.shape[1], _f(a), _f(b), _f(c), ksplit), "gemm_bf16")
+    return c
