@@ -184,6 +184,94 @@ def batch_bench(args):
     }), flush=True)
 
 
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def surrogate_bench(args):
+    """Configurations 3 and 4 on the geometry of configuration 3 (401^2 points, 3000 steps decimated
+    to 500 samples, 101 receivers x (vx, vy, p) = 151 500 outputs), with the model counts cut to fit
+    a few minutes (the options say how many).
+
+    The network part times the full-shape ensemble member (4 -> 512 -> 512 -> 512 -> 151 500, bf16
+    on the matrix cores) with hipEvents: one forward pass over an inference batch of 1024 and one
+    training step (forward, backward, Adam) over a batch of 256; FLOPs are 2 x 78.09 M MACs per
+    sample forward and 3 x that per training sample.  The pipeline part draws models, solves them
+    with the batched solver (solver alone: models/s over every candidate), trains the 5-member
+    ensemble, calibrates the spread threshold on held-out models, screens candidates (prediction +
+    spread-triggered re-solves, the models/s of the surrogate) and re-solves a random 5 % of the
+    accepted ones to judge it (timed apart)."""
+    from poe2d_amd import surrogate as S
+    from poe2d_amd.mlp import Network
+
+    geom = S.CONFIG3
+    sizes = [4, 512, 512, 512, int(np.prod(S.gather_shape(geom)))]
+    macs = sum(a * b for a, b in zip(sizes[:-1], sizes[1:]))
+    with Network(sizes, members=1, max_batch=1024) as net:
+        net.time(256, 2)
+        fwd_ms, _ = net.time(1024, 10)
+        _, train_ms = net.time(256, 10)
+    fwd_tflops = 2 * macs * 1024 / (fwd_ms / 1e3) / 1e12
+    train_tflops = 3 * 2 * macs * 256 / (train_ms / 1e3) / 1e12
+    print(f"[surrogate] network: forward(1024) {fwd_ms:.3f} ms = {fwd_tflops:.0f} TFLOP/s; "
+          f"train step(256) {train_ms:.3f} ms = {train_tflops:.0f} TFLOP/s", file=sys.stderr, flush=True)
+
+    ntr, nval, ncand = args.sur_train, args.sur_val, args.sur_cand
+    params = S.draw_params(ntr + nval, seed=3)
+    gathers, solve = S.simulate(params, geom=geom)
+    print(f"[surrogate] solver alone: {solve.n} models in {solve.wall_seconds:.1f} s "
+          f"({solve.models_per_second:.1f}/s; GPU {solve.gpu_models_per_second:.1f}/s)", file=sys.stderr, flush=True)
+    sur = S.Surrogate(geom, members=5, max_batch=1024)
+    t0 = time.perf_counter()
+    sur.fit(params[:ntr], gathers[:ntr], args.sur_steps,
+            log=lambda k, l: print(f"[surrogate] member {k}: loss {l[-20:].mean():.5f}", file=sys.stderr, flush=True))
+    train_s = time.perf_counter() - t0
+    thr, verr, vspread = sur.calibrate(params[ntr:], gathers[ntr:], bound=0.05, quantile=args.sur_quantile)
+    tradeoff = {}
+    for q in (0.5, 0.9, 0.99):
+        t = S.spread_threshold(verr, vspread, 0.05, q)
+        tradeoff[f"p{round(q * 100)}<=5%"] = {"accepted": round(float(np.mean(vspread <= t)), 3)}
+    del gathers
+    cand = S.draw_params(ncand, seed=4)
+    res = sur.screen(cand, thr)
+    sur.close()
+    je = res.judge_errors
+    ffb = len(res.fallback) / res.n
+    value = res.models_per_second
+    print(json.dumps({
+        "metric": "surrogate-screened models/s (configuration 4 screening, configuration 3 geometry)",
+        "value": round(value, 2), "unit": "models/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": round(res.screen_seconds * 1e3, 1), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16 network (FP32 accumulate), f32 solver", "data": "synthetic",
+        "config": {"workload": f"{ncand} candidates screened by a 5-member 4-512-512-512-{sizes[-1]} GELU "
+                               f"ensemble trained {args.sur_steps} steps (batch 256) on {ntr} solver gathers "
+                               f"(401^2, 3000 steps / 6, 101 receivers x 3); threshold for "
+                               f"p{round(args.sur_quantile * 100)} error after fallback <= 5 % on {nval} "
+                               "held-out models",
+                   "step": "one screening pass: prediction + re-solves of the high-spread candidates",
+                   "solver_alone_models_per_s": round(solve.models_per_second, 2),
+                   "solver_alone_gpu_models_per_s": round(solve.gpu_models_per_second, 2),
+                   "speedup_vs_solver": round(value / solve.models_per_second, 2),
+                   "speedup_vs_solver_gpu_time": round(value / solve.gpu_models_per_second, 2),
+                   "fallback_fraction": round(len(res.fallback) / res.n, 4),
+                   "predict_s": round(res.predict_seconds, 3),
+                   "fallback_s": round(res.fallback_timing.wall_seconds, 2),
+                   "judged": len(res.judged), "judge_s": round(res.judge_timing.wall_seconds, 2),
+                   "judged_rel_l2_p50": round(float(np.median(je)), 4) if len(je) else None,
+                   "judged_rel_l2_p99": round(float(np.quantile(je, 0.99)), 4) if len(je) else None,
+                   "after_fallback_rel_l2_p50": round(S.mixture_quantile(je, ffb, 0.5), 4),
+                   "after_fallback_rel_l2_p99": round(S.mixture_quantile(je, ffb, 0.99), 4),
+                   "validation_rel_l2_p50": round(float(np.median(verr)), 4),
+                   "validation_rel_l2_p99": round(float(np.quantile(verr, 0.99)), 4),
+                   "validation_tradeoff": tradeoff, "train_s": round(train_s, 1)},
+        "roofline": {"bound": "mfma", "achieved": round(fwd_tflops, 1), "peak": MFMA_BF16_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(fwd_tflops / MFMA_BF16_PEAK_TFLOPS, 4), "traffic": None,
+                     "per_launch": "one member's forward pass over 1024 samples: 2 x 78.09 M MACs each",
+                     "train_step_tflops": round(train_tflops, 1),
+                     "train_step_frac": round(train_tflops / MFMA_BF16_PEAK_TFLOPS, 4),
+                     "forward_ms": round(fwd_ms, 4), "train_step_ms": round(train_ms, 4)},
+    }), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -191,9 +279,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=1, help="untimed full solves")
     ap.add_argument("--n", type=int, default=4000, help="NX = NZ (grid has n+1 points per side)")
     ap.add_argument("--nt", type=int, default=None, help="time steps per solve (6000; 3000 for the batches)")
-    ap.add_argument("--workload", default="config1", choices=("config1", "batchA", "batchB"),
+    ap.add_argument("--workload", default="config1", choices=("config1", "batchA", "batchB", "surrogate"),
                     help="config1: the 4001^2 single shot (the headline line); batchA / batchB: "
-                         "configuration 2's batches")
+                         "configuration 2's batches; surrogate: configurations 3 and 4")
+    ap.add_argument("--sur-train", type=int, default=1024, help="surrogate: training models")
+    ap.add_argument("--sur-val", type=int, default=256, help="surrogate: held-out models for the threshold")
+    ap.add_argument("--sur-cand", type=int, default=2048, help="surrogate: candidates screened")
+    ap.add_argument("--sur-steps", type=int, default=4000, help="surrogate: training steps per member")
+    ap.add_argument("--sur-quantile", type=float, default=0.99,
+                    help="surrogate: error quantile after fallback held within 5 %% by the threshold")
     ap.add_argument("--cpu-steps", type=int, default=12, help="time steps of the CPU baseline sample (0: skip)")
     ap.add_argument("--tile", default="auto", metavar="BWxTH",
                     help="tile shape of the step kernel (block width x tile rows); 'auto' runs the "
@@ -213,6 +307,9 @@ def main():
         print(f"bench.py runs on one GPU only (--gpus {args.gpus} requested): every box here has one "
               "GPU; the multi-GPU slab decomposition is a design, not a measured path.", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "surrogate":
+        surrogate_bench(args)
+        return
     if args.workload != "config1":
         batch_bench(args)
         return

@@ -7,8 +7,9 @@
 //   backward:  dW_l = G_l^T H_l,  db_l = sum_rows G_l,  G_{l-1} = (G_l W_l) * gelu'(Z_{l-1})
 //   update:    Adam on every member's W_l, b_l (bias-corrected, step count per member).
 // Widths are padded to multiples of 128 in storage (zero rows / columns that stay zero), so every
-// GEMM is an NT product of K-contiguous bf16 rows; G^T and H^T come from transposes, and each
-// layer keeps a transposed bf16 copy of W_l for the G_l W_l product.  Members run one after the
+// GEMM is an NT product of K-contiguous bf16 rows; G^T and H^T come from transposes (the output
+// gradient's from the MSE pass itself), and each layer keeps a transposed bf16 copy of W_l, which
+// the Adam pass writes, for the G_l W_l product.  Members run one after the
 // other on the handle's stream and share the activation buffers.
 #include <hip/hip_runtime.h>
 
@@ -237,11 +238,12 @@ poe2d_status backward(poe2d_mlp *m, Member &e, int n, float *loss)
 {
     const int L = m->L;
     Layer &top = e.layers[L - 1];
-    HIP_TRY(mse(m->y, top.outp, m->tgt, top.outp, n, top.out, m->g16[L - 1], top.outp, m->partial, m->nred,
-                m->stream));
+    // the output gradient, its transpose and the last bias gradient in one pass
+    HIP_TRY(mse(m->y, top.outp, m->tgt, top.outp, n, top.out, m->g16[L - 1], top.outp, m->g16t[L - 1], m->max_batch,
+                top.gb, m->partial, m->stream));
     if (loss) {
-        std::vector<float> part(m->nred);
-        HIP_TRY(hipMemcpyAsync(part.data(), m->partial, m->nred * sizeof(float), hipMemcpyDeviceToHost, m->stream));
+        std::vector<float> part(mse_blocks(top.out));
+        HIP_TRY(hipMemcpyAsync(part.data(), m->partial, part.size() * sizeof(float), hipMemcpyDeviceToHost, m->stream));
         HIP_TRY(hipStreamSynchronize(m->stream));
         double s = 0;
         for (float v : part) s += v;
@@ -250,14 +252,14 @@ poe2d_status backward(poe2d_mlp *m, Member &e, int n, float *loss)
     for (int li = L - 1; li >= 0; --li) {
         Layer &l = e.layers[li];
         // dW = G^T H: A = G^T [outp][n], B = H^T [inp][n]
-        HIP_TRY(transpose_bf16(m->g16[li], l.outp, m->g16t[li], m->max_batch, n, l.outp, m->stream));
+        if (li < L - 1) HIP_TRY(transpose_bf16(m->g16[li], l.outp, m->g16t[li], m->max_batch, n, l.outp, m->stream));
         HIP_TRY(transpose_bf16(m->h16[li], l.inp, m->h16t[li], m->max_batch, n, l.inp, m->stream));
         GemmOut o;
         o.f32 = l.gw;
         o.ldf = l.inp;
         HIP_TRY(gemm_bf16(m->g16t[li], m->max_batch, m->h16t[li], m->max_batch, l.outp, l.inp, n, EPI_LINEAR, o,
                           m->work, m->work_floats, 1, m->stream));
-        HIP_TRY(bias_gradient(m->g16[li], l.outp, n, l.outp, l.gb, m->stream));
+        if (li < L - 1) HIP_TRY(bias_gradient(m->g16[li], l.outp, n, l.outp, l.gb, m->stream));
         if (li == 0) break;
         // G_{l-1} = (G_l W_l) * gelu'(Z_{l-1}): A = G_l [n][outp], B = W_l^T [inp][outp]
         const int K = l.outp;
@@ -280,9 +282,9 @@ poe2d_status adam_all(poe2d_mlp *m, Member &e, float lr)
 {
     e.t += 1;
     for (Layer &l : e.layers) {
-        HIP_TRY(adam_step(l.w, l.mw, l.vw, l.gw, (size_t)l.outp * l.inp, lr, 0.9f, 0.999f, 1e-8f, e.t, l.w16, m->stream));
+        HIP_TRY(adam_matrix_step(l.w, l.mw, l.vw, l.gw, l.outp, l.inp, lr, 0.9f, 0.999f, 1e-8f, e.t, l.w16, l.w16t,
+                                 m->stream));
         HIP_TRY(adam_step(l.b, l.mb, l.vb, l.gb, (size_t)l.outp, lr, 0.9f, 0.999f, 1e-8f, e.t, nullptr, m->stream));
-        HIP_TRY(transpose_bf16(l.w16, l.inp, l.w16t, l.outp, l.outp, l.inp, m->stream));
     }
     return POE2D_OK;
 }
@@ -374,7 +376,7 @@ poe2d_status poe2d_mlp_create(int nlayers, const int *sizes, int members, int ma
     }
     const int outp = pad128(sizes[nlayers]);
     m->work_floats = (size_t)64 * B * hidden_max;
-    m->nred = std::max(m->nred, B);
+    m->nred = std::max({m->nred, B, mse_blocks(sizes[nlayers])});
     if ((st = dmalloc(&m->dh, (size_t)B * std::max(hidden_max, pad128(sizes[0])))) ||
         (st = dmalloc(&m->y, (size_t)B * outp)) || (st = dmalloc(&m->tgt, (size_t)B * outp)) ||
         (st = dmalloc(&m->xin, (size_t)B * sizes[0])) || (st = dmalloc(&m->work, m->work_floats)) ||

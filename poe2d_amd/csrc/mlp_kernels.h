@@ -55,10 +55,12 @@ hipError_t transpose_bf16(const uint16_t *in, int ldi, uint16_t *out, int ldo, i
 // g = dh * gelu'(pre) (bf16), M x N with leading dimension ld (ldg for g).
 hipError_t gelu_bwd(const float *dh, const float *pre, int M, int N, int ld, uint16_t *g, int ldg, hipStream_t stream);
 
-// Mean squared error over M x N: gradient 2 (y - t) / (M N) into g (bf16); per-block sums of
-// squares into partial[nblocks].
-hipError_t mse(const float *y, int ldy, const float *t, int ldt, int M, int N, uint16_t *g, int ldg, float *partial,
-               int nblocks, hipStream_t stream);
+// Mean squared error over M x N: gradient 2 (y - t) / (M N) as bf16 into g and its transpose gt
+// ([N][ldgt]), its column sums (the bias gradient) into db[N], and per-block sums of squares into
+// partial[mse_blocks(N)].
+int mse_blocks(int N);
+hipError_t mse(const float *y, int ldy, const float *t, int ldt, int M, int N, uint16_t *g, int ldg, uint16_t *gt,
+               int ldgt, float *db, float *partial, hipStream_t stream);
 
 // db[n] = sum_m g[m][n].
 hipError_t bias_gradient(const uint16_t *g, int ldg, int M, int N, float *db, hipStream_t stream);
@@ -66,6 +68,11 @@ hipError_t bias_gradient(const uint16_t *g, int ldg, int M, int N, float *db, hi
 // Adam step t (1-based) on n FP32 master weights; their bf16 copy w16 (may be null) follows.
 hipError_t adam_step(float *w, float *m, float *v, const float *g, size_t n, float lr, float b1, float b2, float eps,
                      long long t, uint16_t *w16, hipStream_t stream);
+
+// Adam step t on an R x C weight matrix (R, C multiples of 64) refreshing its bf16 copy w16 and
+// the transposed copy w16t [C][R].
+hipError_t adam_matrix_step(float *w, float *m, float *v, const float *g, int R, int C, float lr, float b1, float b2,
+                            float eps, long long t, uint16_t *w16, uint16_t *w16t, hipStream_t stream);
 
 hipError_t convert_bf16(const float *in, uint16_t *out, size_t n, hipStream_t stream);
 

@@ -176,22 +176,49 @@ __global__ void gelu_backward(const float *__restrict__ dh, const float *__restr
     g[(size_t)m * ldg + n] = bf16_rne(dh[(size_t)m * ld + n] * gelu_grad(pre[(size_t)m * ld + n]));
 }
 
-// MSE over the valid outputs: g = 2 (y - t) / (M * N) as bf16, and per-block partial sums of
-// (y - t)^2 (loss).
-__global__ void mse_grad(const float *__restrict__ y, int ldy, const float *__restrict__ tgt, int ldt, int M, int N,
-                         uint16_t *__restrict__ g, int ldg, float *__restrict__ partial)
+// The kernels below work on 64-column strips with 256 threads: thread (group q = t / 64, lane)
+// owns column c0 + lane and rows q, q + 4, q + 8, ... of each 64-row chunk, so a wave reads 64
+// consecutive elements of a row; 64 x 64 tiles pass through LDS where a transposed copy is written.
+
+// MSE over the valid M x N outputs: g = bf16(2 (y - t) / (M N)) into g and its transpose gt
+// ([N][ldgt]); db[c] = sum over the rows of the bf16 g (the bias gradient); partial[block] = the
+// block's sum of (y - t)^2.
+__global__ void __launch_bounds__(256) mse_grad(const float *__restrict__ y, int ldy, const float *__restrict__ tgt,
+                                                int ldt, int M, int N, uint16_t *__restrict__ g, int ldg,
+                                                uint16_t *__restrict__ gt, int ldgt, float *__restrict__ db,
+                                                float *__restrict__ partial)
 {
+    __shared__ uint16_t tile[64][66];
     __shared__ float red[256];
-    const size_t total = (size_t)M * N;
-    const float scale = 2.0f / (float)total;
-    float acc = 0.f;
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
-        const int m = (int)(i / N), n = (int)(i % N);
-        const float d = y[(size_t)m * ldy + n] - tgt[(size_t)m * ldt + n];
-        acc += d * d;
-        g[(size_t)m * ldg + n] = bf16_rne(scale * d);
+    const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const int c0 = blockIdx.x * 64, c = c0 + lane;
+    const float scale = 2.0f / ((float)M * (float)N);
+    float loss = 0.f, col = 0.f;
+    for (int r0 = 0; r0 < M; r0 += 64) {
+        for (int k = 0; k < 16; ++k) {
+            const int rr = q + 4 * k, r = r0 + rr;
+            uint16_t v = 0;
+            if (r < M && c < N) {
+                const float d = y[(size_t)r * ldy + c] - tgt[(size_t)r * ldt + c];
+                loss += d * d;
+                v = bf16_rne(scale * d);
+                g[(size_t)r * ldg + c] = v;
+                col += bf16_to_f32(v);
+            }
+            tile[rr][lane] = v;
+        }
+        __syncthreads();
+        for (int k = 0; k < 16; ++k) {
+            const int cc = q + 4 * k, r = r0 + lane;
+            if (c0 + cc < N && r < M) gt[(size_t)(c0 + cc) * ldgt + r] = tile[lane][cc];
+        }
+        __syncthreads();
     }
-    red[threadIdx.x] = acc;
+    red[threadIdx.x] = col;
+    __syncthreads();
+    if (q == 0 && c < N) db[c] = (red[lane] + red[lane + 64]) + (red[lane + 128] + red[lane + 192]);
+    __syncthreads();
+    red[threadIdx.x] = loss;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
@@ -200,29 +227,65 @@ __global__ void mse_grad(const float *__restrict__ y, int ldy, const float *__re
     if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
-// db[n] = sum_m g[m][n] for a bf16 gradient (M rows).
-__global__ void bias_grad(const uint16_t *__restrict__ g, int ldg, int M, int N, float *__restrict__ db)
+// db[c] = sum over the M rows of a bf16 gradient g.
+__global__ void __launch_bounds__(256) bias_grad(const uint16_t *__restrict__ g, int ldg, int M, int N,
+                                                 float *__restrict__ db)
 {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
+    __shared__ float red[256];
+    const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const int c = blockIdx.x * 64 + lane;
     float s = 0.f;
-    for (int m = 0; m < M; ++m) s += bf16_to_f32(g[(size_t)m * ldg + n]);
-    db[n] = s;
+    if (c < N)
+        for (int r = q; r < M; r += 4) s += bf16_to_f32(g[(size_t)r * ldg + c]);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    if (q == 0 && c < N) db[c] = (red[lane] + red[lane + 64]) + (red[lane + 128] + red[lane + 192]);
 }
 
-// One Adam step on n master weights (FP32), refreshing their bf16 copy.
+__device__ __forceinline__ float adam_update(float *w, float *mom, float *vel, float g, size_t i, float lr, float b1,
+                                             float b2, float eps, float c1, float c2)
+{
+    const float m = b1 * mom[i] + (1.f - b1) * g;
+    const float v = b2 * vel[i] + (1.f - b2) * g * g;
+    mom[i] = m;
+    vel[i] = v;
+    const float x = w[i] - lr * (m / c1) / (sqrtf(v / c2) + eps);
+    w[i] = x;
+    return x;
+}
+
+// One Adam step on n master weights (FP32), refreshing their bf16 copy (when given).
 __global__ void adam(float *__restrict__ w, float *__restrict__ mom, float *__restrict__ vel, const float *__restrict__ grad,
                      size_t n, float lr, float b1, float b2, float eps, float c1, float c2, uint16_t *__restrict__ w16)
 {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const float g = grad[i];
-        const float m = b1 * mom[i] + (1.f - b1) * g;
-        const float v = b2 * vel[i] + (1.f - b2) * g * g;
-        mom[i] = m;
-        vel[i] = v;
-        const float x = w[i] - lr * (m / c1) / (sqrtf(v / c2) + eps);
-        w[i] = x;
+        const float x = adam_update(w, mom, vel, grad[i], i, lr, b1, b2, eps, c1, c2);
         if (w16) w16[i] = bf16_rne(x);
+    }
+}
+
+// Adam on an R x C weight matrix (row-major, leading dimension C) refreshing its bf16 copy w16
+// and the transposed copy w16t ([C][R]); one 64 x 64 tile per block (R, C multiples of 64).
+__global__ void __launch_bounds__(256) adam_matrix(float *__restrict__ w, float *__restrict__ mom,
+                                                   float *__restrict__ vel, const float *__restrict__ grad, int R, int C,
+                                                   float lr, float b1, float b2, float eps, float c1, float c2,
+                                                   uint16_t *__restrict__ w16, uint16_t *__restrict__ w16t)
+{
+    __shared__ uint16_t tile[64][66];
+    const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const int tiles_c = C / 64;
+    const int r0 = (blockIdx.x / tiles_c) * 64, c0 = (blockIdx.x % tiles_c) * 64;
+    for (int k = 0; k < 16; ++k) {
+        const int rr = q + 4 * k;
+        const size_t i = (size_t)(r0 + rr) * C + c0 + lane;
+        const uint16_t v = bf16_rne(adam_update(w, mom, vel, grad[i], i, lr, b1, b2, eps, c1, c2));
+        w16[i] = v;
+        tile[rr][lane] = v;
+    }
+    __syncthreads();
+    for (int k = 0; k < 16; ++k) {
+        const int cc = q + 4 * k;
+        w16t[(size_t)(c0 + cc) * R + r0 + lane] = tile[lane][cc];
     }
 }
 
@@ -272,16 +335,18 @@ hipError_t gelu_bwd(const float *dh, const float *pre, int M, int N, int ld, uin
     return hipGetLastError();
 }
 
-hipError_t mse(const float *y, int ldy, const float *t, int ldt, int M, int N, uint16_t *g, int ldg, float *partial,
-               int nblocks, hipStream_t stream)
+int mse_blocks(int N) { return (N + 63) / 64; }
+
+hipError_t mse(const float *y, int ldy, const float *t, int ldt, int M, int N, uint16_t *g, int ldg, uint16_t *gt,
+               int ldgt, float *db, float *partial, hipStream_t stream)
 {
-    mse_grad<<<nblocks, 256, 0, stream>>>(y, ldy, t, ldt, M, N, g, ldg, partial);
+    mse_grad<<<mse_blocks(N), 256, 0, stream>>>(y, ldy, t, ldt, M, N, g, ldg, gt, ldgt, db, partial);
     return hipGetLastError();
 }
 
 hipError_t bias_gradient(const uint16_t *g, int ldg, int M, int N, float *db, hipStream_t stream)
 {
-    bias_grad<<<(N + 255) / 256, 256, 0, stream>>>(g, ldg, M, N, db);
+    bias_grad<<<(N + 63) / 64, 256, 0, stream>>>(g, ldg, M, N, db);
     return hipGetLastError();
 }
 
@@ -290,6 +355,15 @@ hipError_t adam_step(float *w, float *m, float *v, const float *g, size_t n, flo
 {
     const float c1 = 1.f - powf(b1, (float)t), c2 = 1.f - powf(b2, (float)t);
     adam<<<grid_for(n), 256, 0, stream>>>(w, m, v, g, n, lr, b1, b2, eps, c1, c2, w16);
+    return hipGetLastError();
+}
+
+hipError_t adam_matrix_step(float *w, float *m, float *v, const float *g, int R, int C, float lr, float b1, float b2,
+                            float eps, long long t, uint16_t *w16, uint16_t *w16t, hipStream_t stream)
+{
+    if (R % 64 || C % 64) return hipErrorInvalidValue;
+    const float c1 = 1.f - powf(b1, (float)t), c2 = 1.f - powf(b2, (float)t);
+    adam_matrix<<<(R / 64) * (C / 64), 256, 0, stream>>>(w, m, v, g, R, C, lr, b1, b2, eps, c1, c2, w16, w16t);
     return hipGetLastError();
 }
 

@@ -1,0 +1,269 @@
+"""Surrogate screening of patchy models (configurations 3 and 4).
+
+A model is four parameters: gas fraction U[0, 0.3], patch radius log-U[2, 20] m (one radius per
+model), porosity U[0.15, 0.35] and permeability log-U[1e-14, 1e-11] m^2.  Its gather is what the
+solver records for it: the brine rock of configuration 0 with that porosity and permeability, gas
+patches of that radius painted at seeded random centres up to that fraction, a Ricker vertical
+force at the centre and a line of receivers below it recording v_x, v_y and p, decimated in time.
+
+The patch centres come from one fixed seed (LAYOUT_SEED), so a model is a function of its four
+parameters.  With a fresh seed per model the gather would also depend on where the patches fell,
+which the four inputs do not say: two layouts with the same parameters differ by about 50 %
+relative L2 (pore pressure at a receiver drops about 40-fold when a gas patch covers it), far
+beyond what any surrogate of the parameters could reach.
+
+The network (4 -> 512 -> 512 -> 512 -> outputs, GELU, bf16 on the matrix cores, 5 members) is
+trained on gathers the solver itself produced, each component scaled by its RMS over the training
+set.  Screening predicts every candidate; a candidate whose ensemble spread exceeds the threshold
+is re-solved with the batched solver (fallback), and a random 5 % of the accepted ones are
+re-solved as well to judge the surrogate (timed apart from the screening).  Errors are relative L2
+over the whole gather after the per-component scaling; after fallback, a re-solved model's error
+is zero.
+
+CONFIG3 is the geometry of configuration 3: 401 x 401 points at 0.5 m, 3000 steps of 5e-5 s
+decimated by 6 to 500 samples, 101 receivers at x = 50 + 3k, 40 points below the source (3 x 101 x
+500 = 151 500 outputs).  SMALL is a reduced geometry with the same layout for quick tests.
+"""
+import time
+from concurrent.futures import ThreadPoolExecutor
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _abi
+from .configs import BRINE, GAS, biot_fields
+from .material import material_biot
+from .mlp import Network
+from .patchy import patchy_mask
+from .solver import Batch, Problem, ReceiverSpec, Solver, SourceSpec
+
+CONFIG3 = dict(n=400, dx=0.5, dt=5e-5, nt=3000, dec=6, npml=20, nrec=101, rec_dx=3, rec_dz=40, f0=80.0)
+LAYOUT_SEED = 1
+SMALL = dict(n=120, dx=0.5, dt=5e-5, nt=600, dec=6, npml=20, nrec=21, rec_dx=3, rec_dz=20, f0=80.0)
+RANGES = dict(gas=(0.0, 0.3), radius=(2.0, 20.0), phi=(0.15, 0.35), kappa=(1e-14, 1e-11))
+
+
+def draw_params(n, seed):
+    """n parameter sets (gas, radius [m], phi, kappa [m^2])."""
+    rng = np.random.default_rng(seed)
+    gas = rng.uniform(*RANGES["gas"], n)
+    radius = np.exp(rng.uniform(np.log(RANGES["radius"][0]), np.log(RANGES["radius"][1]), n))
+    phi = rng.uniform(*RANGES["phi"], n)
+    kappa = np.exp(rng.uniform(np.log(RANGES["kappa"][0]), np.log(RANGES["kappa"][1]), n))
+    return np.stack([gas, radius, phi, kappa], axis=1)
+
+
+INPUT_SCALE = 4.0
+
+
+def features(params):
+    """Network inputs: each parameter mapped linearly (radius, kappa: in log) onto [-1, 1], times
+    INPUT_SCALE (wider inputs let the first layer form the sharp changes that patches crossing a
+    receiver make)."""
+    p = np.asarray(params, np.float64)
+    lo_r, hi_r = np.log(RANGES["radius"][0]), np.log(RANGES["radius"][1])
+    lo_k, hi_k = np.log(RANGES["kappa"][0]), np.log(RANGES["kappa"][1])
+    f = np.stack([p[:, 0] / 0.15 - 1.0,
+                  2.0 * (np.log(p[:, 1]) - lo_r) / (hi_r - lo_r) - 1.0,
+                  (p[:, 2] - 0.25) / 0.1,
+                  2.0 * (np.log(p[:, 3]) - lo_k) / (hi_k - lo_k) - 1.0], axis=1)
+    return (INPUT_SCALE * f).astype(np.float32)
+
+
+def model_problem(p, seed=LAYOUT_SEED, geom=CONFIG3):
+    gas, radius, phi, kappa = (float(v) for v in p)
+    n, dx = geom["n"], geom["dx"]
+    mask = patchy_mask(n, n, dx, dx, gas, seed, radius, radius) if gas > 0 else None
+    rock = dict(BRINE, phi=phi, kappa=kappa)
+    fluid = dict(GAS, phi=phi, kappa=kappa)
+    coeffs = material_biot(**biot_fields(n, n, mask, rock, fluid))
+    c = n // 2
+    ix, iz = receiver_points(geom)
+    rec = ReceiverSpec(ix=ix, iz=iz, decimate=geom["dec"])
+    return Problem(nx=n, nz=n, dx=dx, dz=dx, dt=geom["dt"], nt=geom["nt"], coeffs=coeffs,
+                   source=SourceSpec(kind=_abi.SRC_FORCE_Y_RICKER, ix=c, iz=c, amp=1.0e10),
+                   receivers=rec, npml=geom["npml"], f0=geom["f0"])
+
+
+def receiver_points(geom=CONFIG3):
+    c = geom["n"] // 2
+    k = np.arange(geom["nrec"])
+    ix = (c - geom["rec_dx"] * (geom["nrec"] // 2) + geom["rec_dx"] * k).astype(np.int32)
+    return ix, np.full(geom["nrec"], c + geom["rec_dz"], np.int32)
+
+
+def gather_shape(geom=CONFIG3):
+    return (3, geom["nrec"], -(-geom["nt"] // geom["dec"]))
+
+
+@dataclass
+class SolveTiming:
+    n: int
+    wall_seconds: float     # model building on the host (overlapped) + GPU solves + read-back
+    gpu_seconds: float      # time-step launches only, as the batches report it
+
+    @property
+    def models_per_second(self):
+        return self.n / self.wall_seconds
+
+    @property
+    def gpu_models_per_second(self):
+        return self.n / self.gpu_seconds
+
+
+def simulate(params, seeds=None, geom=CONFIG3, batch=64, workers=8):
+    """Gathers [n, 3, nrec, nsamp] (v_x, v_y, p) of the models, solved in batches of `batch`
+    handles advanced together while host threads build the next models; returns (gathers,
+    SolveTiming).  seeds: patch-layout seed per model (LAYOUT_SEED for all when None)."""
+    t0 = time.perf_counter()
+    n = len(params)
+    seeds = np.full(n, LAYOUT_SEED) if seeds is None else np.asarray(seeds)
+    out = np.zeros((n,) + gather_shape(geom), np.float32)
+    gpu_ms = 0.0
+    build = lambda k: model_problem(params[k], seeds[k], geom)
+    with ThreadPoolExecutor(workers) as pool:
+        pending = [pool.submit(build, k) for k in range(min(batch, n))]
+        for i in range(0, n, batch):
+            problems = [f.result() for f in pending]
+            pending = [pool.submit(build, k) for k in range(i + batch, min(i + 2 * batch, n))]
+            solvers = [Solver(p) for p in problems]
+            del problems
+            try:
+                with Batch(solvers) as b:
+                    b.run(geom["nt"])
+                    gpu_ms += b.last_run_ms()[0]
+                for k, s in enumerate(solvers):
+                    out[i + k] = s.seismograms()
+            finally:
+                for s in solvers:
+                    s.close()
+    return out, SolveTiming(n, time.perf_counter() - t0, gpu_ms * 1e-3)
+
+
+def after_fallback(err, spread, threshold):
+    """Error of each model once screening is done: the surrogate's error where its spread is at most
+    `threshold`, zero where the solver re-solved it."""
+    return np.where(np.asarray(spread) <= threshold, err, 0.0)
+
+
+def mixture_quantile(judge_errors, fallback_fraction, q):
+    """Quantile q of the errors after fallback over all candidates, from the judged sample of the
+    accepted ones: a share `fallback_fraction` is exact (zero error), the rest distributed like the
+    judged errors."""
+    if q <= fallback_fraction or len(judge_errors) == 0:
+        return 0.0
+    return float(np.quantile(judge_errors, (q - fallback_fraction) / (1.0 - fallback_fraction)))
+
+
+def spread_threshold(err, spread, bound=0.05, quantile=0.99):
+    """Largest spread threshold (one of the given spreads) at which the `quantile` of the errors after
+    fallback stays within `bound`; -inf when even the smallest spread exceeds it (everything falls
+    back)."""
+    err = np.asarray(err, np.float64)
+    spread = np.asarray(spread, np.float64)
+    best = -np.inf
+    for t in np.unique(spread):
+        if np.quantile(after_fallback(err, spread, t), quantile) <= bound:
+            best = t
+    return float(best)
+
+
+@dataclass
+class ScreenResult:
+    n: int
+    fallback: np.ndarray        # indices re-solved because of their spread
+    judged: np.ndarray          # indices re-solved to judge the surrogate
+    judge_errors: np.ndarray    # surrogate error on the judged models
+    screen_seconds: float       # prediction + fallback re-solves (wall)
+    predict_seconds: float
+    fallback_timing: SolveTiming
+    judge_timing: SolveTiming
+
+    @property
+    def models_per_second(self):
+        return self.n / self.screen_seconds
+
+
+class Surrogate:
+    """Ensemble network from features(params) to a model's gather.  The network's targets are the
+    gathers with each component divided by its RMS over the training set."""
+
+    def __init__(self, geom=CONFIG3, members=5, hidden=512, max_batch=1024, seed=1):
+        self.geom = dict(geom)
+        self.shape = gather_shape(geom)
+        self.nout = int(np.prod(self.shape))
+        self.net = Network([4, hidden, hidden, hidden, self.nout], members=members, max_batch=max_batch, seed=seed)
+        self.scale = np.ones(self.shape[0], np.float32)    # RMS of each component of the gathers
+
+    def _scaled(self, g):
+        return (g / self.scale[None, :, None, None]).reshape(len(g), -1)
+
+    def fit(self, params, gathers, steps, batch=256, lr=1e-3, lr_final=1e-5, chunks=20, seed=1, log=None):
+        """Train every member on its own shuffles of the data, the learning rate falling from `lr`
+        to `lr_final` on a cosine over `chunks` pieces; returns the loss curves.  log(member,
+        losses) is called after each member."""
+        g = np.asarray(gathers, np.float32)
+        rms = np.sqrt(np.mean(g.astype(np.float64) ** 2, axis=(0, 2, 3)))
+        self.scale = np.where(rms > 0, rms, 1.0).astype(np.float32)
+        self.net.set_dataset(features(params), self._scaled(g))
+        rng = np.random.default_rng(seed)
+        n = len(params)
+        cuts = np.linspace(0, steps, chunks + 1).round().astype(int)
+        curves = []
+        for member in range(self.net.members):
+            need = steps * batch
+            idx = np.concatenate([rng.permutation(n) for _ in range(need // n + 1)])[:need]
+            losses = []
+            for c in range(chunks):
+                if cuts[c + 1] == cuts[c]:
+                    continue
+                rate = lr_final + 0.5 * (lr - lr_final) * (1 + np.cos(np.pi * c / chunks))
+                losses.append(self.net.train(idx[cuts[c] * batch:cuts[c + 1] * batch], batch, rate, member))
+            curves.append(np.concatenate(losses))
+            if log:
+                log(member, curves[-1])
+        return curves
+
+    def predict(self, params):
+        """(gathers [n, 3, nrec, nsamp], ensemble spread [n] in units of the scaled targets)."""
+        mean, spread = self.net.predict(features(params))
+        g = mean.reshape((len(params),) + self.shape)
+        g *= self.scale[None, :, None, None]
+        return g, spread
+
+    def errors(self, pred, true):
+        """Relative L2 per model over the whole gather, each component divided by its RMS over the
+        training set."""
+        a, b = self._scaled(pred), self._scaled(true)
+        return np.linalg.norm(a - b, axis=1) / np.maximum(np.linalg.norm(b, axis=1), 1e-30)
+
+    def calibrate(self, params, gathers, bound=0.05, quantile=0.99):
+        """(threshold, errors, spreads) on a validation set: spread_threshold of its errors."""
+        pred, spread = self.predict(params)
+        err = self.errors(pred, gathers)
+        return spread_threshold(err, spread, bound, quantile), err, spread
+
+    def screen(self, params, threshold, seeds=None, judge_frac=0.05, seed=7):
+        """Predict every candidate, re-solve those whose spread exceeds `threshold`, then re-solve
+        a random `judge_frac` of the others to measure the surrogate's error (timed apart)."""
+        t0 = time.perf_counter()
+        pred, spread = self.predict(params)
+        t_pred = time.perf_counter() - t0
+        fallback = np.nonzero(spread > threshold)[0]
+        fb = SolveTiming(0, 0.0, 0.0)
+        if len(fallback):
+            _, fb = simulate(params[fallback], None if seeds is None else seeds[fallback], self.geom)
+        t_screen = time.perf_counter() - t0
+        accepted = np.setdiff1d(np.arange(len(params)), fallback)
+        rng = np.random.default_rng(seed)
+        nj = max(1, int(round(judge_frac * len(accepted)))) if len(accepted) else 0
+        judged = np.sort(rng.choice(accepted, nj, replace=False)) if nj else np.zeros(0, int)
+        err = np.zeros(0)
+        jt = SolveTiming(0, 0.0, 0.0)
+        if nj:
+            true, jt = simulate(params[judged], None if seeds is None else seeds[judged], self.geom)
+            err = self.errors(pred[judged], true)
+        return ScreenResult(len(params), fallback, judged, err, t_screen, t_pred, fb, jt)
+
+    def close(self):
+        self.net.close()

@@ -1,0 +1,75 @@
+"""Surrogate screening on the GPU at the reduced geometry (121 x 121 points, 600 steps decimated by
+6, 21 receivers): data from the batched solver, ensemble training, the spread threshold, and what
+screening gains over solving every candidate."""
+import numpy as np
+import pytest
+
+from poe2d_amd import surrogate as S
+from poe2d_amd.solver import Problem, Solver
+
+pytestmark = pytest.mark.gpu
+
+GEOM = S.SMALL
+
+
+def say(msg):
+    print(f"[gpu-test] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def trained():
+    say("surrogate: simulate 1280 models")
+    params = S.draw_params(1280, seed=21)
+    gathers, timing = S.simulate(params, geom=GEOM)
+    say(f"solver alone: {timing.models_per_second:.0f} models/s wall, {timing.gpu_models_per_second:.0f} on the GPU")
+    sur = S.Surrogate(GEOM, members=5, max_batch=1024)
+    curves = sur.fit(params[:1024], gathers[:1024], steps=3000, lr=1e-3)
+    say("losses " + ", ".join(f"{c[:20].mean():.3f} -> {c[-20:].mean():.4f}" for c in curves))
+    yield sur, params, gathers, timing, curves
+    sur.close()
+
+
+def test_simulate_matches_single_solves(trained):
+    _, params, gathers, _, _ = trained
+    for k in (0, 777):
+        with Solver(S.model_problem(params[k], geom=GEOM)) as s:
+            s.run()
+            assert np.array_equal(s.seismograms(), gathers[k])
+
+
+def test_training_converges_and_generalises(trained):
+    sur, params, gathers, _, curves = trained
+    for c in curves:
+        assert np.all(np.isfinite(c)) and c[-20:].mean() < 0.05 * c[:20].mean()
+    pred, spread = sur.predict(params[1024:])
+    err = sur.errors(pred, gathers[1024:])
+    say(f"validation rel L2 p50 {np.median(err):.4f} p90 {np.quantile(err, 0.9):.4f}; spread p50 {np.median(spread):.4f}")
+    assert np.median(err) < 0.12
+    # the spread ranks the errors: the low-spread half is the more accurate one
+    low = spread <= np.median(spread)
+    assert np.median(err[low]) < np.median(err[~low])
+
+
+def test_screening_gains_over_the_solver_alone(trained):
+    sur, params, gathers, timing, _ = trained
+    thr, err, spread = sur.calibrate(params[1024:], gathers[1024:], bound=0.05, quantile=0.5)
+    say(f"threshold {thr:.4f} accepts {np.mean(spread <= thr):.2f} of the validation models")
+    assert np.isfinite(thr)
+    cand = S.draw_params(512, seed=22)
+    res = sur.screen(cand, thr, judge_frac=0.25)
+    n_acc = res.n - len(res.fallback)
+    ffb = len(res.fallback) / res.n
+    p50 = S.mixture_quantile(res.judge_errors, ffb, 0.5)
+    say(f"screened {res.n}: {len(res.fallback)} re-solved, {n_acc} accepted, {len(res.judged)} judged; "
+        f"{res.models_per_second:.0f} models/s vs {timing.models_per_second:.0f} solving all "
+        f"(predict {res.predict_seconds:.3f} s); judged p50 {np.median(res.judge_errors):.4f}, "
+        f"p50 after fallback {p50:.4f}")
+    assert len(res.judged) == max(1, round(0.25 * n_acc))
+    assert not set(res.judged) & set(res.fallback)
+    assert res.models_per_second > 1.5 * timing.models_per_second
+    # the threshold holds the median error after fallback near 5 % (sampling margin) ...
+    assert p50 <= 0.065
+    # ... and the spread sends the worse models to the solver
+    assert np.median(res.judge_errors) < np.median(err)
+    # the surrogate's own rate, fallback aside
+    assert res.n / res.predict_seconds > 20 * timing.gpu_models_per_second

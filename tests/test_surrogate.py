@@ -1,0 +1,105 @@
+"""Host side of the surrogate screening (no GPU): parameter draws, network inputs, the models built
+from parameters, and the choice of the spread threshold."""
+import numpy as np
+import pytest
+
+from poe2d_amd import surrogate as S
+
+
+def test_draws_cover_the_ranges():
+    p = S.draw_params(20000, seed=5)
+    gas, radius, phi, kappa = p.T
+    assert gas.min() >= 0 and gas.max() <= 0.3
+    assert radius.min() >= 2 and radius.max() <= 20
+    assert phi.min() >= 0.15 and phi.max() <= 0.35
+    assert kappa.min() >= 1e-14 and kappa.max() <= 1e-11
+    # log-uniform: the median sits at the geometric mean of the ends
+    assert abs(np.median(np.log(radius)) - np.log(np.sqrt(40.0))) < 0.05
+    assert abs(np.median(np.log10(kappa)) + 12.5) < 0.05
+    assert abs(np.median(gas) - 0.15) < 0.01
+    assert np.array_equal(p, S.draw_params(20000, seed=5))
+
+
+def test_features_map_range_ends_to_the_input_scale():
+    lo = [[0.0, 2.0, 0.15, 1e-14]]
+    hi = [[0.3, 20.0, 0.35, 1e-11]]
+    mid = [[0.15, np.sqrt(40.0), 0.25, 10 ** -12.5]]
+    np.testing.assert_allclose(S.features(lo), -S.INPUT_SCALE, atol=1e-6)
+    np.testing.assert_allclose(S.features(hi), S.INPUT_SCALE, atol=1e-6)
+    np.testing.assert_allclose(S.features(mid), 0.0, atol=1e-6)
+
+
+def test_model_is_a_function_of_its_parameters():
+    geom = S.SMALL
+    p = np.array([0.2, 5.0, 0.3, 1e-13])
+    a, b = S.model_problem(p, geom=geom), S.model_problem(p, geom=geom)
+    for k in a.coeffs:
+        assert np.array_equal(a.coeffs[k], b.coeffs[k])
+    c = S.model_problem(np.array([0.2, 6.0, 0.3, 1e-13]), geom=geom)
+    assert not np.array_equal(a.coeffs["M"], c.coeffs["M"])
+    # the gas fraction is reached
+    gas = np.isclose(a.coeffs["M"], a.coeffs["M"].min())
+    assert 0.2 <= gas.mean() < 0.25
+    # a different layout seed moves the patches
+    d = S.model_problem(p, seed=S.LAYOUT_SEED + 1, geom=geom)
+    assert not np.array_equal(a.coeffs["M"], d.coeffs["M"])
+    # no gas: the plain rock with the drawn porosity and permeability
+    e = S.model_problem(np.array([0.0, 5.0, 0.3, 1e-13]), geom=geom)
+    assert np.ptp(e.coeffs["M"]) == 0 and np.ptp(e.coeffs["b"]) == 0
+
+
+@pytest.mark.parametrize("geom", [S.SMALL, S.CONFIG3])
+def test_geometry(geom):
+    prob = S.model_problem(np.array([0.1, 4.0, 0.25, 1e-12]), geom=geom)
+    n, npml = geom["n"], geom["npml"]
+    rx, rz = prob.receivers.ix, prob.receivers.iz
+    assert rx.min() > npml and rx.max() < n - npml and np.all(rz < n - npml)
+    assert prob.source.ix == n // 2 and prob.source.iz == n // 2
+    assert S.gather_shape(geom) == (3, geom["nrec"], geom["nt"] // geom["dec"])
+    if geom is S.CONFIG3:
+        assert S.gather_shape(geom) == (3, 101, 500) and np.prod(S.gather_shape(geom)) == 151500
+        assert rx[0] == 50 and rx[-1] == 350 and rz[0] == 240
+
+
+def test_after_fallback_zeroes_the_re_solved_models():
+    err = np.array([0.01, 0.2, 0.03])
+    spread = np.array([0.1, 0.5, 0.2])
+    np.testing.assert_array_equal(S.after_fallback(err, spread, 0.2), [0.01, 0.0, 0.03])
+    np.testing.assert_array_equal(S.after_fallback(err, spread, -np.inf), [0, 0, 0])
+
+
+def test_spread_threshold_known_cases():
+    err = np.array([0.01, 0.02, 0.2, 0.03])
+    spread = np.array([0.1, 0.2, 0.3, 0.4])
+    assert S.spread_threshold(err, spread, 0.05, 0.99) == 0.2   # accepting model 2 breaks p99
+    assert S.spread_threshold(err, spread, 0.05, 0.5) == 0.4    # the median holds with all accepted
+    assert S.spread_threshold(np.full(4, 0.5), spread) == -np.inf
+
+
+def test_spread_threshold_is_the_largest_that_keeps_the_bound():
+    rng = np.random.default_rng(3)
+    for trial in range(20):
+        n = 50
+        spread = rng.uniform(0, 1, n)
+        err = np.clip(0.08 * spread + rng.normal(0, 0.02, n), 0, None)
+        for q in (0.5, 0.9, 0.99):
+            t = S.spread_threshold(err, spread, 0.05, q)
+            ok = lambda th: np.quantile(np.where(spread <= th, err, 0.0), q) <= 0.05
+            if np.isfinite(t):
+                assert ok(t)
+            assert not any(ok(th) for th in spread[spread > t])
+
+
+def test_mixture_quantile():
+    je = np.linspace(0.0, 0.1, 101)
+    assert S.mixture_quantile(je, 0.5, 0.4) == 0.0
+    assert S.mixture_quantile(je, 0.0, 0.5) == pytest.approx(0.05)
+    # half re-solved: the overall p75 is the judged median
+    assert S.mixture_quantile(je, 0.5, 0.75) == pytest.approx(0.05)
+    assert S.mixture_quantile(np.zeros(0), 0.1, 0.99) == 0.0
+    # agrees with the quantile of an explicit mixture
+    rng = np.random.default_rng(0)
+    e = rng.uniform(0, 0.2, 4000)
+    full = np.concatenate([e, np.zeros(1000)])
+    for q in (0.3, 0.5, 0.9, 0.99):
+        assert abs(S.mixture_quantile(e, 0.2, q) - np.quantile(full, q)) < 2e-3
