@@ -215,6 +215,24 @@ def surrogate_bench(args):
     print(f"[surrogate] network: forward(1024) {fwd_ms:.3f} ms = {fwd_tflops:.0f} TFLOP/s; "
           f"train step(256) {train_ms:.3f} ms = {train_tflops:.0f} TFLOP/s", file=sys.stderr, flush=True)
 
+    net_roofline = {"bound": "mfma", "achieved": round(fwd_tflops, 1), "peak": MFMA_BF16_PEAK_TFLOPS,
+                    "unit": "TFLOP/s", "frac": round(fwd_tflops / MFMA_BF16_PEAK_TFLOPS, 4), "traffic": None,
+                    "per_launch": "one member's forward pass over 1024 samples: 2 x 78.09 M MACs each",
+                    "train_step_tflops": round(train_tflops, 1),
+                    "train_step_frac": round(train_tflops / MFMA_BF16_PEAK_TFLOPS, 4),
+                    "forward_ms": round(fwd_ms, 4), "train_step_ms": round(train_ms, 4)}
+    if args.sur_train == 0:
+        print(json.dumps({
+            "metric": "surrogate network TFLOP/s (bf16 forward pass, batch 1024)", "value": round(fwd_tflops, 1),
+            "unit": "TFLOP/s", "n_gpus": 1, "steps": 10, "warmup": 1, "ms_per_step": round(fwd_ms, 4),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "bf16 (FP32 accumulate)", "data": "synthetic",
+            "config": {"workload": f"one ensemble member 4-512-512-512-{sizes[-1]} GELU; forward over 1024 "
+                                   "samples, training step over 256",
+                       "step": "one forward pass"},
+            "roofline": net_roofline}), flush=True)
+        return
+
     ntr, nval, ncand = args.sur_train, args.sur_val, args.sur_cand
     params = S.draw_params(ntr + nval, seed=3)
     gathers, solve = S.simulate(params, geom=geom)
@@ -263,12 +281,7 @@ def surrogate_bench(args):
                    "validation_rel_l2_p50": round(float(np.median(verr)), 4),
                    "validation_rel_l2_p99": round(float(np.quantile(verr, 0.99)), 4),
                    "validation_tradeoff": tradeoff, "train_s": round(train_s, 1)},
-        "roofline": {"bound": "mfma", "achieved": round(fwd_tflops, 1), "peak": MFMA_BF16_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(fwd_tflops / MFMA_BF16_PEAK_TFLOPS, 4), "traffic": None,
-                     "per_launch": "one member's forward pass over 1024 samples: 2 x 78.09 M MACs each",
-                     "train_step_tflops": round(train_tflops, 1),
-                     "train_step_frac": round(train_tflops / MFMA_BF16_PEAK_TFLOPS, 4),
-                     "forward_ms": round(fwd_ms, 4), "train_step_ms": round(train_ms, 4)},
+        "roofline": net_roofline,
     }), flush=True)
 
 
@@ -282,7 +295,8 @@ def main():
     ap.add_argument("--workload", default="config1", choices=("config1", "batchA", "batchB", "surrogate"),
                     help="config1: the 4001^2 single shot (the headline line); batchA / batchB: "
                          "configuration 2's batches; surrogate: configurations 3 and 4")
-    ap.add_argument("--sur-train", type=int, default=1024, help="surrogate: training models")
+    ap.add_argument("--sur-train", type=int, default=1024,
+                    help="surrogate: training models (0: time the network alone)")
     ap.add_argument("--sur-val", type=int, default=256, help="surrogate: held-out models for the threshold")
     ap.add_argument("--sur-cand", type=int, default=2048, help="surrogate: candidates screened")
     ap.add_argument("--sur-steps", type=int, default=4000, help="surrogate: training steps per member")

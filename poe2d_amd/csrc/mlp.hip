@@ -259,7 +259,7 @@ poe2d_status backward(poe2d_mlp *m, Member &e, int n, float *loss)
         o.ldf = l.inp;
         HIP_TRY(gemm_bf16(m->g16t[li], m->max_batch, m->h16t[li], m->max_batch, l.outp, l.inp, n, EPI_LINEAR, o,
                           m->work, m->work_floats, 1, m->stream));
-        if (li < L - 1) HIP_TRY(bias_gradient(m->g16[li], l.outp, n, l.outp, l.gb, m->stream));
+        if (li < L - 1) HIP_TRY(bias_gradient(m->g16t[li], m->max_batch, l.outp, n, l.gb, m->stream));
         if (li == 0) break;
         // G_{l-1} = (G_l W_l) * gelu'(Z_{l-1}): A = G_l [n][outp], B = W_l^T [inp][outp]
         const int K = l.outp;

@@ -62,8 +62,8 @@ int mse_blocks(int N);
 hipError_t mse(const float *y, int ldy, const float *t, int ldt, int M, int N, uint16_t *g, int ldg, uint16_t *gt,
                int ldgt, float *db, float *partial, hipStream_t stream);
 
-// db[n] = sum_m g[m][n].
-hipError_t bias_gradient(const uint16_t *g, int ldg, int M, int N, float *db, hipStream_t stream);
+// db[r] = sum_k gt[r][k] over the K = batch entries of the transposed gradient gt [R][ldgt].
+hipError_t bias_gradient(const uint16_t *gt, int ldgt, int R, int K, float *db, hipStream_t stream);
 
 // Adam step t (1-based) on n FP32 master weights; their bf16 copy w16 (may be null) follows.
 hipError_t adam_step(float *w, float *m, float *v, const float *g, size_t n, float lr, float b1, float b2, float eps,

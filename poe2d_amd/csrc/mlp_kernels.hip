@@ -5,7 +5,7 @@
 // GEMM layout ("NT"): C[m][n] = sum_k A[m][k] B[n][k], both operands K-contiguous bf16 rows.  A
 // block of 256 threads (4 waves, 2 x 2) owns a 128 x 128 tile of C; each wave computes 64 x 64
 // with 2 x 2 v_mfma_f32_32x32x16_bf16.  Operand tiles of 128 rows x 32 k go through LDS, double
-// buffered, one barrier per k tile.  MFMA lane layout (gfx950, 32x32x16 bf16): lane l supplies
+// buffered, one barrier per k tile; loads run two k tiles ahead of the MFMAs.  MFMA lane layout (gfx950, 32x32x16 bf16): lane l supplies
 // row l % 32 of A and column l % 32 of B with 8 k values (the same 8 for both; here k = (l / 32) * 8
 // + j of the 16-wide k step), and element i of its 16 results is C[(i / 4) * 8 + (l / 32) * 4 + i % 4]
 // [l % 32].
@@ -43,6 +43,7 @@ __global__ void __launch_bounds__(256) gemm_nt(const uint16_t *__restrict__ A, i
 {
     __shared__ uint16_t la[2][BM * LDS_ROW];
     __shared__ uint16_t lb[2][BN * LDS_ROW];
+    __shared__ float lbias[BN];
     const int tiles_n = (N + BN - 1) / BN;
     const int tile = blockIdx.x;
     const int tm = tile / tiles_n, tn = tile % tiles_n;
@@ -51,17 +52,27 @@ __global__ void __launch_bounds__(256) gemm_nt(const uint16_t *__restrict__ A, i
     const int k0 = split * kper, k1 = min(K, k0 + kper);
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wm = wave >> 1, wn = wave & 1;
+    if (t < BN) lbias[t] = (EPI != EPI_PARTIAL && out.bias && tn * BN + t < N) ? out.bias[tn * BN + t] : 0.f;
 
     // each thread loads 2 x 16 bytes of each operand tile: rows r0 and r0 + 64, k chunk kc * 8
     const int r0 = t >> 2, kc = t & 3;
+    // Loads are unconditional (rows past the edge read the last row, and are zeroed when stored to
+    // LDS), so every path keeps the same number of loads in flight and the waits before the LDS
+    // stores can leave the younger set's loads outstanding.
+    bool a_ok[2], b_ok[2];
+    for (int h = 0; h < 2; ++h) {
+        a_ok[h] = tm * BM + r0 + h * 64 < M;
+        b_ok[h] = tn * BN + r0 + h * 64 < N;
+    }
     auto load_tile = [&](const uint16_t *P, int ld, int rows, int row0, int k, u16x8 *v) {
         for (int h = 0; h < 2; ++h) {
-            const int r = row0 + r0 + h * 64;
-            v[h] = r < rows ? *(const u16x8 *)(P + (size_t)r * ld + k + kc * 8) : u16x8{0, 0, 0, 0, 0, 0, 0, 0};
+            const int r = min(row0 + r0 + h * 64, rows - 1);
+            v[h] = *(const u16x8 *)(P + (size_t)r * ld + k + kc * 8);
         }
     };
-    auto store_tile = [&](uint16_t *L, const u16x8 *v) {
-        for (int h = 0; h < 2; ++h) *(u16x8 *)(L + (r0 + h * 64) * LDS_ROW + kc * 8) = v[h];
+    auto store_tile = [&](uint16_t *L, const u16x8 *v, const bool *ok) {
+        for (int h = 0; h < 2; ++h)
+            *(u16x8 *)(L + (r0 + h * 64) * LDS_ROW + kc * 8) = ok[h] ? v[h] : u16x8{0, 0, 0, 0, 0, 0, 0, 0};
     };
 
     f32x16 acc[2][2];
@@ -69,22 +80,11 @@ __global__ void __launch_bounds__(256) gemm_nt(const uint16_t *__restrict__ A, i
         for (int j = 0; j < 2; ++j)
             for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.f;
 
-    u16x8 va[2], vb[2];
-    int buf = 0;
-    if (k0 < k1) {
-        load_tile(A, lda, M, tm * BM, k0, va);
-        load_tile(B, ldb, N, tn * BN, k0, vb);
-        store_tile(la[0], va);
-        store_tile(lb[0], vb);
-    }
-    __syncthreads();
-    for (int k = k0; k < k1; k += BK) {
-        const bool more = k + BK < k1;
-        if (more) {
-            load_tile(A, lda, M, tm * BM, k + BK, va);
-            load_tile(B, ldb, N, tn * BN, k + BK, vb);
-        }
-        const uint16_t *LA = la[buf], *LB = lb[buf];
+    // Two register sets carry operand tiles from memory: the tile loaded during k step i is stored
+    // to LDS at the end of step i + 1, so each load has two steps of MFMA work to land.
+    u16x8 va0[2], vb0[2], va1[2], vb1[2];
+    const int nk = k1 > k0 ? (k1 - k0) / BK : 0;
+    auto compute = [&](const uint16_t *LA, const uint16_t *LB) {
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 16) {
             bf16x8 fa[2], fb[2];
@@ -101,37 +101,75 @@ __global__ void __launch_bounds__(256) gemm_nt(const uint16_t *__restrict__ A, i
                 for (int j = 0; j < 2; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
         }
-        if (more) {
-            store_tile(la[buf ^ 1], va);
-            store_tile(lb[buf ^ 1], vb);
+    };
+    const int last = k0 + (nk - 1) * BK;  // tiles past the end re-read the last one (never used)
+    if (nk > 0) {
+        load_tile(A, lda, M, tm * BM, k0, va0);
+        load_tile(B, ldb, N, tn * BN, k0, vb0);
+        load_tile(A, lda, M, tm * BM, min(k0 + BK, last), va1);
+        load_tile(B, ldb, N, tn * BN, min(k0 + BK, last), vb1);
+        store_tile(la[0], va0, a_ok);
+        store_tile(lb[0], vb0, b_ok);
+    }
+    __syncthreads();
+    int it = 0;
+    for (; it + 2 <= nk; it += 2) {
+        // LDS buffer 0 holds tile it; set 1 carries tile it + 1
+        load_tile(A, lda, M, tm * BM, min(k0 + (it + 2) * BK, last), va0);
+        load_tile(B, ldb, N, tn * BN, min(k0 + (it + 2) * BK, last), vb0);
+        compute(la[0], lb[0]);
+        store_tile(la[1], va1, a_ok);
+        store_tile(lb[1], vb1, b_ok);
+        __syncthreads();
+        // LDS buffer 1 holds tile it + 1; set 0 carries tile it + 2
+        load_tile(A, lda, M, tm * BM, min(k0 + (it + 3) * BK, last), va1);
+        load_tile(B, ldb, N, tn * BN, min(k0 + (it + 3) * BK, last), vb1);
+        compute(la[1], lb[1]);
+        if (it + 2 < nk) {
+            store_tile(la[0], va0, a_ok);
+            store_tile(lb[0], vb0, b_ok);
         }
         __syncthreads();
-        buf ^= 1;
     }
+    if (it < nk) compute(la[0], lb[0]);  // odd count: the last tile is in buffer 0
 
-    // epilogue
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const int n = tn * BN + wn * 64 + j * 32 + (lane & 31);
-            if (n >= N) continue;
-#pragma unroll
-            for (int e = 0; e < 16; ++e) {
-                const int m = tm * BM + wm * 64 + i * 32 + (e >> 2) * 8 + (lane >> 5) * 4 + (e & 3);
-                if (m >= M) continue;
-                float v = acc[i][j][e];
-                if (EPI == EPI_PARTIAL) {
-                    out.f32[(size_t)split * M * out.ldf + (size_t)m * out.ldf + n] = v;
-                    continue;
-                }
-                if (out.bias) v += out.bias[n];
-                if (out.pre) out.pre[(size_t)m * out.ldp + n] = v;
-                if (EPI == EPI_GELU) v = gelu(v);
-                if (out.f32) out.f32[(size_t)m * out.ldf + n] = v;
-                if (out.b16) out.b16[(size_t)m * out.ldb + n] = bf16_rne(v);
-            }
+    // epilogue: full tiles store without per-element bounds checks; the bias comes from LDS (staged
+    // before the k loop), so no global load is pending here and the stores need no waits
+    const bool full = (tm + 1) * BM <= M && (tn + 1) * BN <= N;
+    auto emit = [&](int m, int n, float v) {
+        if (EPI == EPI_PARTIAL) {
+            out.f32[(size_t)split * M * out.ldf + (size_t)m * out.ldf + n] = v;
+            return;
         }
+        v += lbias[n - tn * BN];
+        if (out.pre) out.pre[(size_t)m * out.ldp + n] = v;
+        if (EPI == EPI_GELU) v = gelu(v);
+        if (out.f32) out.f32[(size_t)m * out.ldf + n] = v;
+        if (out.b16) out.b16[(size_t)m * out.ldb + n] = bf16_rne(v);
+    };
+    if (full) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int n = tn * BN + wn * 64 + j * 32 + (lane & 31);
+#pragma unroll
+                for (int e = 0; e < 16; ++e)
+                    emit(tm * BM + wm * 64 + i * 32 + (e >> 2) * 8 + (lane >> 5) * 4 + (e & 3), n, acc[i][j][e]);
+            }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int n = tn * BN + wn * 64 + j * 32 + (lane & 31);
+#pragma unroll
+                for (int e = 0; e < 16; ++e) {
+                    const int m = tm * BM + wm * 64 + i * 32 + (e >> 2) * 8 + (lane >> 5) * 4 + (e & 3);
+                    if (m < M && n < N) emit(m, n, acc[i][j][e]);
+                }
+            }
+    }
 }
 
 // out[m][n] = sum over splits of part[s][m][n] (+ bias), then as the GEMM epilogue.
@@ -227,19 +265,16 @@ __global__ void __launch_bounds__(256) mse_grad(const float *__restrict__ y, int
     if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
-// db[c] = sum over the M rows of a bf16 gradient g.
-__global__ void __launch_bounds__(256) bias_grad(const uint16_t *__restrict__ g, int ldg, int M, int N,
+// db[r] = sum of row r (K entries) of the transposed bf16 gradient gt [R][ldgt]: one wave per row.
+__global__ void __launch_bounds__(256) bias_grad(const uint16_t *__restrict__ gt, int ldgt, int R, int K,
                                                  float *__restrict__ db)
 {
-    __shared__ float red[256];
-    const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
-    const int c = blockIdx.x * 64 + lane;
+    const int r = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (r >= R) return;
     float s = 0.f;
-    if (c < N)
-        for (int r = q; r < M; r += 4) s += bf16_to_f32(g[(size_t)r * ldg + c]);
-    red[threadIdx.x] = s;
-    __syncthreads();
-    if (q == 0 && c < N) db[c] = (red[lane] + red[lane + 64]) + (red[lane + 128] + red[lane + 192]);
+    for (int k = lane; k < K; k += 64) s += bf16_to_f32(gt[(size_t)r * ldgt + k]);
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) db[r] = s;
 }
 
 __device__ __forceinline__ float adam_update(float *w, float *mom, float *vel, float g, size_t i, float lr, float b1,
@@ -344,9 +379,9 @@ hipError_t mse(const float *y, int ldy, const float *t, int ldt, int M, int N, u
     return hipGetLastError();
 }
 
-hipError_t bias_gradient(const uint16_t *g, int ldg, int M, int N, float *db, hipStream_t stream)
+hipError_t bias_gradient(const uint16_t *gt, int ldgt, int R, int K, float *db, hipStream_t stream)
 {
-    bias_grad<<<(N + 63) / 64, 256, 0, stream>>>(g, ldg, M, N, db);
+    bias_grad<<<(R + 3) / 4, 256, 0, stream>>>(gt, ldgt, R, K, db);
     return hipGetLastError();
 }
 
