@@ -59,7 +59,26 @@ typedef struct {
                           coefficients as max sqrt((lambdac + 2 mu + alpha^2 M) / rho)             */
     int precision;     /* POE2D_F32 or POE2D_F64                                                  */
     int check_cfl;     /* nonzero: refuse Courant > 6/7/sqrt(2) like the reference (:1211)         */
+    int stress_law;    /* POE2D_STRESS_AS_WRITTEN (0) or POE2D_STRESS_DRAINED                      */
+    int staggered_material; /* POE2D_MATERIAL_AS_WRITTEN (0) or POE2D_MATERIAL_AVERAGED            */
 } poe2d_grid;
+
+/* Stress law.  AS_WRITTEN is the reference's: sigma_xx = (lambda_c + 2 mu) e_xx + lambda_c e_yy
+ * - alpha p with the undrained lambda_c = lambda_o + alpha^2 M and p = -alpha M (e_xx + e_yy) + M
+ * zeta (seismic_CPML_2D_poroelastic_withCUDA.cu:241-242), which counts alpha^2 M twice in the
+ * P-wave modulus (lambda_c + 2 mu + alpha^2 M).  DRAINED uses lambda_o = lambda_c - alpha^2 M in
+ * place of lambda_c (the coefficient array still holds lambda_c), which is Biot's law: the
+ * low-frequency P-wave modulus becomes Gassmann's lambda_c + 2 mu.  The P velocity that sets d0
+ * and the Courant check follows the law. */
+enum { POE2D_STRESS_AS_WRITTEN = 0, POE2D_STRESS_DRAINED = 1 };
+
+/* Material at the staggered points.  AS_WRITTEN is the reference's: sigma_xy at (x, z+1/2) uses
+ * mu of point (x, z+1) (the harmonic mean computed at seismic_CPML_2D_poroelastic_withCUDA.cu:217
+ * is overwritten at :218) and the v_y / w_y update at (x, z+1/2) uses rho, rho_f, m, b of point
+ * (x, z+1) (:330-333).  AVERAGED uses the harmonic mean 2 / (1/mu(x,z) + 1/mu(x,z+1)) for sigma_xy
+ * and the arithmetic means of rho, rho_f, m and b over points (x, z) and (x, z+1) for v_y / w_y.
+ * Handles advanced together in a batch must share this setting. */
+enum { POE2D_MATERIAL_AS_WRITTEN = 0, POE2D_MATERIAL_AVERAGED = 1 };
 
 /* The eight per-point coefficient arrays the time loop reads (the reference's rho, rhof, rsm,
  * rmu, rlambdac, rbM, alpha, etaokappa uploaded at seismic_CPML_2D_poroelastic_withCUDA.cu:1433-1461).

@@ -146,6 +146,7 @@ struct Solver : Base {
     size_t P = 0;
     T DX = 0, DZ = 0, DT = 0;
     int nth = 1;
+    bool avg = false; /* POE2D_MATERIAL_AVERAGED */
     std::vector<T> rho, rhof, rsm, rmu, lc, rbM, alph, eok;
     std::vector<T> vx, vz, wx, wz, g11, g22, xi1, xi2, sig2, sxx, szz, sxz;
     std::vector<T> psi[12];
@@ -284,7 +285,9 @@ struct Solver : Base {
         for (int z = 1; z < nz - 1; z++)
             for (int x = 2; x < nx; x++) {
                 const size_t o = x + z * Pl;
-                const T c33 = RMU[o + Pl];
+                /* averaged: the harmonic mean the reference computes at CU:217 (in its double
+                 * arithmetic) before overwriting it with mu of (x, z+1) at CU:218 */
+                const T c33 = avg ? (T)(2.0 / (1.0 / (double)RMU[o] + 1.0 / (double)RMU[o + Pl])) : RMU[o + Pl];
                 T ddx = d4(VZ[o], VZ[o - 1], VZ[o + 1], VZ[o - 2], DXl);
                 T ddz = d4(VX[o + Pl], VX[o], VX[o + 2 * Pl], VX[o - Pl], DZl);
                 p5[o] = bx[x] * p5[o] + ax[x] * ddx;
@@ -331,7 +334,11 @@ struct Solver : Base {
             for (int x = 1; x < nx - 1; x++) {
                 const size_t o = x + z * Pl;
                 const size_t t = o + Pl;
-                const T r = RHO[t], sm = RSM[t], rf = RHOF[t], ek = EOK[t];
+                /* material of (x, z+1) as written (CU:330-333); averaged: means of (x, z) and (x, z+1) */
+                const T r = avg ? (T)(0.50 * (RHO[o] + RHO[t])) : RHO[t];
+                const T sm = avg ? (T)(0.50 * (RSM[o] + RSM[t])) : RSM[t];
+                const T rf = avg ? (T)(0.50 * (RHOF[o] + RHOF[t])) : RHOF[t];
+                const T ek = avg ? (T)(0.50 * (EOK[o] + EOK[t])) : EOK[t];
                 const T co = (r * sm - rf * rf) / DTl;
                 const T c1 = co + r * ek * 0.50;
                 const T c2 = co - r * ek * 0.50;
@@ -385,6 +392,14 @@ int build(const poe2d_grid *g, const poe2d_coeffs *c, Solver<T> *s)
             for (int x = 0; x <= g->nx; ++x)
                 (*mats[k])[x + z * s->P] = (T)coeff_at(c, srcs[k], x + z * pitch);
     }
+    /* stress law: drained replaces lambda_c by lambda_c - alpha^2 M (computed in double) */
+    auto lambda_at = [&](size_t i) {
+        const double lc = coeff_at(c, c->lambdac, i), al = coeff_at(c, c->alpha, i), M = coeff_at(c, c->M, i);
+        return g->stress_law == POE2D_STRESS_DRAINED ? lc - al * al * M : lc;
+    };
+    for (int z = 0; z <= g->nz; ++z)
+        for (int x = 0; x <= g->nx; ++x) s->lc[x + z * s->P] = (T)lambda_at(x + z * pitch);
+    s->avg = g->staggered_material == POE2D_MATERIAL_AVERAGED;
     double cpmax = g->cp_max;
     if (cpmax <= 0) {
         cpmax = 0;
@@ -392,7 +407,7 @@ int build(const poe2d_grid *g, const poe2d_coeffs *c, Solver<T> *s)
             for (int x = 0; x <= g->nx; ++x) {
                 size_t i = x + z * pitch;
                 double al = coeff_at(c, c->alpha, i), M = coeff_at(c, c->M, i);
-                double H = coeff_at(c, c->lambdac, i) + 2 * coeff_at(c, c->mu, i) + al * al * M;
+                double H = lambda_at(i) + 2 * coeff_at(c, c->mu, i) + al * al * M;
                 double v = std::sqrt(H / coeff_at(c, c->rho, i));
                 if (v > cpmax) cpmax = v;
             }
@@ -432,6 +447,10 @@ int check_grid(const poe2d_grid *g)
     if (g->npower < 1) return fail(POE2D_E_INVALID, "oracle: npower must be >= 1");
     if (g->precision != POE2D_F32 && g->precision != POE2D_F64)
         return fail(POE2D_E_INVALID, "oracle: precision must be 32 or 64");
+    if (g->stress_law != POE2D_STRESS_AS_WRITTEN && g->stress_law != POE2D_STRESS_DRAINED)
+        return fail(POE2D_E_INVALID, "oracle: unknown stress law");
+    if (g->staggered_material != POE2D_MATERIAL_AS_WRITTEN && g->staggered_material != POE2D_MATERIAL_AVERAGED)
+        return fail(POE2D_E_INVALID, "oracle: unknown staggered material setting");
     return 0;
 }
 

@@ -22,6 +22,11 @@ STATUS_NAMES = {
 F32 = 32
 F64 = 64
 
+STRESS_AS_WRITTEN = 0
+STRESS_DRAINED = 1
+MATERIAL_AS_WRITTEN = 0
+MATERIAL_AVERAGED = 1
+
 SRC_PRESSURE_DGAUSS = 0
 SRC_FORCE_Y_RICKER = 1
 
@@ -51,6 +56,7 @@ class Grid(ctypes.Structure):
         ("npower", ctypes.c_double), ("rcoef", ctypes.c_double), ("kmax", ctypes.c_double),
         ("alpha_max", ctypes.c_double), ("f0", ctypes.c_double), ("cp_max", ctypes.c_double),
         ("precision", ctypes.c_int), ("check_cfl", ctypes.c_int),
+        ("stress_law", ctypes.c_int), ("staggered_material", ctypes.c_int),
     ]
 
 

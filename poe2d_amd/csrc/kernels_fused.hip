@@ -39,7 +39,7 @@
 
 namespace poe2d {
 
-template <class T, int BW, bool PAL>
+template <class T, int BW, bool PAL, bool AVG>
 __global__ void __launch_bounds__(BW)
     fused_step(const StepArgs<T> *__restrict__ models, const WorkItem *__restrict__ work, int nwork, int per_xcd,
                const long long *__restrict__ base, int k)
@@ -270,7 +270,9 @@ __global__ void __launch_bounds__(BW)
                 ddz = cpml(ddz, pz(A.in, r) + PZ_DVX, pz(A.out, r) + PZ_DVX, store_s,
                            tab[TB_AZH * A.tn + s], tab[TB_BZH * A.tn + s], tab[TB_RKZ * A.tn + s]);
             }
-            sxy_n = so.w + ms_nxt.w * (ddx + ddz) * dt;  // mu of point (x, z+1)
+            // mu of point (x, z+1), or the harmonic mean of points (x, z) and (x, z+1)
+            const T mu_xy = AVG ? harmonic_mu(ms_cur.w, ms_nxt.w) : ms_nxt.w;
+            sxy_n = so.w + mu_xy * (ddx + ddz) * dt;
         }
         if (store_s) *(GN *)(Sn + off(s)) = nvec4<T>{exx_n, eyy_n, zeta_n, sxy_n};
         {
@@ -329,7 +331,9 @@ __global__ void __launch_bounds__(BW)
                 }
                 T div = e1 + e2;
                 if (src_f && j == A.src_z) div += A.src_val;
-                biot_velocity(v.y, v.w, div, e3, mv_j1.x, mv_j1.y, mv_j1.z, mv_j1.w, A.inv_dt);
+                // material of point (x, z+1), or the mean of points (x, z) and (x, z+1)
+                const Vec4<T> mvy = AVG ? mean_rows(mv_j, mv_j1) : mv_j1;
+                biot_velocity(v.y, v.w, div, e3, mvy.x, mvy.y, mvy.z, mvy.w, A.inv_dt);
             }
             *(GN *)(Vn + off(j)) = nvec4<T>{v.x, v.y, v.z, v.w};
         }
@@ -349,15 +353,15 @@ __global__ void __launch_bounds__(BW)
     }
 }
 
-template <class T, bool PAL>
+template <class T, bool PAL, bool AVG>
 hipError_t launch_fused_bw(const StepArgs<T> *models, const WorkItem *work, int nwork, int per_xcd, int bw,
                            const long long *base, int k, hipStream_t stream)
 {
     const dim3 grid(8 * per_xcd);
     switch (bw) {
-    case 128: fused_step<T, 128, PAL><<<grid, 128, 0, stream>>>(models, work, nwork, per_xcd, base, k); break;
-    case 256: fused_step<T, 256, PAL><<<grid, 256, 0, stream>>>(models, work, nwork, per_xcd, base, k); break;
-    case 512: fused_step<T, 512, PAL><<<grid, 512, 0, stream>>>(models, work, nwork, per_xcd, base, k); break;
+    case 128: fused_step<T, 128, PAL, AVG><<<grid, 128, 0, stream>>>(models, work, nwork, per_xcd, base, k); break;
+    case 256: fused_step<T, 256, PAL, AVG><<<grid, 256, 0, stream>>>(models, work, nwork, per_xcd, base, k); break;
+    case 512: fused_step<T, 512, PAL, AVG><<<grid, 512, 0, stream>>>(models, work, nwork, per_xcd, base, k); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -365,16 +369,19 @@ hipError_t launch_fused_bw(const StepArgs<T> *models, const WorkItem *work, int 
 
 template <class T>
 hipError_t launch_fused(const StepArgs<T> *models, const WorkItem *work, int nwork, int per_xcd, int bw,
-                        bool palette, const long long *base, int k, hipStream_t stream)
+                        bool palette, bool avg, const long long *base, int k, hipStream_t stream)
 {
     if (nwork < 1 || per_xcd < 1 || 8LL * per_xcd < nwork) return hipErrorInvalidValue;
-    return palette ? launch_fused_bw<T, true>(models, work, nwork, per_xcd, bw, base, k, stream)
-                   : launch_fused_bw<T, false>(models, work, nwork, per_xcd, bw, base, k, stream);
+    if (avg)
+        return palette ? launch_fused_bw<T, true, true>(models, work, nwork, per_xcd, bw, base, k, stream)
+                       : launch_fused_bw<T, false, true>(models, work, nwork, per_xcd, bw, base, k, stream);
+    return palette ? launch_fused_bw<T, true, false>(models, work, nwork, per_xcd, bw, base, k, stream)
+                   : launch_fused_bw<T, false, false>(models, work, nwork, per_xcd, bw, base, k, stream);
 }
 
-template hipError_t launch_fused<float>(const StepArgs<float> *, const WorkItem *, int, int, int, bool,
+template hipError_t launch_fused<float>(const StepArgs<float> *, const WorkItem *, int, int, int, bool, bool,
                                         const long long *, int, hipStream_t);
-template hipError_t launch_fused<double>(const StepArgs<double> *, const WorkItem *, int, int, int, bool,
+template hipError_t launch_fused<double>(const StepArgs<double> *, const WorkItem *, int, int, int, bool, bool,
                                          const long long *, int, hipStream_t);
 
 namespace {

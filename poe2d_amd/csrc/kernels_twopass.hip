@@ -12,7 +12,7 @@
 
 namespace poe2d {
 
-template <class T>
+template <class T, bool AVG>
 __global__ void __launch_bounds__(256) stress_pass(StepArgs<T> A)
 {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -73,12 +73,13 @@ __global__ void __launch_bounds__(256) stress_pass(StepArgs<T> A)
             const long long k = (long long)sz * P + x;
             ddz = cpml(ddz, psi_z(A, b, k) + PZ_DVX, psi_z(A, b, k) + PZ_DVX, true, tab[TB_AZH * tn + z], tab[TB_BZH * tn + z], tab[TB_RKZ * tn + z]);
         }
-        st.w = st.w + MS[o + P].w * (ddx + ddz) * A.dt;
+        const T mu_xy = AVG ? harmonic_mu(MS[o].w, MS[o + P].w) : MS[o + P].w;
+        st.w = st.w + mu_xy * (ddx + ddz) * A.dt;
     }
     S[o] = st;
 }
 
-template <class T>
+template <class T, bool AVG>
 __global__ void __launch_bounds__(256) velocity_pass(StepArgs<T> A)
 {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -128,24 +129,28 @@ __global__ void __launch_bounds__(256) velocity_pass(StepArgs<T> A)
         }
         T div = e1 + e2;
         if (A.src_kind == POE2D_SRC_FORCE_Y_RICKER && x == A.src_x && z == A.src_z) div += A.src_val;
-        const Vec4<T> m = MV[o + P];  // material of point (x, z+1) (reference :330-333)
+        // material of point (x, z+1) (reference :330-333), or the mean of (x, z) and (x, z+1)
+        const Vec4<T> m = AVG ? mean_rows(MV[o], MV[o + P]) : MV[o + P];
         biot_velocity(v.y, v.w, div, e3, m.x, m.y, m.z, m.w, A.inv_dt);
     }
     V[o] = v;
 }
 
 template <class T>
-void launch_twopass(const StepArgs<T> &A, hipStream_t stream, int phase)
+void launch_twopass(const StepArgs<T> &A, hipStream_t stream, int phase, bool avg)
 {
     dim3 block(64, 4);
     dim3 grid((A.nx + 1 + 63) / 64, (A.nz + 1 + 3) / 4);
-    if (phase == 0)
-        stress_pass<T><<<grid, block, 0, stream>>>(A);
-    else
-        velocity_pass<T><<<grid, block, 0, stream>>>(A);
+    if (phase == 0) {
+        if (avg) stress_pass<T, true><<<grid, block, 0, stream>>>(A);
+        else stress_pass<T, false><<<grid, block, 0, stream>>>(A);
+    } else {
+        if (avg) velocity_pass<T, true><<<grid, block, 0, stream>>>(A);
+        else velocity_pass<T, false><<<grid, block, 0, stream>>>(A);
+    }
 }
 
-template void launch_twopass<float>(const StepArgs<float> &, hipStream_t, int);
-template void launch_twopass<double>(const StepArgs<double> &, hipStream_t, int);
+template void launch_twopass<float>(const StepArgs<float> &, hipStream_t, int, bool);
+template void launch_twopass<double>(const StepArgs<double> &, hipStream_t, int, bool);
 
 }  // namespace poe2d

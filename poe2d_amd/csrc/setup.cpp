@@ -40,7 +40,7 @@ double cp_max_from_coeffs(const poe2d_grid &g, const poe2d_coeffs &c)
         for (int x = 0; x <= g.nx; ++x) {
             const size_t i = (size_t)z * pitch + x;
             const double al = at(c.alpha, i), M = at(c.M, i);
-            const double modulus = at(c.lambdac, i) + 2.0 * at(c.mu, i) + al * al * M;
+            const double modulus = effective_lambda(g, at(c.lambdac, i), al, M) + 2.0 * at(c.mu, i) + al * al * M;
             const double v = std::sqrt(modulus / at(c.rho, i));
             if (v > vmax) vmax = v;
         }

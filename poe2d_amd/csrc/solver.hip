@@ -134,16 +134,18 @@ struct StepList {
     size_t cap_models = 0, cap_work = 0, cap_ptr = 0, cap_ent = 0;
     int nmodels = 0, nwork = 0, per_xcd = 0, bw = 0;
     bool palette = false;
+    bool avg = false;  // POE2D_MATERIAL_AVERAGED (all members alike)
     int precision = POE2D_F32;
     // graph of kGraphSteps launches plus the base advance, and the launch it was captured for
     hipGraphExec_t exec = nullptr;
     struct Sig {
         void *models, *work, *base;
-        int nwork, per_xcd, bw, palette, precision;
+        int nwork, per_xcd, bw, palette, avg, precision;
         bool operator==(const Sig &o) const
         {
             return models == o.models && work == o.work && base == o.base && nwork == o.nwork &&
-                   per_xcd == o.per_xcd && bw == o.bw && palette == o.palette && precision == o.precision;
+                   per_xcd == o.per_xcd && bw == o.bw && palette == o.palette && avg == o.avg &&
+                   precision == o.precision;
         }
     } sig{};
 
@@ -151,7 +153,7 @@ struct StepList {
     {
         Sig g{};
         g.models = models; g.work = work; g.base = base;
-        g.nwork = nwork; g.per_xcd = per_xcd; g.bw = bw; g.palette = palette; g.precision = precision;
+        g.nwork = nwork; g.per_xcd = per_xcd; g.bw = bw; g.palette = palette; g.avg = avg; g.precision = precision;
         return g;
     }
 
@@ -287,8 +289,12 @@ poe2d_status upload_coeffs(Model *md, const poe2d_coeffs *c)
     const size_t cpitch = c->pitch ? c->pitch : (size_t)g.nx + 1;
     const void *ms_src[4] = {c->alpha, c->M, c->lambdac, c->mu};
     const void *mv_src[4] = {c->rho, c->rhof, c->m, c->b};
-    auto at = [&](const void *a, size_t i) {
+    auto raw = [&](const void *a, size_t i) {
         return c->dtype == POE2D_F64 ? ((const double *)a)[i] : (double)((const float *)a)[i];
+    };
+    // lambda (slot 2 of the MS group) follows the stress law
+    auto at = [&](const void *a, size_t i) {
+        return a == c->lambdac ? effective_lambda(g, raw(c->lambdac, i), raw(c->alpha, i), raw(c->M, i)) : raw(a, i);
     };
     std::vector<Vec4<T>> host(md->group_points(), Vec4<T>{T(0), T(0), T(0), T(0)});
     for (int grp = 0; grp < 2; ++grp) {
@@ -477,6 +483,10 @@ poe2d_status build_list(StepList &L, const std::vector<poe2d_solver *> &members,
                         hipStream_t stream)
 {
     const int nm = (int)members.size();
+    const int material = members[0]->grid.staggered_material;
+    for (const poe2d_solver *m : members)
+        if (m->grid.staggered_material != material)
+            return fail(POE2D_E_INVALID, "run: batch members differ in staggered material");
     std::vector<StepArgs<T>> args(nm);
     std::vector<WorkItem> work;
     std::vector<int> ptr;
@@ -550,6 +560,7 @@ poe2d_status build_list(StepList &L, const std::vector<poe2d_solver *> &members,
     L.per_xcd = (L.nwork + 7) / 8;
     L.bw = bw;
     L.palette = palette;
+    L.avg = material == POE2D_MATERIAL_AVERAGED;
     L.precision = sizeof(T) == 4 ? POE2D_F32 : POE2D_F64;
     return POE2D_OK;
 }
@@ -560,7 +571,7 @@ hipError_t launch_list_chunk(const StepList &L, int count, hipStream_t stream)
 {
     for (int k = 0; k < count; ++k) {
         hipError_t e = launch_fused<T>((const StepArgs<T> *)L.models, L.work, L.nwork, L.per_xcd, L.bw, L.palette,
-                                       L.base, k, stream);
+                                       L.avg, L.base, k, stream);
         if (e != hipSuccess) return e;
     }
     return launch_base_advance(L.base, count, stream);
@@ -628,12 +639,12 @@ poe2d_status run_steps(poe2d_solver *s, int nsteps)
         for (int k = 0; k < nsteps; ++k) {
             const int n = s->step + k;
             A.src_val = (T)s->source_value(n);
-            launch_twopass<T>(A, s->stream, 0);
+            launch_twopass<T>(A, s->stream, 0, s->grid.staggered_material == POE2D_MATERIAL_AVERAGED);
             if (s->nrec > 0 && n % s->dec == 0 && n / s->dec < s->nsamp)
                 // in place: v is still v^n, strains are already those of step n
                 launch_gather<T>(A, s->cur, s->cur, R, (T *)s->traces + (size_t)(n / s->dec) * s->ncomp * s->nrec,
                                  s->stream);
-            launch_twopass<T>(A, s->stream, 1);
+            launch_twopass<T>(A, s->stream, 1, s->grid.staggered_material == POE2D_MATERIAL_AVERAGED);
         }
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(s->ev1, s->stream));
@@ -814,6 +825,10 @@ poe2d_status poe2d_create(const poe2d_grid *g, const poe2d_coeffs *c, poe2d_solv
         return fail(POE2D_E_INVALID, "create: null coefficient array");
     if (c->pitch && c->pitch < (size_t)g->nx + 1) return fail(POE2D_E_INVALID, "create: coefficient pitch < nx+1");
     if (!(g->f0 > 0)) return fail(POE2D_E_INVALID, "create: f0 must be > 0");
+    if (g->stress_law != POE2D_STRESS_AS_WRITTEN && g->stress_law != POE2D_STRESS_DRAINED)
+        return fail(POE2D_E_INVALID, "create: unknown stress law");
+    if (g->staggered_material != POE2D_MATERIAL_AS_WRITTEN && g->staggered_material != POE2D_MATERIAL_AVERAGED)
+        return fail(POE2D_E_INVALID, "create: unknown staggered material setting");
 
     Model *md = new (std::nothrow) Model;
     if (!md) return fail(POE2D_E_NOMEM, "create: out of host memory");

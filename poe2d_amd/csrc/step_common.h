@@ -222,8 +222,24 @@ __device__ __forceinline__ T pore_pressure(T exx, T eyy, T zeta, T al, T M)
 // stress3 x in [1,NX-1), z in [2,NZ) (:237-239); vx x in [2,NX), z in [2,NZ) (:289-291);
 // vz x in [1,NX-1), z in [1,NZ-1) (:326-328).
 
+// Material at the staggered points under POE2D_MATERIAL_AVERAGED: the shear modulus of sigma_xy
+// at (x, z+1/2) as the harmonic mean of the two points, the velocity-row coefficients of v_y at
+// (x, z+1/2) as the arithmetic means.
 template <class T>
-void launch_twopass(const StepArgs<T> &A, hipStream_t stream, int phase);
+__device__ __forceinline__ T harmonic_mu(T a, T b)
+{
+    return T(2) / (T(1) / a + T(1) / b);
+}
+
+template <class T>
+__device__ __forceinline__ Vec4<T> mean_rows(const Vec4<T> &a, const Vec4<T> &b)
+{
+    return Vec4<T>{T(0.5) * (a.x + b.x), T(0.5) * (a.y + b.y), T(0.5) * (a.z + b.z), T(0.5) * (a.w + b.w)};
+}
+
+// phase 0: stresses, 1: velocities; avg: POE2D_MATERIAL_AVERAGED.
+template <class T>
+void launch_twopass(const StepArgs<T> &A, hipStream_t stream, int phase, bool avg);
 
 // One work item of the single-pass step: tile `tile` of model `model`.
 struct WorkItem {
@@ -234,9 +250,10 @@ struct WorkItem {
 // index *base + k, model arguments from models[] (each with its own tiling and receivers; in, out
 // and the source sample are set from the step).  With `palette` every model is read through its
 // one-byte material index (needs npal > 0 for each).  nwork items, placed per_xcd to an XCD.
+// avg: every model uses POE2D_MATERIAL_AVERAGED (else every model uses AS_WRITTEN).
 template <class T>
 hipError_t launch_fused(const StepArgs<T> *models, const WorkItem *work, int nwork, int per_xcd, int bw,
-                        bool palette, const long long *base, int k, hipStream_t stream);
+                        bool palette, bool avg, const long long *base, int k, hipStream_t stream);
 
 // *base += delta (one thread).
 hipError_t launch_base_advance(long long *base, int delta, hipStream_t stream);

@@ -63,6 +63,8 @@ class Problem:
     f0: float = 80.0
     cp_max: float = 0.0          # <= 0: from the coefficients
     check_cfl: int = 1
+    stress_law: int = _abi.STRESS_AS_WRITTEN
+    staggered_material: int = _abi.MATERIAL_AS_WRITTEN
     wavelet: Optional[np.ndarray] = None
     meta: dict = field(default_factory=dict)
 
@@ -73,7 +75,7 @@ class Problem:
     def grid_struct(self, precision):
         return _abi.Grid(self.nx, self.nz, self.dx, self.dz, self.dt, self.npml, self.npower,
                          self.rcoef, self.kmax, self.alpha_max, self.f0, self.cp_max,
-                         precision, self.check_cfl)
+                         precision, self.check_cfl, self.stress_law, self.staggered_material)
 
 
 class CoeffBuffers:

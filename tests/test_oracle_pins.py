@@ -177,8 +177,8 @@ DT = 5e-5
 INNER = (slice(30, 71), slice(30, 71))
 
 
-def _one_step(init, coeffs=None):
-    prob = small_problem(NXS, NXS, DX, DT, nt=1, coeffs=coeffs, npml=10)
+def _one_step(init, coeffs=None, **opts):
+    prob = small_problem(NXS, NXS, DX, DT, nt=1, coeffs=coeffs, npml=10, **opts)
     s = OracleSolver(prob, precision=F64, threads=1)
     for name, arr in init.items():
         s.set_field(name, arr)
@@ -271,6 +271,62 @@ def test_vy_uses_material_of_the_point_below():
     assert abs(vy[48, 40] - _cn_solve(cb["rho"], cb["rhof"], cb["m"], cb["b"], T2, 0.0)[0]) < 1e-9 * abs(vy[48, 40])
 
 
+def test_drained_law_one_step():
+    """Under the drained law the normal stress uses lambda_o = lambda_c - alpha^2 M, so a cubic
+    e_xx drives v_x with the Biot/Gassmann modulus lambda_c + 2 mu (not lambda_c + 2 mu + alpha^2
+    M as written), while the pressure gradient (-alpha M e_xx) is the same in both laws."""
+    x, _ = _grids()
+    c = biot_constants(BRINE)
+    aM = c["alpha"] * c["M"]
+    for law, H in ((_abi.STRESS_DRAINED, c["lam_c"] + 2 * c["mu"]),
+                   (_abi.STRESS_AS_WRITTEN, c["lam_c"] + 2 * c["mu"] + c["alpha"] ** 2 * c["M"])):
+        s = _one_step({"exx": cubic(x + DX / 2)}, stress_law=law)
+        v = s.snapshot("vx")[INNER]
+        T = H * dcubic(x)[INNER]
+        P = -aM * dcubic(x)[INNER]
+        ev = np.array([_cn_solve(c["rho"], c["rhof"], c["m"], c["b"], t, p)[0] for t, p in zip(T.ravel(), P.ravel())])
+        np.testing.assert_allclose(v.ravel(), ev, rtol=1e-8, atol=1e-12 * np.abs(ev).max())
+
+
+def test_averaged_material_one_step():
+    """Averaged staggered material, in a medium layered at row 50 (rows >= 50: gas with a softer
+    frame, mu 4 GPa against 8 GPa): the row-49 v_y update uses the means of rho, rho_f, m, b of
+    rows 49 and 50, and the row-49 shear stress uses the harmonic mean of their mu; rows away
+    from the interface are unchanged."""
+    mask = np.zeros((NXS + 1, NXS + 1), dtype=bool)
+    mask[50:, :] = True
+    soft = dict(GAS, mu_fr=4.0e9)
+    coeffs = oracle_coeffs(NXS, NXS, mask, patch=soft)
+    x, z = _grids()
+    avg = _abi.MATERIAL_AVERAGED
+    cb, cg = biot_constants(BRINE), biot_constants(soft)
+    mean = {k: 0.5 * (cb[k] + cg[k]) for k in ("rho", "rhof", "m", "b")}
+    s = _one_step({"sxy": cubic(x)}, coeffs, staggered_material=avg)
+    vy = s.snapshot("vy")
+    T = dcubic(x[49, 40] + DX / 2)
+    want = _cn_solve(mean["rho"], mean["rhof"], mean["m"], mean["b"], T, 0.0)[0]
+    assert abs(vy[49, 40] - want) < 1e-9 * abs(want)
+    T2 = dcubic(x[48, 40] + DX / 2)
+    assert abs(vy[48, 40] - _cn_solve(cb["rho"], cb["rhof"], cb["m"], cb["b"], T2, 0.0)[0]) < 1e-9 * abs(vy[48, 40])
+    # shear stress at (x, z + 1/2) from v_x = cubic(z)
+    s = _one_step({"vx": cubic(z)}, coeffs, staggered_material=avg)
+    sxy = s.snapshot("sxy")
+    h = 2.0 / (1.0 / cb["mu"] + 1.0 / cg["mu"])
+    for row, mu in ((49, h), (48, cb["mu"]), (51, cg["mu"])):
+        want = DT * mu * dcubic(z[row, 40] + DX / 2)
+        assert abs(sxy[row, 40] - want) < 1e-9 * abs(want), (row, sxy[row, 40], want)
+    # as written: mu of the point below
+    s = _one_step({"vx": cubic(z)}, coeffs)
+    want = DT * cg["mu"] * dcubic(z[49, 40] + DX / 2)
+    assert abs(s.snapshot("sxy")[49, 40] - want) < 1e-9 * abs(want)
+
+
+def test_options_rejected_when_unknown():
+    for opts in (dict(stress_law=2), dict(staggered_material=-1)):
+        with pytest.raises(_abi.Poe2dError):
+            OracleSolver(small_problem(40, 40, DX, DT, nt=1, npml=5, **opts), precision=F64)
+
+
 def test_sources_first_step():
     """Pressure source: p at the source after step 0 equals the wavelet sample; vertical force:
     v_y there equals Biot's momentum solve with T = force."""
@@ -347,12 +403,12 @@ def test_mirror_symmetry_vertical_force():
         np.testing.assert_allclose(f[:, right], sign * f[:, left], rtol=0, atol=1e-10 * scale)
 
 
-def _plane_wave_run(params, inits, nt=400, nx=640, nz=480, x0=150, dt=7.5e-5):
+def _plane_wave_run(params, inits, nt=400, nx=640, nz=480, x0=150, dt=7.5e-5, **opts):
     coeffs = pyoracle.material_biot(**biot_fields(nx, nz, None, params))
     row = nz // 2
     rec = ReceiverSpec(ix=np.arange(nx + 1, dtype=np.int32), iz=np.full(nx + 1, row, np.int32),
                        comp_mask=_abi.VX | _abi.VY | _abi.P, nsamp=nt)
-    prob = small_problem(nx, nz, DX, dt, nt=nt, coeffs=coeffs, rec=rec, npml=5)
+    prob = small_problem(nx, nz, DX, dt, nt=nt, coeffs=coeffs, rec=rec, npml=5, **opts)
     s = OracleSolver(prob, precision=F64)
     x = np.arange(nx + 1) * DX
     w = 4.0
@@ -386,6 +442,24 @@ def test_plane_wave_speeds_low_frequency():
     assert abs(got_p / vp - 1) < 2e-3, (got_p, vp)
     assert abs(got_s / vs - 1) < 2e-3, (got_s, vs)
     assert abs(got_p / vp_hf - 1) > 5e-3 and abs(got_s / vs_hf - 1) > 1e-2
+
+
+@pytest.mark.slow
+def test_plane_wave_speed_drained_law_is_gassmann():
+    """Drained law, same brine rock in the fluid-locked regime: P travels at Gassmann's
+    sqrt((lambda_c + 2 mu) / rho) = sqrt((K_fr + 4/3 mu_fr + alpha^2 M) / rho) (plane strain), about
+    8 % slower than with the stress law as written; S is unchanged."""
+    seis, x0 = _plane_wave_run(BRINE, [("exx", DX / 2), ("vy", DX / 2)], stress_law=_abi.STRESS_DRAINED)
+    c = biot_constants(BRINE)
+    K_sat = BRINE["K_fr"] + c["alpha"] ** 2 * c["M"]
+    vp = np.sqrt((K_sat + 4.0 / 3.0 * c["mu"]) / c["rho"])
+    _, vs = biot_speeds_lf(BRINE)
+    vp_written, _ = biot_speeds_lf(BRINE)
+    got_p = _speed(seis, 0, 150, 399, (x0 + 60, 640), (x0 + 150, 640))
+    got_s = _speed(seis, 1, 150, 399, (x0 + 20, 640), (x0 + 60, 640))
+    assert abs(got_p / vp - 1) < 2e-3, (got_p, vp)
+    assert abs(got_s / vs - 1) < 2e-3, (got_s, vs)
+    assert vp_written / vp - 1 > 0.05
 
 
 @pytest.mark.slow
