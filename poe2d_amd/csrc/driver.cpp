@@ -38,6 +38,8 @@ struct Options {
     int precision = 32;
     std::string tile = "auto";
     bool quiet = false;
+    int stress_law = POE2D_STRESS_AS_WRITTEN;
+    int material = POE2D_MATERIAL_AS_WRITTEN;
 };
 
 // Per-medium fluid tables of the reference (seismic_CPML_2D_poroelastic_withCUDA.cu:585-598, the
@@ -68,6 +70,10 @@ void usage()
         "  --outdir D           where the output files go (.)\n"
         "  --precision 32|64    arithmetic of the time loop (32)\n"
         "  --tile auto|BWxTH    tile shape of the step kernel (auto: autotune)\n"
+        "  --stress-law written|drained   lambda_c as the reference writes it, or Biot's drained\n"
+        "                       lambda in the normal stresses (written)\n"
+        "  --material written|averaged    staggered-point material as written, or harmonic mu for\n"
+        "                       sigma_xy and mean material for v_y (written)\n"
         "  --quiet\n");
 }
 
@@ -105,6 +111,15 @@ bool parse(int argc, char **argv, Options &o)
         else if (k == "--precision") o.precision = std::atoi(val());
         else if (k == "--tile") o.tile = val();
         else if (k == "--quiet") o.quiet = true;
+        else if (k == "--stress-law") {
+            const std::string v = val();
+            if (v != "written" && v != "drained") return false;
+            o.stress_law = v == "drained" ? POE2D_STRESS_DRAINED : POE2D_STRESS_AS_WRITTEN;
+        } else if (k == "--material") {
+            const std::string v = val();
+            if (v != "written" && v != "averaged") return false;
+            o.material = v == "averaged" ? POE2D_MATERIAL_AVERAGED : POE2D_MATERIAL_AS_WRITTEN;
+        }
         else if (k == "-h" || k == "--help") {
             usage();
             std::exit(0);
@@ -212,6 +227,8 @@ int main(int argc, char **argv)
     g.cp_max = maxcp;
     g.precision = o.precision == 32 ? POE2D_F32 : POE2D_F64;
     g.check_cfl = 0;  // checked above, the reference's way
+    g.stress_law = o.stress_law;
+    g.staggered_material = o.material;
     poe2d_coeffs c{};
     c.dtype = POE2D_F32;
     c.rho = coef[0].data();

@@ -56,14 +56,25 @@ def draw_params(n, seed):
 INPUT_SCALE = 4.0
 
 
-def features(params):
-    """Network inputs: each parameter mapped linearly (radius, kappa: in log) onto [-1, 1], times
-    INPUT_SCALE (wider inputs let the first layer form the sharp changes that patches crossing a
-    receiver make)."""
+def patch_count(params, geom=CONFIG3):
+    """Nominal number of patches, gas fraction x model area / (pi radius^2)."""
+    p = np.asarray(params, np.float64)
+    area = (geom["n"] * geom["dx"]) ** 2
+    return p[:, 0] * area / (np.pi * p[:, 1] ** 2)
+
+
+def features(params, geom=CONFIG3):
+    """Network inputs: log(1 + nominal patch count), log radius, porosity and log permeability,
+    each mapped linearly onto [-1, 1] over the parameter ranges, times INPUT_SCALE.  The patch
+    count replaces the gas fraction (the two determine each other given the radius): patch k of
+    the fixed layout is painted once the count passes about k and covers a receiver once the
+    radius passes its distance, so the jumps a receiver sees sit along the input axes.  Wider
+    inputs let the first layer form those jumps sharply."""
     p = np.asarray(params, np.float64)
     lo_r, hi_r = np.log(RANGES["radius"][0]), np.log(RANGES["radius"][1])
     lo_k, hi_k = np.log(RANGES["kappa"][0]), np.log(RANGES["kappa"][1])
-    f = np.stack([p[:, 0] / 0.15 - 1.0,
+    nmax = patch_count([[RANGES["gas"][1], RANGES["radius"][0], 0, 0]], geom)[0]
+    f = np.stack([2.0 * np.log1p(patch_count(p, geom)) / np.log1p(nmax) - 1.0,
                   2.0 * (np.log(p[:, 1]) - lo_r) / (hi_r - lo_r) - 1.0,
                   (p[:, 2] - 0.25) / 0.1,
                   2.0 * (np.log(p[:, 3]) - lo_k) / (hi_k - lo_k) - 1.0], axis=1)
@@ -185,7 +196,7 @@ class ScreenResult:
 
 
 class Surrogate:
-    """Ensemble network from features(params) to a model's gather.  The network's targets are the
+    """Ensemble network from features(params, geom) to a model's gather.  The network's targets are the
     gathers with each component divided by its RMS over the training set."""
 
     def __init__(self, geom=CONFIG3, members=5, hidden=512, max_batch=1024, seed=1):
@@ -205,7 +216,7 @@ class Surrogate:
         g = np.asarray(gathers, np.float32)
         rms = np.sqrt(np.mean(g.astype(np.float64) ** 2, axis=(0, 2, 3)))
         self.scale = np.where(rms > 0, rms, 1.0).astype(np.float32)
-        self.net.set_dataset(features(params), self._scaled(g))
+        self.net.set_dataset(features(params, self.geom), self._scaled(g))
         rng = np.random.default_rng(seed)
         n = len(params)
         cuts = np.linspace(0, steps, chunks + 1).round().astype(int)
@@ -226,7 +237,7 @@ class Surrogate:
 
     def predict(self, params):
         """(gathers [n, 3, nrec, nsamp], ensemble spread [n] in units of the scaled targets)."""
-        mean, spread = self.net.predict(features(params))
+        mean, spread = self.net.predict(features(params, self.geom))
         g = mean.reshape((len(params),) + self.shape)
         g *= self.scale[None, :, None, None]
         return g, spread

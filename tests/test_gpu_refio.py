@@ -44,17 +44,19 @@ def make_case(d, n=199, nstep=400, src=(100, 90), a0=130, a1=150, f0=80):
     return dict(n=n, nstep=nstep, src=src, a0=a0, a1=a1, f0=f0, medium=medium, cp=cp, cs=cs, rho=rho)
 
 
-def test_reference_program_matches_oracle(tmp_path):
+@pytest.mark.parametrize("law,material", [("written", "written"), ("drained", "averaged")])
+def test_reference_program_matches_oracle(tmp_path, law, material):
     c = make_case(tmp_path)
     n, nstep, (is_, js) = c["n"], c["nstep"], c["src"]
     panj, panj2, samp = 50, 40, 150
     out = tmp_path / "out"
     out.mkdir()
-    say(f"poe2d-run {n + 1}^2 NSTEP={nstep}")
+    say(f"poe2d-run {n + 1}^2 NSTEP={nstep} stress law {law}, material {material}")
     r = subprocess.run([refio.RUNNER, "--nx", str(n), "--nz", str(n), "--medium", "medium.bin", "--cp", "cp.bin",
                         "--cs", "cs.bin", "--rho", "rho.bin", "--layout", "z", "--media", "media.txt",
                         "--panj", str(panj), "--panj2", str(panj2), "--samp", str(samp), "--outdir", "out",
-                        "--tile", "256x14"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+                        "--tile", "256x14", "--stress-law", law, "--material", material],
+                       cwd=tmp_path, capture_output=True, text=True, timeout=300)
     say(r.stdout.strip().replace("\n", " | "))
     assert r.returncode == 0, r.stderr
 
@@ -66,7 +68,9 @@ def test_reference_program_matches_oracle(tmp_path):
     prob = Problem(nx=n, nz=n, dx=1.25, dz=1.25, dt=95e-6, nt=nstep, coeffs=coeffs,
                    source=SourceSpec(kind=_abi.SRC_PRESSURE_DGAUSS, ix=is_, iz=js, amp=1e10, t_cut=0.4),
                    receivers=ReceiverSpec(ix=ix, iz=iz, comp_mask=_abi.VX | _abi.VY), npml=10, f0=80.0,
-                   cp_max=float(c["cp"].max()), check_cfl=0)
+                   cp_max=float(c["cp"].max()), check_cfl=0,
+                   stress_law=_abi.STRESS_DRAINED if law == "drained" else _abi.STRESS_AS_WRITTEN,
+                   staggered_material=_abi.MATERIAL_AVERAGED if material == "averaged" else _abi.MATERIAL_AS_WRITTEN)
     o = OracleSolver(prob, precision=_abi.F32)
     o.run()
     seis = o.seismograms()  # [vx, vy][receiver][sample]

@@ -21,12 +21,20 @@ def test_draws_cover_the_ranges():
 
 
 def test_features_map_range_ends_to_the_input_scale():
+    # most patches: largest gas fraction, smallest radius; none: no gas
     lo = [[0.0, 2.0, 0.15, 1e-14]]
     hi = [[0.3, 20.0, 0.35, 1e-11]]
-    mid = [[0.15, np.sqrt(40.0), 0.25, 10 ** -12.5]]
-    np.testing.assert_allclose(S.features(lo), -S.INPUT_SCALE, atol=1e-6)
-    np.testing.assert_allclose(S.features(hi), S.INPUT_SCALE, atol=1e-6)
-    np.testing.assert_allclose(S.features(mid), 0.0, atol=1e-6)
+    f_lo, f_hi = S.features(lo)[0], S.features(hi)[0]
+    np.testing.assert_allclose(f_lo[1:], -S.INPUT_SCALE, atol=1e-6)
+    np.testing.assert_allclose(f_hi[1:], S.INPUT_SCALE, atol=1e-6)
+    assert f_lo[0] == pytest.approx(-S.INPUT_SCALE)
+    assert S.features([[0.3, 2.0, 0.25, 1e-12]])[0][0] == pytest.approx(S.INPUT_SCALE)
+    np.testing.assert_allclose(S.features([[0.15, np.sqrt(40.0), 0.25, 10 ** -12.5]])[0][1:], 0.0, atol=1e-6)
+    # the nominal patch count: area x gas / (pi r^2) over the 200 m x 200 m model
+    assert S.patch_count([[0.1, 5.0, 0.2, 1e-12]])[0] == pytest.approx(0.1 * 200.0 ** 2 / (np.pi * 25.0))
+    # monotone in the count, which grows with gas and falls with radius
+    a = S.features([[0.1, 5.0, 0.2, 1e-12], [0.2, 5.0, 0.2, 1e-12], [0.2, 10.0, 0.2, 1e-12]])[:, 0]
+    assert a[0] < a[1] and a[2] < a[1]
 
 
 def test_model_is_a_function_of_its_parameters():
