@@ -62,19 +62,21 @@ def strip_counts(prob):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fused_step_4001x4001_traffic.txt")
 
 
-def recorded_traffic(tile, n):
-    """HBM bytes per step launch from the committed PMC summary ("grid_n: N" and
-    "tile BWxTH: bytes" lines), if it was measured on this grid with this tile shape."""
+def recorded_traffic(tile, n, schedule=0):
+    """HBM bytes per time step from the committed PMC summary ("grid_n: N", "tile BWxTH: bytes"
+    for the one-step kernel and "two-step BWxTH: bytes" for the two-step kernel, bytes per
+    launch), if it was measured on this grid with this kernel and tile shape."""
     try:
         lines = [ln.strip() for ln in open(TRAFFIC_FILE) if ln.strip() and not ln.startswith("#")]
     except OSError:
         return None, None
     kv = dict(ln.split(":", 1) for ln in lines if ":" in ln)
     kv = {k.strip(): v.strip() for k, v in kv.items()}
+    key, per = (f"tile {tile[0]}x{tile[1]}", 1) if schedule == 0 else (f"two-step {tile[0]}x{tile[1]}", 2)
     try:
         if int(kv.get("grid_n", -1)) != n + 1:
             return None, None
-        return float(kv[f"tile {tile[0]}x{tile[1]}"]), os.path.relpath(TRAFFIC_FILE, ROOT)
+        return float(kv[key]) / per, os.path.relpath(TRAFFIC_FILE, ROOT)
     except (KeyError, ValueError):
         return None, None
 
@@ -306,6 +308,9 @@ def main():
     ap.add_argument("--tile", default="auto", metavar="BWxTH",
                     help="tile shape of the step kernel (block width x tile rows); 'auto' runs the "
                          "library's autotuner first")
+    ap.add_argument("--schedule", default="auto", choices=("auto", "0", "3"),
+                    help="step kernel: 0 = one step per launch, 3 = two steps per launch, auto = the "
+                         "faster after autotuning both")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="instead of the benchmark: time a plain device copy of MIB MiB (torch; the copy "
                          "that scales the FETCH_SIZE / WRITE_SIZE counters), then the library's stream "
@@ -339,12 +344,28 @@ def main():
 
     solver = Solver(prob, precision=F32)
     palette = solver.material_palette()
+    schedules = [0, 3] if args.schedule == "auto" else [int(args.schedule)]
     if args.tile == "auto":
-        tile = solver.autotune(10)
+        # autotune each schedule (one and two steps per launch), keep the faster
+        best = None
+        for sched in schedules:
+            solver.set_schedule(sched, 256, 12 if sched == 3 else 8)
+            t = solver.autotune(10)
+            solver.reset()
+            solver.set_schedule(sched, *t)
+            solver.run(20)
+            ms, n = solver.last_run_ms()
+            print(f"[bench] schedule {sched}: autotuned {t[0]}x{t[1]}, {ms / n * 1e3:.1f} us per step",
+                  file=sys.stderr, flush=True)
+            if best is None or ms / n < best[0]:
+                best = (ms / n, sched, t)
+        _, schedule, tile = best
+        solver.set_schedule(schedule, *tile)
         tile_note = "autotuned"
     else:
+        schedule = schedules[0]
         tile = tuple(int(v) for v in args.tile.lower().split("x"))
-        solver.set_schedule(0, *tile)
+        solver.set_schedule(schedule, *tile)
         tile_note = "fixed"
     for _ in range(args.warmup):
         solver.reset()
@@ -366,7 +387,7 @@ def main():
     gcell = npts * args.nt / (ms_solve / 1e3) / 1e9
     step_ms = ms_solve / args.nt
     achieved = bytes_step / (step_ms / 1e3) / 1e9
-    traffic, traffic_src = recorded_traffic(tile, args.n)
+    traffic, traffic_src = recorded_traffic(tile, args.n, schedule)
     out = {
         "metric": "Gcell-updates/s (FP32 Biot C-PML time loop, 4001x4001, 6000 steps)",
         "value": round(gcell * 1.0, 3),
@@ -386,11 +407,14 @@ def main():
                    "gas_fraction": prob.meta.get("gas_fraction_realised"),
                    "step": "one full solve", "wall_ms_median": round(statistics.median(wall_ms), 3),
                    "traces_finite": finite, "tile": f"{tile[0]}x{tile[1]} ({tile_note})",
+                   "schedule": {0: "one step per launch", 3: "two steps per launch"}[schedule],
                    "material_palette": palette},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None if traffic is None else round(traffic),
-                     "kernel": "fused_step (one launch per time step, 32-step HIP graphs)",
+                     "kernel": ("fused_step (one launch per time step, 32-step HIP graphs)" if schedule == 0 else
+                                "fused_step2 (one launch per two time steps, 32-step HIP graphs); per step: "
+                                "half a launch"),
                      "step_us": round(step_ms * 1e3, 2),
                      "per_launch": f"one time step: {bytes_step / 1e9:.4f} GB algorithmic "
                                    f"(104 B x {npts} points + 48 B x strip points, {cols} strip columns, "

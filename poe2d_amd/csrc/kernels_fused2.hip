@@ -1,0 +1,423 @@
+// Two time steps per launch (temporal blocking of the single-pass step, palette grids).
+//
+// A block owns BW - 12 output columns (6 halo columns on each side) and th rows of one model and
+// marches down its rows as the single-pass kernel does, running two copies of that kernel's row
+// step one after the other, each on its own time level:
+//   stage 1 (step n)   at row step s: stresses of row s, velocities of row s - 2, from the state
+//                      read from HBM (as the single-pass kernel);
+//   stage 2 (step n+1) at row step s: stresses of row s - 4, velocities of row s - 6, from stage
+//                      1's results, which never leave the chip.
+// Stage 2 lags by 4 rows because its stresses of row r need the step-(n+1) velocities of rows
+// r-2 .. r+2 (x velocity at r+2 for the shear-stress z derivative), the last of which stage 1
+// produces at row step r+4.  Only stage 2 stores state (rows [zA, zB), the owned lanes); stage
+// 1 computes every row and lane it can, and the rows and lanes whose inputs are incomplete hold
+// values nothing valid depends on:
+//   stage-1 stresses  valid on lanes [2, BW-3]  (x neighbours of the velocities read from HBM)
+//   stage-1 velocities             [3, BW-4]
+//   stage-2 stresses               [4, BW-6] (sxx, p), [5, BW-5] (sxy)
+//   stage-2 velocities             [6, BW-7]   -> owned lanes [6, BW-6)
+// and in rows, stage 2's rows [zA, zB) need stage-2 stresses of rows [zA-2, zB+1], step-(n+1)
+// velocities of rows [zA-4, zB+3] and stage-1 stresses of rows [zA-6, zB+5]: the row steps run
+// s = zA-6 .. zB+5.
+//
+// C-PML memory variables of step n+1 go through a third record set (psi_mid): stage 1 writes the
+// records of the points whose stage-1 values are valid, stage 2 reads them back from the same
+// lane four row steps later (a thread's store and its later load of the same address are
+// ordered).  Neighbouring tiles compute the same stage-1 values for the halo points they share
+// and store identical bits there.  Receivers record step n from stage 1 (row s-1 at row step s)
+// and step n+1 from stage 2 (row s-5 at row step s), from the LDS rings.
+//
+// The state of step n is read from buffer `in` and that of step n+2 written to buffer in ^ 1, so
+// the buffer holding the state flips once per launch: in = ((n - pair0) / 2 + pcur) & 1.
+#include "step_common.h"
+
+namespace poe2d {
+
+template <class T, int BW, bool AVG>
+__global__ void __launch_bounds__(BW)
+    fused_step2(const StepArgs<T> *__restrict__ models, const WorkItem *__restrict__ work, int nwork, int per_xcd,
+                const long long *__restrict__ base, int k)
+{
+    constexpr int W = fused2_tile_width(BW);
+    constexpr uint32_t VB = sizeof(Vec4<T>);
+    typedef __attribute__((address_space(1))) nvec4<T> GN;
+    typedef __attribute__((address_space(1))) T GT;
+    __shared__ T lv[3][3][BW + 4];   // ring of 3 step-n velocity rows: vx, wx, vy (entry lane + 2)
+    __shared__ T ls[3][3][BW + 4];   // ring of 3 stage-1 stress rows: sxx, p, sxy
+    __shared__ T lv2[6][3][BW + 4];  // ring of 6 step-(n+1) velocity rows
+    __shared__ T ls2[3][3][BW + 4];  // ring of 3 stage-2 stress rows
+    __shared__ Vec4<T> lpal[2 * kMaxPalette];  // MS entries, then MV entries
+
+    const int wi = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+    if (wi >= nwork) return;
+    const WorkItem item = work[wi];
+    StepArgs<T> A = models[item.model];
+    const long long n = *(const kcll *)base + k;  // the launch advances steps n and n + 1
+    A.in = (int)((((n - A.pair0) >> 1) + A.pcur) & 1);
+    A.out = A.in ^ 1;
+    typedef __attribute__((address_space(1))) const T GCT;
+    auto sample = [&](long long m) -> T {
+        const long long q = m - A.src_n0;
+        return (A.src_series && q >= 0 && q < A.src_len) ? ((GCT *)A.src_series)[q] : T(0);
+    };
+    const T src1 = sample(n), src2 = sample(n + 1);
+    const int tile = item.tile;
+    const int tx = tile % A.ntx;
+    const int tz = tile / A.ntx;
+    const int lane = threadIdx.x;
+    const int nx = A.nx, nz = A.nz;
+    const int c = tx * W - 6 + lane;
+    const int zA = tz * A.th;
+    const int zB = min(zA + A.th, nz + 1);
+    const int P = A.pitch;
+    const bool own = lane >= 6 && lane < BW - 6 && c <= nx;
+    const bool ps1 = lane >= 2 && lane <= BW - 3;  // stage-1 stress records worth keeping
+    const bool pv1 = lane >= 3 && lane <= BW - 4;  // stage-1 velocity records worth keeping
+    const T rdx = A.rdx, rdz = A.rdz, dt = A.dt;
+
+    gchar *const gb = (gchar *)A.grp;
+    const long long gsb = A.group_stride * (long long)VB;
+    const gchar *Vo = gb + group_v(A.in) * gsb;
+    const gchar *So = gb + group_s(A.in) * gsb;
+    gchar *Vn = gb + group_v(A.out) * gsb;
+    gchar *Sn = gb + group_s(A.out) * gsb;
+    const GT *const tab = (const GT *)A.tab;
+    GT *const psi = (GT *)A.psi;
+    GT *const psim = (GT *)A.psi_mid;
+    // memory-variable records: x strip record r / z strip record r of state buffer b, and of the
+    // step-(n+1) set
+    auto px = [&](int b, long long r) -> GT * { return psi + ((long long)b * A.pxn + r) * NPSI_X; };
+    auto pz = [&](int b, long long r) -> GT * {
+        return psi + 2LL * NPSI_X * A.pxn + ((long long)b * A.pzn + r) * NPSI_Z;
+    };
+    auto pxm = [&](long long r) -> GT * { return psim + r * NPSI_X; };
+    auto pzm = [&](long long r) -> GT * { return psim + A.pxn * NPSI_X + r * NPSI_Z; };
+    // Lanes left of the grid's pad columns (c < -kPadCols at the first tile column) read pad column
+    // -kPadCols, lanes right of the grid column nx+1: zero either way.  Rows past the bottom pad
+    // rows (reached only by the over-run of the last group of row steps) read the last pad row.
+    const int cc = min(max(c, -kPadCols), nx + 1);
+    const uint32_t colb = (uint32_t)(cc + kPadCols) * VB;
+    const uint32_t rowb = (uint32_t)P * VB;
+    const int last_row = nz + kPadRows;
+    auto off = [&](int row) -> uint32_t { return colb + (uint32_t)(min(row, last_row) + kPadRows) * rowb; };
+    auto ld = [&](const gchar *g, int row) -> Vec4<T> {
+        const nvec4<T> v = *(const GN *)(g + off(row));
+        return Vec4<T>{v.x, v.y, v.z, v.w};
+    };
+    const Vec4<T> zero4 = {T(0), T(0), T(0), T(0)};
+    const uint32_t colm = (uint32_t)(cc + kPadCols);
+    gcuchar *const mid = (gcuchar *)A.mid;
+    auto ldm = [&](int row) -> int { return mid[colm + (uint32_t)(min(row, last_row) + kPadRows) * (uint32_t)P]; };
+    {
+        const GN *const pal = (const GN *)A.pal;
+        for (int i = lane; i < 2 * kMaxPalette; i += BW)
+            if ((i & (kMaxPalette - 1)) < A.npal) {
+                const nvec4<T> e = pal[i];
+                lpal[i] = Vec4<T>{e.x, e.y, e.z, e.w};
+            }
+    }
+
+    // per-column C-PML profile (x strips only)
+    const int sx = (c >= 0 && c <= nx) ? ((gcint *)A.xmap)[c] : -1;
+    T ax = 0, bx = 1, rkx = 1, axh = 0, bxh = 1, rkxh = 1;
+    if (sx >= 0) {
+        ax = tab[TB_AX * A.tn + c]; bx = tab[TB_BX * A.tn + c]; rkx = tab[TB_RKX * A.tn + c];
+        axh = tab[TB_AXH * A.tn + c]; bxh = tab[TB_BXH * A.tn + c]; rkxh = tab[TB_RKXH * A.tn + c];
+    }
+    const bool xr1 = c >= 1 && c < nx;      // stress1
+    const bool xr2 = c >= 2 && c < nx;      // stress2
+    const bool xr3 = c >= 1 && c < nx - 1;  // stress3
+    const bool xr4 = c >= 2 && c < nx;      // vx
+    const bool xr5 = c >= 1 && c < nx - 1;  // vz
+    const bool src_p = A.src_kind == POE2D_SRC_PRESSURE_DGAUSS && c == A.src_x;
+    const bool src_f = A.src_kind == POE2D_SRC_FORCE_Y_RICKER && c == A.src_x;
+    // receivers of the two steps: slots and the tile's entries per row
+    kcint *const rec_ptr = (kcint *)A.rec_tile_ptr + (long long)tx * (nz + 1);
+    const bool rec_tile = A.rec_n > 0 && rec_ptr[zA] < rec_ptr[zB];
+    const int n1 = (int)n, n2 = n1 + 1;
+    const bool rec1 = rec_tile && n1 % A.rec_dec == 0 && n1 / A.rec_dec < A.rec_nsamp;
+    const bool rec2 = rec_tile && n2 % A.rec_dec == 0 && n2 / A.rec_dec < A.rec_nsamp;
+    GT *const trace1 = (GT *)A.traces + (rec1 ? (long long)(n1 / A.rec_dec) * A.rec_ncomp * A.rec_n : 0);
+    GT *const trace2 = (GT *)A.traces + (rec2 ? (long long)(n2 / A.rec_dec) * A.rec_ncomp * A.rec_n : 0);
+    typedef int nint2 __attribute__((ext_vector_type(2)));
+    const __attribute__((address_space(1))) nint2 *const rec_ent = (const __attribute__((address_space(1))) nint2 *)A.rec_entries;
+    auto gather = [&](GT *trace, int row, const T (*xv)[BW + 4], const T (*xs)[BW + 4]) {
+        const int lo = rec_ptr[row], hi = rec_ptr[row + 1];
+        for (int e = lo + lane; e < hi; e += BW) {
+            const nint2 en = rec_ent[e];
+            const int li = en.x - tx * W + 8;  // LDS entry of column en.x
+            int q = 0;
+            if (A.rec_mask & POE2D_VX) trace[(long long)(q++) * A.rec_n + en.y] = xv[0][li];
+            if (A.rec_mask & POE2D_VY) trace[(long long)(q++) * A.rec_n + en.y] = xv[2][li];
+            if (A.rec_mask & POE2D_P) trace[(long long)q * A.rec_n + en.y] = xs[1][li];
+        }
+    };
+
+    // One stress update of row r at this lane (the single-pass kernel's, with the velocity rows
+    // r-2 .. r+2 given, the x neighbours of row r in xv, the memory records read through po_*
+    // and written through pn_* when `keep`).  Returns the new (e_xx, e_yy, zeta, sigma_xy) and
+    // sets sxx, p, syy.
+    auto stress = [&](int r, const Vec4<T> &vm2, const Vec4<T> &vm1, const Vec4<T> &v0, const Vec4<T> &vp1,
+                      const Vec4<T> &vp2, const Vec4<T> &so, const T (*xv)[BW + 4], const Vec4<T> &ms_cur,
+                      const Vec4<T> &ms_nxt, auto pxo, auto pxn_, auto pzo, auto pzn_, bool keep, T srcv, T &sxx,
+                      T &p, T &syy) __attribute__((always_inline)) -> Vec4<T> {
+        const bool zr1 = r >= 2 && r < nz - 1;
+        const bool zr2 = r >= 1 && r < nz - 1;
+        const int szs = zstrip(A, r);
+        T exx_n = so.x, eyy_n = so.y, zeta_n = so.z, sxy_n = so.w;
+        p = 0;
+        sxx = 0;
+        syy = 0;
+        if (xr1 && zr1) {
+            T dvx = d4(xv[0][lane + 3], v0.x, xv[0][lane + 4], xv[0][lane + 1], rdx);
+            T dwx = d4(xv[1][lane + 3], v0.z, xv[1][lane + 4], xv[1][lane + 1], rdx);
+            T dvz = d4(v0.y, vm1.y, vp1.y, vm2.y, rdz);
+            T dwz = d4(v0.w, vm1.w, vp1.w, vm2.w, rdz);
+            if (sx >= 0) {
+                const long long q = (long long)r * A.wxs + sx;
+                dvx = cpml(dvx, pxo(q) + PX_DVX, pxn_(q) + PX_DVX, keep, axh, bxh, rkxh);
+                dwx = cpml(dwx, pxo(q) + PX_DWX, pxn_(q) + PX_DWX, keep, axh, bxh, rkxh);
+            }
+            if (szs >= 0) {
+                const long long q = (long long)szs * P + c;
+                const T az = tab[TB_AZ * A.tn + r], bz = tab[TB_BZ * A.tn + r], rkz = tab[TB_RKZ * A.tn + r];
+                dvz = cpml(dvz, pzo(q) + PZ_DVZ, pzn_(q) + PZ_DVZ, keep, az, bz, rkz);
+                dwz = cpml(dwz, pzo(q) + PZ_DWZ, pzn_(q) + PZ_DWZ, keep, az, bz, rkz);
+            }
+            exx_n = so.x + dt * dvx;
+            eyy_n = so.y + dt * dvz;
+            zeta_n = (so.z - dwx * dt) - dwz * dt;
+            p = pore_pressure(exx_n, eyy_n, zeta_n, ms_cur.x, ms_cur.y);
+            if (src_p && r == A.src_z) p += srcv;
+            if (xr3) {
+                const T lc = ms_cur.z, al = ms_cur.x;
+                const T l2m = lc + T(2) * ms_cur.w;
+                sxx = l2m * exx_n + lc * eyy_n - al * p;
+                syy = lc * exx_n + l2m * eyy_n - al * p;
+            }
+        }
+        if (xr2 && zr2) {
+            T ddx = d4(v0.y, xv[2][lane + 1], xv[2][lane + 3], xv[2][lane], rdx);
+            T ddz = d4(vp1.x, v0.x, vp2.x, vm1.x, rdz);
+            if (sx >= 0) {
+                const long long q = (long long)r * A.wxs + sx;
+                ddx = cpml(ddx, pxo(q) + PX_DVY, pxn_(q) + PX_DVY, keep, ax, bx, rkx);
+            }
+            if (szs >= 0) {
+                // half-grid memory coefficients, full-grid K_z divisor (reference :224-226)
+                const long long q = (long long)szs * P + c;
+                ddz = cpml(ddz, pzo(q) + PZ_DVX, pzn_(q) + PZ_DVX, keep, tab[TB_AZH * A.tn + r],
+                           tab[TB_BZH * A.tn + r], tab[TB_RKZ * A.tn + r]);
+            }
+            const T mu_xy = AVG ? harmonic_mu(ms_cur.w, ms_nxt.w) : ms_nxt.w;
+            sxy_n = so.w + mu_xy * (ddx + ddz) * dt;
+        }
+        return Vec4<T>{exx_n, eyy_n, zeta_n, sxy_n};
+    };
+
+    // One velocity update of row j at this lane: v the step's old velocities of row j, xs the
+    // new stress row j (x neighbours), sxy of rows j-2 .. j+1 and syy, p of rows j-1 .. j+2.
+    auto velocity = [&](int j, Vec4<T> v, const T (*xs)[BW + 4], T sxy_m2, T sxy_m1, T sxy_0, T sxy_p1, T syy_m1,
+                        T syy_0, T syy_p1, T syy_p2, T p_m1, T p_0, T p_p1, T p_p2, const Vec4<T> &mv_j,
+                        const Vec4<T> &mv_j1, auto pxo, auto pxn_, auto pzo, auto pzn_, bool keep,
+                        T srcv) __attribute__((always_inline)) -> Vec4<T> {
+        const int szj = zstrip(A, j);
+        if (xr4 && j >= 2 && j < nz) {
+            T e1 = d4(xs[0][lane + 2], xs[0][lane + 1], xs[0][lane + 3], xs[0][lane], rdx);
+            T e2 = d4(xs[1][lane + 2], xs[1][lane + 1], xs[1][lane + 3], xs[1][lane], rdx);
+            T e3 = d4(sxy_0, sxy_m1, sxy_p1, sxy_m2, rdz);
+            if (sx >= 0) {
+                const long long q = (long long)j * A.wxs + sx;
+                e1 = cpml(e1, pxo(q) + PX_DSXX, pxn_(q) + PX_DSXX, keep, ax, bx, rkx);
+                e2 = cpml(e2, pxo(q) + PX_DPX, pxn_(q) + PX_DPX, keep, ax, bx, rkx);
+            }
+            if (szj >= 0) {
+                const long long q = (long long)szj * P + c;
+                e3 = cpml(e3, pzo(q) + PZ_DSXY, pzn_(q) + PZ_DSXY, keep, tab[TB_AZ * A.tn + j],
+                          tab[TB_BZ * A.tn + j], tab[TB_RKZ * A.tn + j]);
+            }
+            biot_velocity(v.x, v.z, e1 + e3, e2, mv_j.x, mv_j.y, mv_j.z, mv_j.w, A.inv_dt);
+        }
+        if (xr5 && j >= 1 && j < nz - 1) {
+            T e1 = d4(xs[2][lane + 3], xs[2][lane + 2], xs[2][lane + 4], xs[2][lane + 1], rdx);
+            T e2 = d4(syy_p1, syy_0, syy_p2, syy_m1, rdz);
+            T e3 = d4(p_p1, p_0, p_p2, p_m1, rdz);
+            if (sx >= 0) {
+                const long long q = (long long)j * A.wxs + sx;
+                e1 = cpml(e1, pxo(q) + PX_DSXY, pxn_(q) + PX_DSXY, keep, axh, bxh, rkxh);
+            }
+            if (szj >= 0) {
+                const long long q = (long long)szj * P + c;
+                const T azh = tab[TB_AZH * A.tn + j], bzh = tab[TB_BZH * A.tn + j];
+                const T rkzh = tab[TB_RKZH * A.tn + j];
+                e2 = cpml(e2, pzo(q) + PZ_DSYY, pzn_(q) + PZ_DSYY, keep, azh, bzh, rkzh);
+                e3 = cpml(e3, pzo(q) + PZ_DPZ, pzn_(q) + PZ_DPZ, keep, azh, bzh, rkzh);
+            }
+            T div = e1 + e2;
+            if (src_f && j == A.src_z) div += srcv;
+            const Vec4<T> mvy = AVG ? mean_rows(mv_j, mv_j1) : mv_j1;
+            biot_velocity(v.y, v.w, div, e3, mvy.x, mvy.y, mvy.z, mvy.w, A.inv_dt);
+        }
+        return v;
+    };
+    auto px_in = [&](long long q) { return px(A.in, q); };
+    auto pz_in = [&](long long q) { return pz(A.in, q); };
+    auto px_out = [&](long long q) { return px(A.out, q); };
+    auto pz_out = [&](long long q) { return pz(A.out, q); };
+
+    // Register rings; row r of a ring of size R sits in slot (r - s0) mod R, so at phase PH of the
+    // unrolled group row s + k is in slot (PH + k) mod R.
+    const int s0 = zA - 6;
+    const int send = zB + 6;
+    Vec4<T> vr[6];   // step-n velocities, rows s-2 .. s+3
+    Vec4<T> sr[2];   // step-n strains, rows s, s+1
+    int idr[6];      // material index, rows s-1 .. s+2
+    Vec4<T> msr[6];  // (alpha, M, lambda, mu), rows s-4 .. s+1
+    Vec4<T> mvr[6];  // (rho, rho_f, m, b), rows s-6 .. s-1
+    T sxyr[6], syyr[6], pr[6];     // stage-1 stress rows s-4 .. s
+    Vec4<T> v1r[6];  // step-(n+1) velocities, rows s-7 .. s-2
+    Vec4<T> s1r[6];  // step-(n+1) strains, rows s-4 .. s
+    T sxyr2[6], syyr2[6], pr2[6];  // stage-2 stress rows s-8 .. s-4
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        sxyr[q] = syyr[q] = pr[q] = T(0);
+        sxyr2[q] = syyr2[q] = pr2[q] = T(0);
+        idr[q] = 0;
+        msr[q] = mvr[q] = v1r[q] = s1r[q] = zero4;
+    }
+#pragma unroll
+    for (int q = -2; q <= 2; ++q) vr[(q + 6) % 6] = ld(Vo, s0 + q);
+    sr[0] = ld(So, s0);
+    sr[1] = zero4;
+    vr[3] = zero4;
+    idr[5] = ldm(s0 - 1);
+    idr[0] = ldm(s0);
+    idr[1] = ldm(s0 + 1);
+    __syncthreads();  // palette in LDS
+    msr[0] = lpal[idr[0]];
+
+    int s = s0;
+    auto row_step = [&](auto phase) __attribute__((always_inline)) {
+        constexpr int PH = decltype(phase)::value;
+        auto V = [&](int q) -> Vec4<T> & { return vr[(PH + q + 12) % 6]; };    // step-n velocity row s+q
+        auto V1 = [&](int q) -> Vec4<T> & { return v1r[(PH + q + 12) % 6]; };  // step-(n+1) velocity row s+q
+        const Vec4<T> so = sr[PH & 1];
+        // ---- loads for later row steps and the material of this one
+        V(3) = ld(Vo, s + 3);
+        sr[(PH + 1) & 1] = ld(So, s + 1);
+        msr[(PH + 1) % 6] = lpal[idr[(PH + 1) % 6]];                // row s+1
+        mvr[(PH + 5) % 6] = lpal[kMaxPalette + idr[(PH + 5) % 6]];  // row s-1
+        idr[(PH + 2) % 6] = ldm(s + 2);
+        const Vec4<T> &ms_cur = msr[PH % 6], &ms_nxt = msr[(PH + 1) % 6];
+        const Vec4<T> &mv_j = mvr[(PH + 4) % 6], &mv_j1 = mvr[(PH + 5) % 6];
+        const Vec4<T> &ms2_cur = msr[(PH + 2) % 6], &ms2_nxt = msr[(PH + 3) % 6];
+        const Vec4<T> &mv2_j = mvr[PH % 6], &mv2_j1 = mvr[(PH + 1) % 6];
+
+        // ---- x neighbours through LDS: step-n velocities of row s, step-(n+1) velocities of row s-3
+        {
+            T(*xv)[BW + 4] = lv[PH % 3];
+            xv[0][lane + 2] = V(0).x;
+            xv[1][lane + 2] = V(0).z;
+            xv[2][lane + 2] = V(0).y;
+            T(*xv2)[BW + 4] = lv2[(PH + 3) % 6];
+            xv2[0][lane + 2] = V1(-3).x;
+            xv2[1][lane + 2] = V1(-3).z;
+            xv2[2][lane + 2] = V1(-3).y;
+        }
+        __syncthreads();
+
+        // ---- receivers: step n at row s-1 (stage 1), step n+1 at row s-5 (stage 2)
+        if (rec1 && s - 1 >= zA && s - 1 < zB) gather(trace1, s - 1, lv[(PH + 2) % 3], ls[(PH + 2) % 3]);
+        if (rec2 && s - 5 >= zA && s - 5 < zB) gather(trace2, s - 5, lv2[(PH + 1) % 6], ls2[(PH + 1) % 3]);
+
+        // ---- stage 1: stresses of row s (step n), kept for stage 2 in the register ring
+        T sxx, p, syy;
+        const Vec4<T> s1 = stress(s, V(-2), V(-1), V(0), V(1), V(2), so, lv[PH % 3], ms_cur, ms_nxt, px_in, pxm,
+                                  pz_in, pzm, ps1, src1, sxx, p, syy);
+        s1r[PH % 6] = s1;
+        {
+            T(*xs)[BW + 4] = ls[PH % 3];
+            xs[0][lane + 2] = sxx;
+            xs[1][lane + 2] = p;
+            xs[2][lane + 2] = s1.w;
+        }
+        sxyr[PH % 6] = s1.w;
+        syyr[PH % 6] = syy;
+        pr[PH % 6] = p;
+        auto SXY = [&](int q) -> T { return sxyr[(PH + q + 12) % 6]; };  // stage-1 row s+q
+        auto SYY = [&](int q) -> T { return syyr[(PH + q + 12) % 6]; };
+        auto PP = [&](int q) -> T { return pr[(PH + q + 12) % 6]; };
+
+        // ---- stage 1: velocities of row s-2 (step n -> n+1)
+        V1(-2) = velocity(s - 2, V(-2), ls[(PH + 1) % 3], SXY(-4), SXY(-3), SXY(-2), SXY(-1), SYY(-3), SYY(-2),
+                          SYY(-1), SYY(0), PP(-3), PP(-2), PP(-1), PP(0), mv_j, mv_j1, px_in, pxm, pz_in, pzm, pv1,
+                          src1);
+
+        // ---- stage 2: stresses of row s-4 (step n+1)
+        const int r2 = s - 4;
+        const bool srow = own && r2 >= zA && r2 < zB;
+        T sxx2, p2, syy2;
+        const Vec4<T> s2 = stress(r2, V1(-6), V1(-5), V1(-4), V1(-3), V1(-2), s1r[(PH + 2) % 6], lv2[(PH + 2) % 6],
+                                  ms2_cur, ms2_nxt, pxm, px_out, pzm, pz_out, srow, src2, sxx2, p2, syy2);
+        if (srow) *(GN *)(Sn + off(r2)) = nvec4<T>{s2.x, s2.y, s2.z, s2.w};
+        {
+            T(*xs)[BW + 4] = ls2[(PH + 2) % 3];
+            xs[0][lane + 2] = sxx2;
+            xs[1][lane + 2] = p2;
+            xs[2][lane + 2] = s2.w;
+        }
+        sxyr2[(PH + 2) % 6] = s2.w;
+        syyr2[(PH + 2) % 6] = syy2;
+        pr2[(PH + 2) % 6] = p2;
+        auto SXY2 = [&](int q) -> T { return sxyr2[(PH + q + 12) % 6]; };  // stage-2 row s+q
+        auto SYY2 = [&](int q) -> T { return syyr2[(PH + q + 12) % 6]; };
+        auto PP2 = [&](int q) -> T { return pr2[(PH + q + 12) % 6]; };
+
+        // ---- stage 2: velocities of row s-6 (step n+1 -> n+2)
+        const int j2 = s - 6;
+        const bool vrow = own && j2 >= zA && j2 < zB;
+        const Vec4<T> v2 = velocity(j2, V1(-6), ls2[PH % 3], SXY2(-8), SXY2(-7), SXY2(-6), SXY2(-5), SYY2(-7),
+                                    SYY2(-6), SYY2(-5), SYY2(-4), PP2(-7), PP2(-6), PP2(-5), PP2(-4), mv2_j, mv2_j1,
+                                    pxm, px_out, pzm, pz_out, vrow, src2);
+        if (vrow) *(GN *)(Vn + off(j2)) = nvec4<T>{v2.x, v2.y, v2.z, v2.w};
+        ++s;
+    };
+
+    // Whole groups of 6 row steps: the last group may run up to 5 steps past zB + 5, over rows that
+    // store nothing and read zero pad rows (loads clamp to the last pad row).
+    for (int g = (send - s0 + 5) / 6; g > 0; --g) {
+        row_step(std::integral_constant<int, 0>{});
+        row_step(std::integral_constant<int, 1>{});
+        row_step(std::integral_constant<int, 2>{});
+        row_step(std::integral_constant<int, 3>{});
+        row_step(std::integral_constant<int, 4>{});
+        row_step(std::integral_constant<int, 5>{});
+    }
+}
+
+template <class T, bool AVG>
+hipError_t launch_fused2_bw(const StepArgs<T> *models, const WorkItem *work, int nwork, int per_xcd, int bw,
+                            const long long *base, int k, hipStream_t stream)
+{
+    const dim3 grid(8 * per_xcd);
+    switch (bw) {
+    case 128: fused_step2<T, 128, AVG><<<grid, 128, 0, stream>>>(models, work, nwork, per_xcd, base, k); break;
+    case 256: fused_step2<T, 256, AVG><<<grid, 256, 0, stream>>>(models, work, nwork, per_xcd, base, k); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_fused2(const StepArgs<T> *models, const WorkItem *work, int nwork, int per_xcd, int bw, bool avg,
+                         const long long *base, int k, hipStream_t stream)
+{
+    if (nwork < 1 || per_xcd < 1 || 8LL * per_xcd < nwork) return hipErrorInvalidValue;
+    return avg ? launch_fused2_bw<T, true>(models, work, nwork, per_xcd, bw, base, k, stream)
+               : launch_fused2_bw<T, false>(models, work, nwork, per_xcd, bw, base, k, stream);
+}
+
+template hipError_t launch_fused2<float>(const StepArgs<float> *, const WorkItem *, int, int, int, bool,
+                                         const long long *, int, hipStream_t);
+
+}  // namespace poe2d

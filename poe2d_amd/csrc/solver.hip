@@ -52,7 +52,7 @@ const int kGraphSteps = 32;                 // steps per captured graph
     } while (0)
 
 // single-pass (palette when the grid has one), two-pass, single-pass always on the MS/MV groups
-enum { SCHED_FUSED = 0, SCHED_TWOPASS = 1, SCHED_FUSED_GROUPS = 2 };
+enum { SCHED_FUSED = 0, SCHED_TWOPASS = 1, SCHED_FUSED_GROUPS = 2, SCHED_FUSED2 = 3 };
 
 poe2d_status dev_alloc(void **p, size_t bytes)
 {
@@ -135,16 +135,17 @@ struct StepList {
     int nmodels = 0, nwork = 0, per_xcd = 0, bw = 0;
     bool palette = false;
     bool avg = false;  // POE2D_MATERIAL_AVERAGED (all members alike)
+    bool two = false;  // two-step kernel: each launch advances two steps
     int precision = POE2D_F32;
     // graph of kGraphSteps launches plus the base advance, and the launch it was captured for
     hipGraphExec_t exec = nullptr;
     struct Sig {
         void *models, *work, *base;
-        int nwork, per_xcd, bw, palette, avg, precision;
+        int nwork, per_xcd, bw, palette, avg, two, precision;
         bool operator==(const Sig &o) const
         {
             return models == o.models && work == o.work && base == o.base && nwork == o.nwork &&
-                   per_xcd == o.per_xcd && bw == o.bw && palette == o.palette && avg == o.avg &&
+                   per_xcd == o.per_xcd && bw == o.bw && palette == o.palette && avg == o.avg && two == o.two &&
                    precision == o.precision;
         }
     } sig{};
@@ -153,7 +154,8 @@ struct StepList {
     {
         Sig g{};
         g.models = models; g.work = work; g.base = base;
-        g.nwork = nwork; g.per_xcd = per_xcd; g.bw = bw; g.palette = palette; g.avg = avg; g.precision = precision;
+        g.nwork = nwork; g.per_xcd = per_xcd; g.bw = bw; g.palette = palette; g.avg = avg; g.two = two;
+        g.precision = precision;
         return g;
     }
 
@@ -196,6 +198,7 @@ struct poe2d_solver {
     int cur = 0;
     void *arena = nullptr;  // state groups
     void *psi = nullptr;
+    void *psi_mid = nullptr;  // memory variables of the intermediate step (two-step kernel), on first use
     float *red = nullptr;   // reduction scratch: [0] result, [1..] per-block partials
     int red_blocks = 1024;
     void *dense = nullptr;  // (nz+1) x (nx+1) scratch for snapshots, made on first use
@@ -221,7 +224,8 @@ struct poe2d_solver {
     int step = 0;
     double last_ms = 0;
     int last_steps = 0;
-    StepList list;
+    StepList list;   // single-step launches
+    StepList list2;  // two-step launches
 
     size_t group_bytes() const { return m->group_bytes(); }
     size_t trace_bytes() const { return (size_t)nsamp * ncomp * nrec * rs; }
@@ -262,8 +266,10 @@ poe2d_status check_handle(poe2d_solver *s)
 void free_handle(poe2d_solver *s)
 {
     s->list.release();
+    s->list2.release();
     dev_free(s->arena);
     dev_free(s->psi);
+    dev_free(s->psi_mid);
     void *q = s->red; dev_free(q); s->red = nullptr;
     q = s->rix; dev_free(q); s->rix = nullptr;
     q = s->riz; dev_free(q); s->riz = nullptr;
@@ -404,9 +410,9 @@ std::vector<int> strip_map(const std::vector<double> *t, int n, int *count, int 
 }
 
 // Arguments of the step kernels for handle s, reading state buffer `in` and writing `out`, with
-// tiles of bw - 8 columns by th rows.
+// tiles of bw - 8 columns (bw - 12 for the two-step kernel) by th rows.
 template <class T>
-StepArgs<T> step_args(const poe2d_solver *s, int in, int out, int bw, int th)
+StepArgs<T> step_args(const poe2d_solver *s, int in, int out, int bw, int th, bool two = false)
 {
     StepArgs<T> A{};
     const Model *md = s->m;
@@ -447,7 +453,11 @@ StepArgs<T> step_args(const poe2d_solver *s, int in, int out, int bw, int th)
     // step n = s->step reads state buffer cur
     A.par = (int)(((long long)s->cur - s->step) & 1);
     A.th = th;
-    A.ntx = (g.nx + 1 + fused_tile_width(bw) - 1) / fused_tile_width(bw);
+    const int W = two ? fused2_tile_width(bw) : fused_tile_width(bw);
+    A.ntx = (g.nx + 1 + W - 1) / W;
+    A.psi_mid = (T *)s->psi_mid;
+    A.pair0 = s->step;
+    A.pcur = s->cur;
     A.ntiles = A.ntx * ((g.nz + 1 + th - 1) / th);
     A.per_xcd = (A.ntiles + 7) / 8;
     return A;
@@ -480,7 +490,7 @@ poe2d_status upload_source(poe2d_solver *s, long long n0, int k)
 // uploads wait for `stream` first.
 template <class T>
 poe2d_status build_list(StepList &L, const std::vector<poe2d_solver *> &members, int bw, int th, bool palette,
-                        hipStream_t stream)
+                        hipStream_t stream, bool two = false, int par_override = -1)
 {
     const int nm = (int)members.size();
     const int material = members[0]->grid.staggered_material;
@@ -492,10 +502,11 @@ poe2d_status build_list(StepList &L, const std::vector<poe2d_solver *> &members,
     std::vector<int> ptr;
     std::vector<int2> ent;
     std::vector<size_t> ptr_off(nm), ent_off(nm);
-    const int W = fused_tile_width(bw);
+    const int W = two ? fused2_tile_width(bw) : fused_tile_width(bw);
     for (int i = 0; i < nm; ++i) {
         const poe2d_solver *s = members[i];
-        args[i] = step_args<T>(s, 0, 1, bw, th);
+        args[i] = step_args<T>(s, 0, 1, bw, th, two);
+        if (par_override >= 0) args[i].par = par_override;
         if ((long long)work.size() + args[i].ntiles > (1LL << 30))
             return fail(POE2D_E_INVALID, "run: too many tiles in one launch");
         for (int t = 0; t < args[i].ntiles; ++t) work.push_back(WorkItem{i, t});
@@ -561,14 +572,27 @@ poe2d_status build_list(StepList &L, const std::vector<poe2d_solver *> &members,
     L.bw = bw;
     L.palette = palette;
     L.avg = material == POE2D_MATERIAL_AVERAGED;
+    L.two = two;
     L.precision = sizeof(T) == 4 ? POE2D_F32 : POE2D_F64;
     return POE2D_OK;
 }
 
-// Launches k = 0 .. count-1 of list L on `stream` (steps *base + k), then *base += count.
+// Launches k = 0 .. count-1 of list L on `stream` (steps *base + k, or the step pairs *base + 2k
+// for the two-step kernel), then advances *base past them.
 template <class T>
 hipError_t launch_list_chunk(const StepList &L, int count, hipStream_t stream)
 {
+    if (L.two) {
+        if constexpr (sizeof(T) == 4) {
+            for (int k = 0; k < count; ++k) {
+                hipError_t e = launch_fused2<T>((const StepArgs<T> *)L.models, L.work, L.nwork, L.per_xcd, L.bw,
+                                                L.avg, L.base, 2 * k, stream);
+                if (e != hipSuccess) return e;
+            }
+            return launch_base_advance(L.base, 2 * count, stream);
+        }
+        return hipErrorInvalidValue;
+    }
     for (int k = 0; k < count; ++k) {
         hipError_t e = launch_fused<T>((const StepArgs<T> *)L.models, L.work, L.nwork, L.per_xcd, L.bw, L.palette,
                                        L.avg, L.base, k, stream);
@@ -577,22 +601,23 @@ hipError_t launch_list_chunk(const StepList &L, int count, hipStream_t stream)
     return launch_base_advance(L.base, count, stream);
 }
 
-// k steps of list L from step n0 on `stream`: graph replays of kGraphSteps steps, the remainder
-// launched directly.
+// k launches of list L from step n0 on `stream` (k steps, or k step pairs): graph replays of
+// kGraphSteps steps, the remainder launched directly.
 template <class T>
 poe2d_status run_list(StepList &L, long long n0, int k, bool graphs, hipStream_t stream)
 {
     *L.h_base = n0;
     HIP_TRY(hipMemcpyAsync(L.base, L.h_base, sizeof(long long), hipMemcpyHostToDevice, stream));
     int done = 0;
-    if (graphs && k >= kGraphSteps) {
+    const int chunk = L.two ? kGraphSteps / 2 : kGraphSteps;
+    if (graphs && k >= chunk) {
         const StepList::Sig sig = L.signature();
         if (!L.exec || !(L.sig == sig)) {
             if (L.exec) (void)hipGraphExecDestroy(L.exec);
             L.exec = nullptr;
             hipGraph_t graph = nullptr;
             HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-            const hipError_t e = launch_list_chunk<T>(L, kGraphSteps, stream);
+            const hipError_t e = launch_list_chunk<T>(L, chunk, stream);
             const hipError_t e2 = hipStreamEndCapture(stream, &graph);
             if (e != hipSuccess || e2 != hipSuccess) {
                 if (graph) (void)hipGraphDestroy(graph);
@@ -607,7 +632,7 @@ poe2d_status run_list(StepList &L, long long n0, int k, bool graphs, hipStream_t
             }
             L.sig = sig;
         }
-        for (; done + kGraphSteps <= k; done += kGraphSteps) HIP_TRY(hipGraphLaunch(L.exec, stream));
+        for (; done + chunk <= k; done += chunk) HIP_TRY(hipGraphLaunch(L.exec, stream));
     }
     if (done < k) HIP_TRY(launch_list_chunk<T>(L, k - done, stream));
     return POE2D_OK;
@@ -648,9 +673,25 @@ poe2d_status run_steps(poe2d_solver *s, int nsteps)
         }
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(s->ev1, s->stream));
+    } else if (sizeof(T) == 4 && s->schedule == SCHED_FUSED2 && s->m->npal > 0) {
+        // step pairs on the two-step kernel, an odd last step on the single-step kernel
+        const int pairs = nsteps / 2, rem = nsteps & 1;
+        if (!s->psi_mid && (st = dev_alloc(&s->psi_mid, std::max<size_t>(s->m->psi_half_elems(), 1) * s->rs)))
+            return st;
+        if ((st = upload_source<T>(s, s->step, nsteps))) return st;
+        if (pairs && (st = build_list<T>(s->list2, {s}, s->block_w, s->tile_rows, true, s->stream, true))) return st;
+        // the single step at n = step + 2 pairs reads buffer cur + pairs: (n + par) & 1 = (cur + pairs) & 1
+        if (rem && (st = build_list<T>(s->list, {s}, s->block_w, s->tile_rows, true, s->stream, false,
+                                       (int)(((long long)s->cur - s->step - pairs) & 1))))
+            return st;
+        HIP_TRY(hipEventRecord(s->ev0, s->stream));
+        if (pairs && (st = run_list<T>(s->list2, s->step, pairs, s->graphs, s->stream))) return st;
+        if (rem && (st = run_list<T>(s->list, s->step + 2LL * pairs, 1, s->graphs, s->stream))) return st;
+        HIP_TRY(hipEventRecord(s->ev1, s->stream));
+        s->cur = (s->cur + pairs + rem) & 1;
     } else {
         if ((st = upload_source<T>(s, s->step, nsteps))) return st;
-        const bool palette = s->schedule == SCHED_FUSED && s->m->npal > 0;
+        const bool palette = (s->schedule == SCHED_FUSED || s->schedule == SCHED_FUSED2) && s->m->npal > 0;
         if ((st = build_list<T>(s->list, {s}, s->block_w, s->tile_rows, palette, s->stream))) return st;
         HIP_TRY(hipEventRecord(s->ev0, s->stream));
         if ((st = run_list<T>(s->list, s->step, nsteps, s->graphs, s->stream))) return st;
@@ -763,7 +804,7 @@ poe2d_status batch_run_steps(poe2d_batch *b, int nsteps)
     for (poe2d_solver *s : b->members) {
         poe2d_status st = upload_source<T>(s, n0, nsteps);
         if (st) return st;
-        palette = palette && s->schedule == SCHED_FUSED && s->m->npal > 0;
+        palette = palette && (s->schedule == SCHED_FUSED || s->schedule == SCHED_FUSED2) && s->m->npal > 0;
     }
     poe2d_status st = build_list<T>(b->list, b->members, b->block_w, b->tile_rows, palette, b->stream);
     if (st) return st;
@@ -917,11 +958,14 @@ poe2d_status poe2d_set_schedule(poe2d_solver *s, int schedule, int block_w, int 
 {
     poe2d_status st = check_handle(s);
     if (st) return st;
-    if (schedule != SCHED_FUSED && schedule != SCHED_TWOPASS && schedule != SCHED_FUSED_GROUPS)
-        return fail(POE2D_E_INVALID, "set_schedule: schedule must be 0 (single-pass), 1 (two-pass) or 2 "
-                                     "(single-pass on the coefficient groups)");
+    if (schedule != SCHED_FUSED && schedule != SCHED_TWOPASS && schedule != SCHED_FUSED_GROUPS &&
+        schedule != SCHED_FUSED2)
+        return fail(POE2D_E_INVALID, "set_schedule: schedule must be 0 (single-pass), 1 (two-pass), 2 "
+                                     "(single-pass on the coefficient groups) or 3 (two steps per launch)");
     if (block_w != 128 && block_w != 256 && block_w != 512)
         return fail(POE2D_E_INVALID, "set_schedule: block width must be 128, 256 or 512");
+    if (schedule == SCHED_FUSED2 && block_w == 512)
+        return fail(POE2D_E_INVALID, "set_schedule: the two-step schedule takes block widths 128 or 256");
     if (tile_rows < 4 || tile_rows > 1 << 20) return fail(POE2D_E_INVALID, "set_schedule: tile rows out of range");
     s->schedule = schedule;
     s->block_w = block_w;
@@ -1146,9 +1190,12 @@ poe2d_status poe2d_autotune(poe2d_solver *s, int nsteps, int *block_w, int *tile
     if (s->schedule == SCHED_TWOPASS) return fail(POE2D_E_INVALID, "autotune: the two-pass schedule has no tile shape");
     if (s->step != 0) return fail(POE2D_E_STATE, "autotune: needs a handle at step 0 (call poe2d_reset first)");
     if (nsteps <= 0) nsteps = 10;
-    // Tile heights of 6k + 2 rows make whole groups of the kernel's 6-step row loop.
-    static const int widths[] = {128, 256, 512};
-    static const int heights[] = {8, 14, 20, 26, 32, 44, 62, 98};
+    // Tile heights of 6k + 2 rows (6k for the two-step kernel) make whole groups of the kernels'
+    // 6-step row loops.
+    const bool two = s->schedule == SCHED_FUSED2;
+    const std::vector<int> widths = two ? std::vector<int>{128, 256} : std::vector<int>{128, 256, 512};
+    const std::vector<int> heights = two ? std::vector<int>{6, 12, 18, 24, 30, 42, 60}
+                                         : std::vector<int>{8, 14, 20, 26, 32, 44, 62, 98};
     double best = 0;
     int best_w = s->block_w, best_h = s->tile_rows;
     for (int w : widths)

@@ -122,6 +122,12 @@ struct StepArgs {
     T *traces;
     int rec_n, rec_ncomp, rec_dec, rec_nsamp;
     unsigned rec_mask;
+    // the two-step kernel: memory variables of the intermediate step (x strips, then z strips,
+    // one record set), and the buffer rule: the launch at step n reads buffer
+    // ((n - pair0) / 2 + pcur) & 1 and writes the other
+    T *psi_mid;
+    long long pair0;
+    int pcur;
 };
 
 // Step-dependent fields of a work-list launch at step n.
@@ -260,5 +266,12 @@ hipError_t launch_base_advance(long long *base, int delta, hipStream_t stream);
 
 // Output columns of one tile for block width bw (halo of 4 columns on each side).
 inline int fused_tile_width(int bw) { return bw - 8; }
+
+// The two-step kernel (FP32, palette grids): output columns per tile (halo of 6 columns on each
+// side), and the launch; n = *base + k is the first of the two steps.
+__host__ __device__ constexpr int fused2_tile_width(int bw) { return bw - 12; }
+template <class T>
+hipError_t launch_fused2(const StepArgs<T> *models, const WorkItem *work, int nwork, int per_xcd, int bw, bool avg,
+                         const long long *base, int k, hipStream_t stream);
 
 }  // namespace poe2d
