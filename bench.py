@@ -6,15 +6,17 @@ with the inputs resident in HBM; set-up, uploads, the tile autotuning and the tr
 outside the timed region.  value = Gcell-updates/s = 4001^2 * 6000 * K / (device time of K solves,
 hipEvents on the solver's stream).
 
-Roofline: the dominant kernel is the single-pass step kernel, one launch per time step (it also
-records the receivers; the device step counter advances once per 32 steps).  Its algorithmic
-traffic is 104 B per point-update (9 state fields read + written, 8 coefficients read, FP32) plus
-48 B per C-PML strip point and 96 B per corner point; `achieved` divides that by the time per step
-of the timed solves (hipEvents on the solver's stream around the whole run, so launch gaps count
-against the kernel).  `traffic`
-is the measured HBM bytes per launch (rocprofv3 FETCH_SIZE and WRITE_SIZE, each scaled by what the
-same counter reads for a plain copy of known size), read from the committed profile summary when
-it was taken with the same tile shape, else null.
+Roofline: the dominant kernel is the step kernel: the single-pass kernel (one launch per time
+step) or the two-step kernel (one launch per two steps, the intermediate state on chip), whichever
+autotunes faster on the grid; either records the receivers, and the device step counter advances
+once per 32 steps.  The algorithmic traffic of a time step is 104 B per point-update (9 state
+fields read + written, 8 coefficients read, FP32) plus 48 B per C-PML strip point and 96 B per
+corner point; `achieved` divides that by the time per step of the timed solves (hipEvents on the
+solver's stream around the whole run, so launch gaps count against the kernel).  `traffic` is the
+measured HBM bytes per time step (rocprofv3 FETCH_SIZE and WRITE_SIZE per launch, each scaled by
+what the same counter reads for a plain copy of known size, halved for the two-step kernel), read
+from the committed profile summary when it was taken with the same kernel and tile shape, else
+null.
 
 The CPU baseline is the oracle in its reference-structured form (five sweeps plus three full-grid
 receiver scans per step, one thread) timed on a bounded sample of the same model.
@@ -349,11 +351,12 @@ def main():
         # autotune each schedule (one and two steps per launch), keep the faster
         best = None
         for sched in schedules:
+            solver.reset()
             solver.set_schedule(sched, 256, 12 if sched == 3 else 8)
             t = solver.autotune(10)
             solver.reset()
             solver.set_schedule(sched, *t)
-            solver.run(20)
+            solver.run(60)
             ms, n = solver.last_run_ms()
             print(f"[bench] schedule {sched}: autotuned {t[0]}x{t[1]}, {ms / n * 1e3:.1f} us per step",
                   file=sys.stderr, flush=True)

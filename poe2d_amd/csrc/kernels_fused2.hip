@@ -33,8 +33,10 @@
 
 namespace poe2d {
 
+// Three waves per SIMD: at most 168 VGPRs, and LDS for three blocks of 256 threads per CU (the
+// palette copy is cut to kMaxPalette2 entries).
 template <class T, int BW, bool AVG>
-__global__ void __launch_bounds__(BW)
+__global__ void __launch_bounds__(BW, 3)
     fused_step2(const StepArgs<T> *__restrict__ models, const WorkItem *__restrict__ work, int nwork, int per_xcd,
                 const long long *__restrict__ base, int k)
 {
@@ -46,7 +48,7 @@ __global__ void __launch_bounds__(BW)
     __shared__ T ls[3][3][BW + 4];   // ring of 3 stage-1 stress rows: sxx, p, sxy
     __shared__ T lv2[6][3][BW + 4];  // ring of 6 step-(n+1) velocity rows
     __shared__ T ls2[3][3][BW + 4];  // ring of 3 stage-2 stress rows
-    __shared__ Vec4<T> lpal[2 * kMaxPalette];  // MS entries, then MV entries
+    __shared__ Vec4<T> lpal[2 * kMaxPalette2];  // MS entries, then MV entries
 
     const int wi = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
     if (wi >= nwork) return;
@@ -110,11 +112,13 @@ __global__ void __launch_bounds__(BW)
     auto ldm = [&](int row) -> int { return mid[colm + (uint32_t)(min(row, last_row) + kPadRows) * (uint32_t)P]; };
     {
         const GN *const pal = (const GN *)A.pal;
-        for (int i = lane; i < 2 * kMaxPalette; i += BW)
-            if ((i & (kMaxPalette - 1)) < A.npal) {
-                const nvec4<T> e = pal[i];
-                lpal[i] = Vec4<T>{e.x, e.y, e.z, e.w};
+        for (int i = lane; i < 2 * kMaxPalette2; i += BW) {
+            const int e = i % kMaxPalette2;
+            if (e < A.npal) {
+                const nvec4<T> v = pal[(i < kMaxPalette2 ? 0 : kMaxPalette) + e];
+                lpal[i] = Vec4<T>{v.x, v.y, v.z, v.w};
             }
+        }
     }
 
     // per-column C-PML profile (x strips only)
@@ -272,8 +276,7 @@ __global__ void __launch_bounds__(BW)
     Vec4<T> vr[6];   // step-n velocities, rows s-2 .. s+3
     Vec4<T> sr[2];   // step-n strains, rows s, s+1
     int idr[6];      // material index, rows s-1 .. s+2
-    Vec4<T> msr[6];  // (alpha, M, lambda, mu), rows s-4 .. s+1
-    Vec4<T> mvr[6];  // (rho, rho_f, m, b), rows s-6 .. s-1
+    int idr2[6];     // material index, rows s-6 .. s-1 (the rows stage 2 and the v_x update read)
     T sxyr[6], syyr[6], pr[6];     // stage-1 stress rows s-4 .. s
     Vec4<T> v1r[6];  // step-(n+1) velocities, rows s-7 .. s-2
     Vec4<T> s1r[6];  // step-(n+1) strains, rows s-4 .. s
@@ -282,8 +285,8 @@ __global__ void __launch_bounds__(BW)
     for (int q = 0; q < 6; ++q) {
         sxyr[q] = syyr[q] = pr[q] = T(0);
         sxyr2[q] = syyr2[q] = pr2[q] = T(0);
-        idr[q] = 0;
-        msr[q] = mvr[q] = v1r[q] = s1r[q] = zero4;
+        idr[q] = idr2[q] = 0;
+        v1r[q] = s1r[q] = zero4;
     }
 #pragma unroll
     for (int q = -2; q <= 2; ++q) vr[(q + 6) % 6] = ld(Vo, s0 + q);
@@ -294,7 +297,6 @@ __global__ void __launch_bounds__(BW)
     idr[0] = ldm(s0);
     idr[1] = ldm(s0 + 1);
     __syncthreads();  // palette in LDS
-    msr[0] = lpal[idr[0]];
 
     int s = s0;
     auto row_step = [&](auto phase) __attribute__((always_inline)) {
@@ -305,13 +307,12 @@ __global__ void __launch_bounds__(BW)
         // ---- loads for later row steps and the material of this one
         V(3) = ld(Vo, s + 3);
         sr[(PH + 1) & 1] = ld(So, s + 1);
-        msr[(PH + 1) % 6] = lpal[idr[(PH + 1) % 6]];                // row s+1
-        mvr[(PH + 5) % 6] = lpal[kMaxPalette + idr[(PH + 5) % 6]];  // row s-1
+        // material through the palette in LDS, looked up where used: stage 1 reads MS of rows s,
+        // s+1 and MV of rows s-2, s-1; stage 2 MS of rows s-4, s-3 and MV of rows s-6, s-5
+        idr2[(PH + 5) % 6] = idr[(PH + 5) % 6];  // row s-1 moves to the older ring
         idr[(PH + 2) % 6] = ldm(s + 2);
-        const Vec4<T> &ms_cur = msr[PH % 6], &ms_nxt = msr[(PH + 1) % 6];
-        const Vec4<T> &mv_j = mvr[(PH + 4) % 6], &mv_j1 = mvr[(PH + 5) % 6];
-        const Vec4<T> &ms2_cur = msr[(PH + 2) % 6], &ms2_nxt = msr[(PH + 3) % 6];
-        const Vec4<T> &mv2_j = mvr[PH % 6], &mv2_j1 = mvr[(PH + 1) % 6];
+        const Vec4<T> ms_cur = lpal[idr[PH % 6]], ms_nxt = lpal[idr[(PH + 1) % 6]];
+        const Vec4<T> mv_j = lpal[kMaxPalette2 + idr2[(PH + 4) % 6]], mv_j1 = lpal[kMaxPalette2 + idr[(PH + 5) % 6]];
 
         // ---- x neighbours through LDS: step-n velocities of row s, step-(n+1) velocities of row s-3
         {
@@ -348,18 +349,26 @@ __global__ void __launch_bounds__(BW)
         auto SYY = [&](int q) -> T { return syyr[(PH + q + 12) % 6]; };
         auto PP = [&](int q) -> T { return pr[(PH + q + 12) % 6]; };
 
-        // ---- stage 1: velocities of row s-2 (step n -> n+1)
-        V1(-2) = velocity(s - 2, V(-2), ls[(PH + 1) % 3], SXY(-4), SXY(-3), SXY(-2), SXY(-1), SYY(-3), SYY(-2),
-                          SYY(-1), SYY(0), PP(-3), PP(-2), PP(-1), PP(0), mv_j, mv_j1, px_in, pxm, pz_in, pzm, pv1,
-                          src1);
+        // ---- stage 1: velocities of row s-2 (step n -> n+1); stage 2 reads rows [zA-4, zB+3]
+        if (s - 2 >= zA - 4 && s - 2 <= zB + 3)
+            V1(-2) = velocity(s - 2, V(-2), ls[(PH + 1) % 3], SXY(-4), SXY(-3), SXY(-2), SXY(-1), SYY(-3), SYY(-2),
+                              SYY(-1), SYY(0), PP(-3), PP(-2), PP(-1), PP(0), mv_j, mv_j1, px_in, pxm, pz_in, pzm,
+                              pv1, src1);
+        else
+            V1(-2) = V(-2);
 
         // ---- stage 2: stresses of row s-4 (step n+1)
         const int r2 = s - 4;
         const bool srow = own && r2 >= zA && r2 < zB;
-        T sxx2, p2, syy2;
-        const Vec4<T> s2 = stress(r2, V1(-6), V1(-5), V1(-4), V1(-3), V1(-2), s1r[(PH + 2) % 6], lv2[(PH + 2) % 6],
-                                  ms2_cur, ms2_nxt, pxm, px_out, pzm, pz_out, srow, src2, sxx2, p2, syy2);
-        if (srow) *(GN *)(Sn + off(r2)) = nvec4<T>{s2.x, s2.y, s2.z, s2.w};
+        // stage 2's velocities read the stresses of rows [zA-2, zB+1]; other rows are skipped
+        T sxx2 = 0, p2 = 0, syy2 = 0;
+        Vec4<T> s2 = zero4;
+        if (r2 >= zA - 2 && r2 <= zB + 1) {
+            s2 = stress(r2, V1(-6), V1(-5), V1(-4), V1(-3), V1(-2), s1r[(PH + 2) % 6], lv2[(PH + 2) % 6],
+                        lpal[idr2[(PH + 2) % 6]], lpal[idr2[(PH + 3) % 6]], pxm, px_out, pzm, pz_out, srow, src2, sxx2,
+                        p2, syy2);
+            if (srow) *(GN *)(Sn + off(r2)) = nvec4<T>{s2.x, s2.y, s2.z, s2.w};
+        }
         {
             T(*xs)[BW + 4] = ls2[(PH + 2) % 3];
             xs[0][lane + 2] = sxx2;
@@ -375,11 +384,14 @@ __global__ void __launch_bounds__(BW)
 
         // ---- stage 2: velocities of row s-6 (step n+1 -> n+2)
         const int j2 = s - 6;
-        const bool vrow = own && j2 >= zA && j2 < zB;
-        const Vec4<T> v2 = velocity(j2, V1(-6), ls2[PH % 3], SXY2(-8), SXY2(-7), SXY2(-6), SXY2(-5), SYY2(-7),
-                                    SYY2(-6), SYY2(-5), SYY2(-4), PP2(-7), PP2(-6), PP2(-5), PP2(-4), mv2_j, mv2_j1,
-                                    pxm, px_out, pzm, pz_out, vrow, src2);
-        if (vrow) *(GN *)(Vn + off(j2)) = nvec4<T>{v2.x, v2.y, v2.z, v2.w};
+        if (j2 >= zA && j2 < zB) {
+            const bool vrow = own;
+            const Vec4<T> v2 = velocity(j2, V1(-6), ls2[PH % 3], SXY2(-8), SXY2(-7), SXY2(-6), SXY2(-5), SYY2(-7),
+                                        SYY2(-6), SYY2(-5), SYY2(-4), PP2(-7), PP2(-6), PP2(-5), PP2(-4),
+                                        lpal[kMaxPalette2 + idr2[PH % 6]], lpal[kMaxPalette2 + idr2[(PH + 1) % 6]],
+                                        pxm, px_out, pzm, pz_out, vrow, src2);
+            if (vrow) *(GN *)(Vn + off(j2)) = nvec4<T>{v2.x, v2.y, v2.z, v2.w};
+        }
         ++s;
     };
 

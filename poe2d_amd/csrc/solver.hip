@@ -673,7 +673,7 @@ poe2d_status run_steps(poe2d_solver *s, int nsteps)
         }
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(s->ev1, s->stream));
-    } else if (sizeof(T) == 4 && s->schedule == SCHED_FUSED2 && s->m->npal > 0) {
+    } else if (sizeof(T) == 4 && s->schedule == SCHED_FUSED2 && s->m->npal > 0 && s->m->npal <= kMaxPalette2) {
         // step pairs on the two-step kernel, an odd last step on the single-step kernel
         const int pairs = nsteps / 2, rem = nsteps & 1;
         if (!s->psi_mid && (st = dev_alloc(&s->psi_mid, std::max<size_t>(s->m->psi_half_elems(), 1) * s->rs)))

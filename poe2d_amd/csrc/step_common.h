@@ -270,6 +270,7 @@ inline int fused_tile_width(int bw) { return bw - 8; }
 // The two-step kernel (FP32, palette grids): output columns per tile (halo of 6 columns on each
 // side), and the launch; n = *base + k is the first of the two steps.
 __host__ __device__ constexpr int fused2_tile_width(int bw) { return bw - 12; }
+constexpr int kMaxPalette2 = 32;  // palette entries the two-step kernel keeps in LDS
 template <class T>
 hipError_t launch_fused2(const StepArgs<T> *models, const WorkItem *work, int nwork, int per_xcd, int bw, bool avg,
                          const long long *base, int k, hipStream_t stream);
