@@ -413,6 +413,7 @@ hipError_t launch_fused2_bw(const StepArgs<T> *models, const WorkItem *work, int
 {
     const dim3 grid(8 * per_xcd);
     switch (bw) {
+    case 64: fused_step2<T, 64, AVG><<<grid, 64, 0, stream>>>(models, work, nwork, per_xcd, base, k); break;
     case 128: fused_step2<T, 128, AVG><<<grid, 128, 0, stream>>>(models, work, nwork, per_xcd, base, k); break;
     case 256: fused_step2<T, 256, AVG><<<grid, 256, 0, stream>>>(models, work, nwork, per_xcd, base, k); break;
     default: return hipErrorInvalidValue;

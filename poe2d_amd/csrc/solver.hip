@@ -681,7 +681,8 @@ poe2d_status run_steps(poe2d_solver *s, int nsteps)
         if ((st = upload_source<T>(s, s->step, nsteps))) return st;
         if (pairs && (st = build_list<T>(s->list2, {s}, s->block_w, s->tile_rows, true, s->stream, true))) return st;
         // the single step at n = step + 2 pairs reads buffer cur + pairs: (n + par) & 1 = (cur + pairs) & 1
-        if (rem && (st = build_list<T>(s->list, {s}, s->block_w, s->tile_rows, true, s->stream, false,
+        // (the single-step kernel has no 64-thread blocks)
+        if (rem && (st = build_list<T>(s->list, {s}, std::max(s->block_w, 128), s->tile_rows, true, s->stream, false,
                                        (int)(((long long)s->cur - s->step - pairs) & 1))))
             return st;
         HIP_TRY(hipEventRecord(s->ev0, s->stream));
@@ -962,10 +963,10 @@ poe2d_status poe2d_set_schedule(poe2d_solver *s, int schedule, int block_w, int 
         schedule != SCHED_FUSED2)
         return fail(POE2D_E_INVALID, "set_schedule: schedule must be 0 (single-pass), 1 (two-pass), 2 "
                                      "(single-pass on the coefficient groups) or 3 (two steps per launch)");
-    if (block_w != 128 && block_w != 256 && block_w != 512)
-        return fail(POE2D_E_INVALID, "set_schedule: block width must be 128, 256 or 512");
-    if (schedule == SCHED_FUSED2 && block_w == 512)
-        return fail(POE2D_E_INVALID, "set_schedule: the two-step schedule takes block widths 128 or 256");
+    if (schedule == SCHED_FUSED2 ? (block_w != 64 && block_w != 128 && block_w != 256)
+                                 : (block_w != 128 && block_w != 256 && block_w != 512))
+        return fail(POE2D_E_INVALID, "set_schedule: block width must be 128, 256 or 512 (64, 128 or 256 for the "
+                                     "two-step schedule)");
     if (tile_rows < 4 || tile_rows > 1 << 20) return fail(POE2D_E_INVALID, "set_schedule: tile rows out of range");
     s->schedule = schedule;
     s->block_w = block_w;
@@ -1193,7 +1194,7 @@ poe2d_status poe2d_autotune(poe2d_solver *s, int nsteps, int *block_w, int *tile
     // Tile heights of 6k + 2 rows (6k for the two-step kernel) make whole groups of the kernels'
     // 6-step row loops.
     const bool two = s->schedule == SCHED_FUSED2;
-    const std::vector<int> widths = two ? std::vector<int>{128, 256} : std::vector<int>{128, 256, 512};
+    const std::vector<int> widths = two ? std::vector<int>{64, 128, 256} : std::vector<int>{128, 256, 512};
     const std::vector<int> heights = two ? std::vector<int>{6, 12, 18, 24, 30, 42, 60}
                                          : std::vector<int>{8, 14, 20, 26, 32, 44, 62, 98};
     double best = 0;
