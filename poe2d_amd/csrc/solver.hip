@@ -251,7 +251,8 @@ struct poe2d_batch {
     bool graphs = true;
     double last_ms = 0;
     int last_steps = 0;
-    StepList list;
+    StepList list;   // single-step launches
+    StepList list2;  // two-step launches
 };
 
 namespace {
@@ -490,7 +491,7 @@ poe2d_status upload_source(poe2d_solver *s, long long n0, int k)
 // uploads wait for `stream` first.
 template <class T>
 poe2d_status build_list(StepList &L, const std::vector<poe2d_solver *> &members, int bw, int th, bool palette,
-                        hipStream_t stream, bool two = false, int par_override = -1)
+                        hipStream_t stream, bool two = false, int par_shift = 0)
 {
     const int nm = (int)members.size();
     const int material = members[0]->grid.staggered_material;
@@ -506,7 +507,9 @@ poe2d_status build_list(StepList &L, const std::vector<poe2d_solver *> &members,
     for (int i = 0; i < nm; ++i) {
         const poe2d_solver *s = members[i];
         args[i] = step_args<T>(s, 0, 1, bw, th, two);
-        if (par_override >= 0) args[i].par = par_override;
+        // a single step after par_shift step pairs: step n = step + 2 par_shift reads buffer
+        // cur + par_shift, so (n + par) & 1 = (cur + par_shift) & 1
+        if (par_shift) args[i].par = (int)(((long long)s->cur - s->step - par_shift) & 1);
         if ((long long)work.size() + args[i].ntiles > (1LL << 30))
             return fail(POE2D_E_INVALID, "run: too many tiles in one launch");
         for (int t = 0; t < args[i].ntiles; ++t) work.push_back(WorkItem{i, t});
@@ -680,10 +683,9 @@ poe2d_status run_steps(poe2d_solver *s, int nsteps)
             return st;
         if ((st = upload_source<T>(s, s->step, nsteps))) return st;
         if (pairs && (st = build_list<T>(s->list2, {s}, s->block_w, s->tile_rows, true, s->stream, true))) return st;
-        // the single step at n = step + 2 pairs reads buffer cur + pairs: (n + par) & 1 = (cur + pairs) & 1
         // (the single-step kernel has no 64-thread blocks)
         if (rem && (st = build_list<T>(s->list, {s}, std::max(s->block_w, 128), s->tile_rows, true, s->stream, false,
-                                       (int)(((long long)s->cur - s->step - pairs) & 1))))
+                                       pairs)))
             return st;
         HIP_TRY(hipEventRecord(s->ev0, s->stream));
         if (pairs && (st = run_list<T>(s->list2, s->step, pairs, s->graphs, s->stream))) return st;
@@ -801,16 +803,30 @@ template <class T>
 poe2d_status batch_run_steps(poe2d_batch *b, int nsteps)
 {
     const long long n0 = b->members[0]->step;
-    bool palette = true;
+    bool palette = true, two = sizeof(T) == 4;
     for (poe2d_solver *s : b->members) {
         poe2d_status st = upload_source<T>(s, n0, nsteps);
         if (st) return st;
         palette = palette && (s->schedule == SCHED_FUSED || s->schedule == SCHED_FUSED2) && s->m->npal > 0;
+        two = two && s->schedule == SCHED_FUSED2 && s->m->npal > 0 && s->m->npal <= kMaxPalette2;
     }
-    poe2d_status st = build_list<T>(b->list, b->members, b->block_w, b->tile_rows, palette, b->stream);
-    if (st) return st;
+    if (two ? b->block_w == 512 : b->block_w == 64)
+        return fail(POE2D_E_INVALID, two ? "batch_run: the two-step kernel takes block widths 64, 128 or 256"
+                                         : "batch_run: the single-step kernel takes block widths 128, 256 or 512");
+    const int pairs = two ? nsteps / 2 : 0, rem = two ? nsteps & 1 : nsteps;
+    poe2d_status st;
+    if (pairs) {
+        for (poe2d_solver *s : b->members)
+            if (!s->psi_mid && (st = dev_alloc(&s->psi_mid, std::max<size_t>(s->m->psi_half_elems(), 1) * s->rs)))
+                return st;
+        if ((st = build_list<T>(b->list2, b->members, b->block_w, b->tile_rows, true, b->stream, true))) return st;
+    }
+    if (rem && (st = build_list<T>(b->list, b->members, std::max(b->block_w, 128), b->tile_rows, palette, b->stream,
+                                   false, pairs)))
+        return st;
     HIP_TRY(hipEventRecord(b->ev0, b->stream));
-    if ((st = run_list<T>(b->list, n0, nsteps, b->graphs, b->stream))) return st;
+    if (pairs && (st = run_list<T>(b->list2, n0, pairs, b->graphs, b->stream))) return st;
+    if (rem && (st = run_list<T>(b->list, n0 + 2LL * pairs, rem, b->graphs, b->stream))) return st;
     HIP_TRY(hipEventRecord(b->ev1, b->stream));
     HIP_TRY(hipEventSynchronize(b->ev1));
     float ms = 0.f;
@@ -819,7 +835,7 @@ poe2d_status batch_run_steps(poe2d_batch *b, int nsteps)
     b->last_steps = nsteps;
     poe2d_status first = POE2D_OK;
     for (poe2d_solver *s : b->members) {
-        s->cur = (s->cur + nsteps) & 1;
+        s->cur = (s->cur + pairs + rem) & 1;
         s->step += nsteps;
         s->last_ms = ms;
         s->last_steps = nsteps;
@@ -1267,8 +1283,9 @@ poe2d_status poe2d_batch_set_schedule(poe2d_batch *b, int block_w, int tile_rows
 {
     poe2d_status st = check_batch(b);
     if (st) return st;
-    if (block_w != 128 && block_w != 256 && block_w != 512)
-        return fail(POE2D_E_INVALID, "batch_set_schedule: block width must be 128, 256 or 512");
+    if (block_w != 64 && block_w != 128 && block_w != 256 && block_w != 512)
+        return fail(POE2D_E_INVALID, "batch_set_schedule: block width must be 64 (two-step only), 128, 256 or 512 "
+                                     "(single-step only)");
     if (tile_rows < 4 || tile_rows > 1 << 20) return fail(POE2D_E_INVALID, "batch_set_schedule: tile rows out of range");
     b->block_w = block_w;
     b->tile_rows = tile_rows;
@@ -1313,8 +1330,12 @@ poe2d_status poe2d_batch_autotune(poe2d_batch *b, int nsteps, int *block_w, int 
     for (poe2d_solver *s : b->members)
         if (s->step != 0) return fail(POE2D_E_STATE, "batch_autotune: needs every member at step 0");
     if (nsteps <= 0) nsteps = 10;
-    static const int widths[] = {128, 256, 512};
-    static const int heights[] = {8, 14, 20, 26, 32, 44, 62, 98};
+    bool two = b->precision == POE2D_F32;
+    for (poe2d_solver *s : b->members)
+        two = two && s->schedule == SCHED_FUSED2 && s->m->npal > 0 && s->m->npal <= kMaxPalette2;
+    const std::vector<int> widths = two ? std::vector<int>{64, 128, 256} : std::vector<int>{128, 256, 512};
+    const std::vector<int> heights = two ? std::vector<int>{6, 12, 18, 24, 30, 42, 60}
+                                         : std::vector<int>{8, 14, 20, 26, 32, 44, 62, 98};
     int nzmax = 0;
     for (poe2d_solver *s : b->members) nzmax = std::max(nzmax, s->grid.nz);
     double best = 0;
@@ -1351,6 +1372,7 @@ void poe2d_batch_destroy(poe2d_batch *b)
     (void)hipSetDevice(b->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     b->list.release();
+    b->list2.release();
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     if (b->stream) (void)hipStreamDestroy(b->stream);

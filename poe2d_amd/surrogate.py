@@ -103,6 +103,9 @@ def receiver_points(geom=CONFIG3):
     return ix, np.full(geom["nrec"], c + geom["rec_dz"], np.int32)
 
 
+TILE = (128, 24)  # block width and tile rows of the batched two-step kernel
+
+
 def gather_shape(geom=CONFIG3):
     return (3, geom["nrec"], -(-geom["nt"] // geom["dec"]))
 
@@ -140,7 +143,10 @@ def simulate(params, seeds=None, geom=CONFIG3, batch=64, workers=8):
             solvers = [Solver(p) for p in problems]
             del problems
             try:
+                for s in solvers:
+                    s.set_schedule(3, *TILE)  # two steps per launch (bitwise equal to one)
                 with Batch(solvers) as b:
+                    b.set_schedule(*TILE)
                     b.run(geom["nt"])
                     gpu_ms += b.last_run_ms()[0]
                 for k, s in enumerate(solvers):

@@ -140,3 +140,30 @@ def test_autotune_two_step():
         s.run()
         g = s.seismograms()
     assert np.array_equal(g, _run(prob, 0, 256, 14)[0])
+
+
+def test_batch_two_step_equals_members_alone():
+    """A batch of ragged grids and a clone on the two-step kernel equals each member run alone on the
+    single-step kernel, bit for bit (odd step count: pairs, then one single step)."""
+    from poe2d_amd.solver import Batch
+    probs = [_problem(n=200, nt=91, seed=3, src=(100, 90)), _problem(n=260, nt=91, seed=4, src=(130, 120))]
+    alone = []
+    for p in probs:
+        alone.append(_run(p, 0, 256, 14))
+    with Solver(probs[0]) as a, Solver(probs[1]) as b:
+        c = a.clone(source=SourceSpec(kind=_abi.SRC_FORCE_Y_RICKER, ix=60, iz=150, amp=1e10),
+                    receivers=probs[0].receivers)
+        try:
+            for s in (a, b, c):
+                s.set_schedule(3, 256, 12)
+            with Batch([a, b, c]) as batch:
+                batch.set_schedule(128, 18)
+                batch.run(91)
+            _same(_state(a), alone[0], "batch member 0")
+            _same(_state(b), alone[1], "batch member 1")
+            with Solver(probs[0]) as ref:
+                ref.set_source(SourceSpec(kind=_abi.SRC_FORCE_Y_RICKER, ix=60, iz=150, amp=1e10))
+                ref.run(91)
+                _same(_state(c), _state(ref), "batch clone")
+        finally:
+            c.close()
