@@ -212,11 +212,14 @@ poe2d_status poe2d_restore(poe2d_solver *s, const void *blob, size_t bytes);
  * (:387-422, called at :1736). */
 poe2d_status poe2d_reset(poe2d_solver *s);
 
-/* Pick the fastest tile shape of the single-pass schedule for this handle's grid: times
- * nsteps steps (after one warm-up step) for each block width in {128, 256, 512} and tile height
- * in {8, 14, 20, 26, 32, 44, 62, 98}, keeps the fastest and resets the handle.  Needs step
- * count 0 (POE2D_E_STATE otherwise); nsteps <= 0 means 10.  The shape changes speed only, never
- * a result.  No counterpart in the reference, whose launch shape is fixed
+/* Pick the fastest tile shape of the handle's schedule for its grid: times nsteps steps (after
+ * one warm-up step) for each block width in {128, 256, 512} and tile height in {8, 14, 20, 26,
+ * 32, 44, 62, 98} (two-step schedule: widths {64, 128, 256}, heights {6, 12, ..., 60}), keeps the
+ * fastest and resets the handle.  The result is cached for the process per grid class (device,
+ * precision, schedule, grid size, palette): a later call for the same class applies the cached
+ * shape without timing.  nsteps = 0 means 10; nsteps < 0 times -nsteps steps and ignores the
+ * cache.  Needs step count 0 (POE2D_E_STATE otherwise).  The shape changes speed only, never a
+ * result.  No counterpart in the reference, whose launch shape is fixed
  * (seismic_CPML_2D_poroelastic_withCUDA.cu:1620-1634). */
 poe2d_status poe2d_autotune(poe2d_solver *s, int nsteps, int *block_w, int *tile_rows);
 
@@ -245,7 +248,8 @@ poe2d_status poe2d_batch_set_schedule(poe2d_batch *b, int block_w, int tile_rows
 poe2d_status poe2d_batch_set_graphs(poe2d_batch *b, int enable);
 poe2d_status poe2d_batch_run(poe2d_batch *b, int nsteps);
 poe2d_status poe2d_batch_last_run_ms(poe2d_batch *b, double *ms, int *nsteps);
-/* As poe2d_autotune, over the whole batch (every member at step 0; all are reset after). */
+/* As poe2d_autotune, over the whole batch (every member at step 0; all are reset after), cached
+ * per batch class (member grid sizes, schedules, precision). */
 poe2d_status poe2d_batch_autotune(poe2d_batch *b, int nsteps, int *block_w, int *tile_rows);
 void poe2d_batch_destroy(poe2d_batch *b);
 

@@ -28,6 +28,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -1200,13 +1201,61 @@ poe2d_status poe2d_reset(poe2d_solver *s)
     return POE2D_OK;
 }
 
+namespace {
+
+// Tile shapes found by the autotuners, per grid class (device, precision, kernel, palette use and
+// the grid shapes), for the life of the process.
+std::mutex g_tune_mutex;
+std::map<std::string, std::pair<int, int>> g_tune_cache;
+
+std::string tune_key(const std::vector<poe2d_solver *> &members, int kernel)
+{
+    std::string k = std::to_string(members[0]->device) + "/" + std::to_string(members[0]->precision) + "/" +
+                    std::to_string(kernel);
+    std::vector<std::string> shapes;
+    for (const poe2d_solver *s : members)
+        shapes.push_back(std::to_string(s->grid.nx) + "x" + std::to_string(s->grid.nz) + (s->m->npal > 0 ? "p" : ""));
+    std::sort(shapes.begin(), shapes.end());
+    for (const std::string &q : shapes) k += "/" + q;
+    return k;
+}
+
+bool tune_lookup(const std::string &key, int *w, int *h)
+{
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    auto it = g_tune_cache.find(key);
+    if (it == g_tune_cache.end()) return false;
+    *w = it->second.first;
+    *h = it->second.second;
+    return true;
+}
+
+void tune_store(const std::string &key, int w, int h)
+{
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    g_tune_cache[key] = {w, h};
+}
+
+}  // namespace
+
 poe2d_status poe2d_autotune(poe2d_solver *s, int nsteps, int *block_w, int *tile_rows)
 {
     poe2d_status st = check_handle(s);
     if (st) return st;
     if (s->schedule == SCHED_TWOPASS) return fail(POE2D_E_INVALID, "autotune: the two-pass schedule has no tile shape");
     if (s->step != 0) return fail(POE2D_E_STATE, "autotune: needs a handle at step 0 (call poe2d_reset first)");
-    if (nsteps <= 0) nsteps = 10;
+    const std::string key = tune_key({s}, s->schedule);
+    if (nsteps >= 0) {
+        int w, h;
+        if (tune_lookup(key, &w, &h)) {
+            s->block_w = w;
+            s->tile_rows = h;
+            if (block_w) *block_w = w;
+            if (tile_rows) *tile_rows = h;
+            return POE2D_OK;
+        }
+    }
+    nsteps = nsteps == 0 ? 10 : std::abs(nsteps);
     // Tile heights of 6k + 2 rows (6k for the two-step kernel) make whole groups of the kernels'
     // 6-step row loops.
     const bool two = s->schedule == SCHED_FUSED2;
@@ -1232,6 +1281,7 @@ poe2d_status poe2d_autotune(poe2d_solver *s, int nsteps, int *block_w, int *tile
     const poe2d_status rst = poe2d_reset(s);
     if (st) return st;
     if (rst) return rst;
+    tune_store(key, best_w, best_h);
     if (block_w) *block_w = best_w;
     if (tile_rows) *tile_rows = best_h;
     return POE2D_OK;
@@ -1329,7 +1379,20 @@ poe2d_status poe2d_batch_autotune(poe2d_batch *b, int nsteps, int *block_w, int 
     if (st) return st;
     for (poe2d_solver *s : b->members)
         if (s->step != 0) return fail(POE2D_E_STATE, "batch_autotune: needs every member at step 0");
-    if (nsteps <= 0) nsteps = 10;
+    int kinds = 0;
+    for (poe2d_solver *s : b->members) kinds |= 1 << s->schedule;
+    const std::string key = "batch/" + tune_key(b->members, kinds);
+    if (nsteps >= 0) {
+        int w, h;
+        if (tune_lookup(key, &w, &h)) {
+            b->block_w = w;
+            b->tile_rows = h;
+            if (block_w) *block_w = w;
+            if (tile_rows) *tile_rows = h;
+            return POE2D_OK;
+        }
+    }
+    nsteps = nsteps == 0 ? 10 : std::abs(nsteps);
     bool two = b->precision == POE2D_F32;
     for (poe2d_solver *s : b->members)
         two = two && s->schedule == SCHED_FUSED2 && s->m->npal > 0 && s->m->npal <= kMaxPalette2;
@@ -1361,6 +1424,7 @@ poe2d_status poe2d_batch_autotune(poe2d_batch *b, int nsteps, int *block_w, int 
     }
     if (st) return st;
     if (rst) return rst;
+    tune_store(key, best_w, best_h);
     if (block_w) *block_w = best_w;
     if (tile_rows) *tile_rows = best_h;
     return POE2D_OK;

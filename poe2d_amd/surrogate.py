@@ -103,7 +103,7 @@ def receiver_points(geom=CONFIG3):
     return ix, np.full(geom["nrec"], c + geom["rec_dz"], np.int32)
 
 
-TILE = (128, 24)  # block width and tile rows of the batched two-step kernel
+TILE = (128, 24)  # starting tile shape of the batched two-step kernel (then autotuned)
 
 
 def gather_shape(geom=CONFIG3):
@@ -147,6 +147,7 @@ def simulate(params, seeds=None, geom=CONFIG3, batch=64, workers=8):
                     s.set_schedule(3, *TILE)  # two steps per launch (bitwise equal to one)
                 with Batch(solvers) as b:
                     b.set_schedule(*TILE)
+                    b.autotune(4)  # timed once per batch class in the process, then cached
                     b.run(geom["nt"])
                     gpu_ms += b.last_run_ms()[0]
                 for k, s in enumerate(solvers):

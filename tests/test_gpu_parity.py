@@ -330,6 +330,18 @@ def test_autotune_picks_a_shape_and_resets():
         assert ei.value.status == _abi.POE2D_E_STATE
     manual = _run_sched(prob, 0, bw, th)[0]
     assert np.array_equal(tuned, manual)
+    # the shape is cached for this grid class: a second handle gets it without timing, and a
+    # negative step count tunes again
+    import time
+    with Solver(prob) as s2:
+        t0 = time.perf_counter()
+        assert s2.autotune(3) == (bw, th)
+        cached = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        bw2, th2 = s2.autotune(-3)
+        timed = time.perf_counter() - t0
+        say(f"autotune cached {cached * 1e3:.1f} ms, re-timed {timed * 1e3:.1f} ms -> {bw2}x{th2}")
+        assert cached < timed and s2.step_count() == 0
 
 
 @pytest.mark.parametrize("case", ["pressure_patchy", "force_patchy"])
