@@ -2,7 +2,7 @@
  * poe2d_oracle.cpp -- TEST INFRASTRUCTURE ONLY (the oracle).
  *
  * A CPU restatement of the reference's 2-D poroelastic (Biot) staggered-grid C-PML time loop,
- * used as the checker by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.  The
+ * the answer tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg compare the library with.  The
  * product (libpoe2d.so) neither links nor calls it.
  *
  * Followed files (reference, read as text):

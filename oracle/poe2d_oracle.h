@@ -1,7 +1,7 @@
 /*
  * poe2d_oracle.h -- TEST INFRASTRUCTURE ONLY.
  *
- * CPU restatement of the reference's poroelastic C-PML time loop, used as the checker by tests/,
+ * CPU restatement of the reference's poroelastic C-PML time loop, the answer compared with by tests/,
  * __graft_entry__.smoke() and bench.py's cpu_baseline leg.  The product library never links it.
  * It shares the public header's structs (types only) so a test can drive both behind one
  * interface.  Functions return 0 on success, a poe2d_status value otherwise.

@@ -1,6 +1,6 @@
 """TEST INFRASTRUCTURE ONLY: ctypes wrapper of the oracle (oracle/libpoe2d_oracle.so).
 
-Used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as the checker.  It
+Used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg, as the answer the library is compared with.  It
 drives the oracle through the same call set as poe2d_amd.solver.Solver drives the product.
 """
 import ctypes
