@@ -34,8 +34,11 @@
 namespace poe2d {
 
 // Three waves per SIMD: at most 168 VGPRs, and LDS for three blocks of 256 threads per CU (the
-// palette copy is cut to kMaxPalette2 entries).
-template <class T, int BW, bool AVG>
+// palette copy is cut to kMaxPalette2 entries).  EDGE = false: tiles none of whose points (halos
+// included) lie in a C-PML strip and that hold no receivers; the C-PML and receiver code is then
+// compiled out, which frees the registers the kernel needs for its common path (the work list puts
+// such tiles in a launch of their own).
+template <class T, int BW, bool AVG, bool EDGE>
 __global__ void __launch_bounds__(BW, 3)
     fused_step2(const StepArgs<T> *__restrict__ models, const WorkItem *__restrict__ work, int nwork, int per_xcd,
                 const long long *__restrict__ base, int k)
@@ -122,7 +125,7 @@ __global__ void __launch_bounds__(BW, 3)
     }
 
     // per-column C-PML profile (x strips only)
-    const int sx = (c >= 0 && c <= nx) ? ((gcint *)A.xmap)[c] : -1;
+    const int sx = (EDGE && c >= 0 && c <= nx) ? ((gcint *)A.xmap)[c] : -1;
     T ax = 0, bx = 1, rkx = 1, axh = 0, bxh = 1, rkxh = 1;
     if (sx >= 0) {
         ax = tab[TB_AX * A.tn + c]; bx = tab[TB_BX * A.tn + c]; rkx = tab[TB_RKX * A.tn + c];
@@ -137,7 +140,7 @@ __global__ void __launch_bounds__(BW, 3)
     const bool src_f = A.src_kind == POE2D_SRC_FORCE_Y_RICKER && c == A.src_x;
     // receivers of the two steps: slots and the tile's entries per row
     kcint *const rec_ptr = (kcint *)A.rec_tile_ptr + (long long)tx * (nz + 1);
-    const bool rec_tile = A.rec_n > 0 && rec_ptr[zA] < rec_ptr[zB];
+    const bool rec_tile = EDGE && A.rec_n > 0 && rec_ptr[zA] < rec_ptr[zB];
     const int n1 = (int)n, n2 = n1 + 1;
     const bool rec1 = rec_tile && n1 % A.rec_dec == 0 && n1 / A.rec_dec < A.rec_nsamp;
     const bool rec2 = rec_tile && n2 % A.rec_dec == 0 && n2 / A.rec_dec < A.rec_nsamp;
@@ -167,7 +170,7 @@ __global__ void __launch_bounds__(BW, 3)
                       T &p, T &syy) __attribute__((always_inline)) -> Vec4<T> {
         const bool zr1 = r >= 2 && r < nz - 1;
         const bool zr2 = r >= 1 && r < nz - 1;
-        const int szs = zstrip(A, r);
+        const int szs = EDGE ? zstrip(A, r) : -1;
         T exx_n = so.x, eyy_n = so.y, zeta_n = so.z, sxy_n = so.w;
         p = 0;
         sxx = 0;
@@ -225,7 +228,7 @@ __global__ void __launch_bounds__(BW, 3)
                         T syy_0, T syy_p1, T syy_p2, T p_m1, T p_0, T p_p1, T p_p2, const Vec4<T> &mv_j,
                         const Vec4<T> &mv_j1, auto pxo, auto pxn_, auto pzo, auto pzn_, bool keep,
                         T srcv) __attribute__((always_inline)) -> Vec4<T> {
-        const int szj = zstrip(A, j);
+        const int szj = EDGE ? zstrip(A, j) : -1;
         if (xr4 && j >= 2 && j < nz) {
             T e1 = d4(xs[0][lane + 2], xs[0][lane + 1], xs[0][lane + 3], xs[0][lane], rdx);
             T e2 = d4(xs[1][lane + 2], xs[1][lane + 1], xs[1][lane + 3], xs[1][lane], rdx);
@@ -407,15 +410,15 @@ __global__ void __launch_bounds__(BW, 3)
     }
 }
 
-template <class T, bool AVG>
+template <class T, bool AVG, bool EDGE>
 hipError_t launch_fused2_bw(const StepArgs<T> *models, const WorkItem *work, int nwork, int per_xcd, int bw,
                             const long long *base, int k, hipStream_t stream)
 {
     const dim3 grid(8 * per_xcd);
     switch (bw) {
-    case 64: fused_step2<T, 64, AVG><<<grid, 64, 0, stream>>>(models, work, nwork, per_xcd, base, k); break;
-    case 128: fused_step2<T, 128, AVG><<<grid, 128, 0, stream>>>(models, work, nwork, per_xcd, base, k); break;
-    case 256: fused_step2<T, 256, AVG><<<grid, 256, 0, stream>>>(models, work, nwork, per_xcd, base, k); break;
+    case 64: fused_step2<T, 64, AVG, EDGE><<<grid, 64, 0, stream>>>(models, work, nwork, per_xcd, base, k); break;
+    case 128: fused_step2<T, 128, AVG, EDGE><<<grid, 128, 0, stream>>>(models, work, nwork, per_xcd, base, k); break;
+    case 256: fused_step2<T, 256, AVG, EDGE><<<grid, 256, 0, stream>>>(models, work, nwork, per_xcd, base, k); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -423,14 +426,17 @@ hipError_t launch_fused2_bw(const StepArgs<T> *models, const WorkItem *work, int
 
 template <class T>
 hipError_t launch_fused2(const StepArgs<T> *models, const WorkItem *work, int nwork, int per_xcd, int bw, bool avg,
-                         const long long *base, int k, hipStream_t stream)
+                         bool edge, const long long *base, int k, hipStream_t stream)
 {
     if (nwork < 1 || per_xcd < 1 || 8LL * per_xcd < nwork) return hipErrorInvalidValue;
-    return avg ? launch_fused2_bw<T, true>(models, work, nwork, per_xcd, bw, base, k, stream)
-               : launch_fused2_bw<T, false>(models, work, nwork, per_xcd, bw, base, k, stream);
+    if (edge)
+        return avg ? launch_fused2_bw<T, true, true>(models, work, nwork, per_xcd, bw, base, k, stream)
+                   : launch_fused2_bw<T, false, true>(models, work, nwork, per_xcd, bw, base, k, stream);
+    return avg ? launch_fused2_bw<T, true, false>(models, work, nwork, per_xcd, bw, base, k, stream)
+               : launch_fused2_bw<T, false, false>(models, work, nwork, per_xcd, bw, base, k, stream);
 }
 
-template hipError_t launch_fused2<float>(const StepArgs<float> *, const WorkItem *, int, int, int, bool,
+template hipError_t launch_fused2<float>(const StepArgs<float> *, const WorkItem *, int, int, int, bool, bool,
                                          const long long *, int, hipStream_t);
 
 }  // namespace poe2d

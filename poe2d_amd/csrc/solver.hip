@@ -90,6 +90,7 @@ struct Model {
     size_t pitch = 0, alloc_rows = 0;  // points per row, rows per group (with pads)
     int wxs = 0, wzs = 0;              // x-strip columns, z-strip rows
     int ztop = 0, zbot = 0;            // z strips: rows [0, ztop) and [zbot, nz]
+    int xlo = 0, xhi = 0;              // x strips: columns [0, xlo) and [xhi, nx]
     int tn = 0;                        // entries per profile table
     double cp_max = 0;
     void *mat = nullptr;               // MS, MV groups
@@ -137,15 +138,16 @@ struct StepList {
     bool palette = false;
     bool avg = false;  // POE2D_MATERIAL_AVERAGED (all members alike)
     bool two = false;  // two-step kernel: each launch advances two steps
+    int nint = 0;      // two-step kernel: items [0, nint) are interior tiles, the rest edge tiles
     int precision = POE2D_F32;
     // graph of kGraphSteps launches plus the base advance, and the launch it was captured for
     hipGraphExec_t exec = nullptr;
     struct Sig {
         void *models, *work, *base;
-        int nwork, per_xcd, bw, palette, avg, two, precision;
+        int nwork, nint, per_xcd, bw, palette, avg, two, precision;
         bool operator==(const Sig &o) const
         {
-            return models == o.models && work == o.work && base == o.base && nwork == o.nwork &&
+            return models == o.models && work == o.work && base == o.base && nwork == o.nwork && nint == o.nint &&
                    per_xcd == o.per_xcd && bw == o.bw && palette == o.palette && avg == o.avg && two == o.two &&
                    precision == o.precision;
         }
@@ -155,7 +157,7 @@ struct StepList {
     {
         Sig g{};
         g.models = models; g.work = work; g.base = base;
-        g.nwork = nwork; g.per_xcd = per_xcd; g.bw = bw; g.palette = palette; g.avg = avg; g.two = two;
+        g.nwork = nwork; g.nint = nint; g.per_xcd = per_xcd; g.bw = bw; g.palette = palette; g.avg = avg; g.two = two;
         g.precision = precision;
         return g;
     }
@@ -500,7 +502,7 @@ poe2d_status build_list(StepList &L, const std::vector<poe2d_solver *> &members,
         if (m->grid.staggered_material != material)
             return fail(POE2D_E_INVALID, "run: batch members differ in staggered material");
     std::vector<StepArgs<T>> args(nm);
-    std::vector<WorkItem> work;
+    std::vector<WorkItem> work, edge_work;
     std::vector<int> ptr;
     std::vector<int2> ent;
     std::vector<size_t> ptr_off(nm), ent_off(nm);
@@ -513,7 +515,8 @@ poe2d_status build_list(StepList &L, const std::vector<poe2d_solver *> &members,
         if (par_shift) args[i].par = (int)(((long long)s->cur - s->step - par_shift) & 1);
         if ((long long)work.size() + args[i].ntiles > (1LL << 30))
             return fail(POE2D_E_INVALID, "run: too many tiles in one launch");
-        for (int t = 0; t < args[i].ntiles; ++t) work.push_back(WorkItem{i, t});
+        if (!two)
+            for (int t = 0; t < args[i].ntiles; ++t) work.push_back(WorkItem{i, t});
         // receivers sorted by (tile column, row, column), with offsets per (tile column, row)
         const int ntx = args[i].ntx, nrow = s->grid.nz + 1;
         std::vector<int> order(s->nrec);
@@ -534,6 +537,23 @@ poe2d_status build_list(StepList &L, const std::vector<poe2d_solver *> &members,
         for (size_t q = 1; q < cnt.size(); ++q) cnt[q] += cnt[q - 1];
         ptr.insert(ptr.end(), cnt.begin(), cnt.end());
         for (int r : order) ent.push_back(int2{hx[r], r});
+        if (two) {
+            // interior tiles first: no C-PML strip point within the rows and columns the kernel
+            // computes for the tile, and no receivers in its rows
+            const Model *md = s->m;
+            const int nz = s->grid.nz;
+            for (int t = 0; t < args[i].ntiles; ++t) {
+                const int tx = t % ntx, tz = t / ntx;
+                const int zA = tz * th, zB = std::min(zA + th, nz + 1);
+                const int c0 = tx * W - kFused2ColsBefore, c1 = c0 + bw - 1;
+                const int r0 = zA - kFused2RowsBefore, r1 = zB + kFused2RowsAfter;
+                const bool xs = c0 < md->xlo || c1 >= md->xhi;
+                const bool zs = r0 < md->ztop || r1 >= md->zbot;
+                const size_t p0 = (size_t)tx * nrow + zA + 1, p1 = (size_t)tx * nrow + zB + 1;
+                const bool rec = s->nrec > 0 && cnt[p1 - 1] > cnt[p0 - 1];
+                (xs || zs || rec ? edge_work : work).push_back(WorkItem{i, t});
+            }
+        }
         args[i].rec_n = s->nrec;
         args[i].rec_ncomp = s->ncomp;
         args[i].rec_dec = s->dec;
@@ -541,6 +561,8 @@ poe2d_status build_list(StepList &L, const std::vector<poe2d_solver *> &members,
         args[i].rec_mask = s->mask;
         args[i].traces = (T *)s->traces;
     }
+    const int nint = (int)work.size();
+    work.insert(work.end(), edge_work.begin(), edge_work.end());
     poe2d_status st;
     if ((st = grow(&L.models, &L.cap_models, args.size() * sizeof(StepArgs<T>)))) return st;
     void *w = L.work;
@@ -571,6 +593,7 @@ poe2d_status build_list(StepList &L, const std::vector<poe2d_solver *> &members,
     HIP_TRY(hipMemcpy(L.rec_ptr, ptr.data(), ptr.size() * sizeof(int), hipMemcpyHostToDevice));
     if (!ent.empty()) HIP_TRY(hipMemcpy(L.rec_ent, ent.data(), ent.size() * sizeof(int2), hipMemcpyHostToDevice));
     L.nmodels = nm;
+    L.nint = two ? nint : (int)work.size();
     L.nwork = (int)work.size();
     L.per_xcd = (L.nwork + 7) / 8;
     L.bw = bw;
@@ -588,9 +611,16 @@ hipError_t launch_list_chunk(const StepList &L, int count, hipStream_t stream)
 {
     if (L.two) {
         if constexpr (sizeof(T) == 4) {
+            // interior tiles, then the tiles with C-PML strips or receivers, per step pair
+            const int nedge = L.nwork - L.nint;
             for (int k = 0; k < count; ++k) {
-                hipError_t e = launch_fused2<T>((const StepArgs<T> *)L.models, L.work, L.nwork, L.per_xcd, L.bw,
-                                                L.avg, L.base, 2 * k, stream);
+                hipError_t e = hipSuccess;
+                if (L.nint > 0)
+                    e = launch_fused2<T>((const StepArgs<T> *)L.models, L.work, L.nint, (L.nint + 7) / 8, L.bw, L.avg,
+                                         false, L.base, 2 * k, stream);
+                if (e == hipSuccess && nedge > 0)
+                    e = launch_fused2<T>((const StepArgs<T> *)L.models, L.work + L.nint, nedge, (nedge + 7) / 8, L.bw,
+                                         L.avg, true, L.base, 2 * k, stream);
                 if (e != hipSuccess) return e;
             }
             return launch_base_advance(L.base, 2 * count, stream);
@@ -923,8 +953,7 @@ poe2d_status poe2d_create(const poe2d_grid *g, const poe2d_coeffs *c, poe2d_solv
                                              t[3].data(), t[4].data(), t[5].data());
         if (st != POE2D_OK) return bail(st);
     }
-    int xlo = 0, xhi = 0;
-    std::vector<int> xm = strip_map(md->htab + TB_AX, g->nx + 1, &md->wxs, &xlo, &xhi);
+    std::vector<int> xm = strip_map(md->htab + TB_AX, g->nx + 1, &md->wxs, &md->xlo, &md->xhi);
     std::vector<int> zm = strip_map(md->htab + TB_AZ, g->nz + 1, &md->wzs, &md->ztop, &md->zbot);
     md->tn = std::max(g->nx, g->nz) + 1;
 

@@ -271,8 +271,13 @@ inline int fused_tile_width(int bw) { return bw - 8; }
 // side), and the launch; n = *base + k is the first of the two steps.
 __host__ __device__ constexpr int fused2_tile_width(int bw) { return bw - 12; }
 constexpr int kMaxPalette2 = 32;  // palette entries the two-step kernel keeps in LDS
+// edge: the items are tiles touching a C-PML strip or holding receivers (the others run a variant
+// without that code).
 template <class T>
 hipError_t launch_fused2(const StepArgs<T> *models, const WorkItem *work, int nwork, int per_xcd, int bw, bool avg,
-                         const long long *base, int k, hipStream_t stream);
+                         bool edge, const long long *base, int k, hipStream_t stream);
+// Rows and columns around a two-step tile whose points the kernel computes (halos of both steps and
+// the over-run of its row loop): a tile with a C-PML strip point in this range is an edge tile.
+constexpr int kFused2RowsBefore = 10, kFused2RowsAfter = 12, kFused2ColsBefore = 6;
 
 }  // namespace poe2d
