@@ -36,8 +36,9 @@ namespace poe2d {
 // Three waves per SIMD: at most 168 VGPRs, and LDS for three blocks of 256 threads per CU (the
 // palette copy is cut to kMaxPalette2 entries).  EDGE = false: tiles none of whose points (halos
 // included) lie in a C-PML strip and that hold no receivers; the C-PML and receiver code is then
-// compiled out, which frees the registers the kernel needs for its common path (the work list puts
-// such tiles in a launch of their own).
+// compiled out (the work list puts such tiles in a launch of their own), and the registers this
+// frees hold velocity loads further ahead.  Without the stage-2 receiver gather, the step-(n+1)
+// velocity rows need an LDS ring of 3 instead of 6.
 template <class T, int BW, bool AVG, bool EDGE>
 __global__ void __launch_bounds__(BW, 3)
     fused_step2(const StepArgs<T> *__restrict__ models, const WorkItem *__restrict__ work, int nwork, int per_xcd,
@@ -49,7 +50,8 @@ __global__ void __launch_bounds__(BW, 3)
     typedef __attribute__((address_space(1))) T GT;
     __shared__ T lv[3][3][BW + 4];   // ring of 3 step-n velocity rows: vx, wx, vy (entry lane + 2)
     __shared__ T ls[3][3][BW + 4];   // ring of 3 stage-1 stress rows: sxx, p, sxy
-    __shared__ T lv2[6][3][BW + 4];  // ring of 6 step-(n+1) velocity rows
+    constexpr int R2 = EDGE ? 6 : 3;
+    __shared__ T lv2[R2][3][BW + 4];  // ring of R2 step-(n+1) velocity rows
     __shared__ T ls2[3][3][BW + 4];  // ring of 3 stage-2 stress rows
     __shared__ Vec4<T> lpal[2 * kMaxPalette2];  // MS entries, then MV entries
 
@@ -276,9 +278,13 @@ __global__ void __launch_bounds__(BW, 3)
     // unrolled group row s + k is in slot (PH + k) mod R.
     const int s0 = zA - 6;
     const int send = zB + 6;
+    // The HBM loads run ahead: strains of row s+2 and material indices of row s+3 are requested at
+    // the start of row step s, step-n velocities of row s+4 once row s-2's last use in row step s
+    // is done (edge tiles) or of row s+5 at its start, two rows in flight in `pend` (interior).
     Vec4<T> vr[6];   // step-n velocities, rows s-2 .. s+3
-    Vec4<T> sr[2];   // step-n strains, rows s, s+1
-    int idr[6];      // material index, rows s-1 .. s+2
+    Vec4<T> pend[2]; // interior tiles: step-n velocities of rows s+3 (until moved to vr), s+4
+    Vec4<T> sr[3];   // step-n strains, rows s .. s+2
+    int idr[6];      // material index, rows s-1 .. s+3 (s-1 .. s+2 once row s-3 moved on)
     int idr2[6];     // material index, rows s-6 .. s-1 (the rows stage 2 and the v_x update read)
     T sxyr[6], syyr[6], pr[6];     // stage-1 stress rows s-4 .. s
     Vec4<T> v1r[6];  // step-(n+1) velocities, rows s-7 .. s-2
@@ -293,12 +299,19 @@ __global__ void __launch_bounds__(BW, 3)
     }
 #pragma unroll
     for (int q = -2; q <= 2; ++q) vr[(q + 6) % 6] = ld(Vo, s0 + q);
+    if constexpr (EDGE) {
+        vr[3] = ld(Vo, s0 + 3);
+    } else {
+        pend[0] = ld(Vo, s0 + 3);
+        pend[1] = ld(Vo, s0 + 4);
+    }
     sr[0] = ld(So, s0);
-    sr[1] = zero4;
-    vr[3] = zero4;
+    sr[1] = ld(So, s0 + 1);
+    sr[2] = zero4;
     idr[5] = ldm(s0 - 1);
     idr[0] = ldm(s0);
     idr[1] = ldm(s0 + 1);
+    idr[2] = ldm(s0 + 2);
     __syncthreads();  // palette in LDS
 
     int s = s0;
@@ -306,14 +319,17 @@ __global__ void __launch_bounds__(BW, 3)
         constexpr int PH = decltype(phase)::value;
         auto V = [&](int q) -> Vec4<T> & { return vr[(PH + q + 12) % 6]; };    // step-n velocity row s+q
         auto V1 = [&](int q) -> Vec4<T> & { return v1r[(PH + q + 12) % 6]; };  // step-(n+1) velocity row s+q
-        const Vec4<T> so = sr[PH & 1];
+        const Vec4<T> so = sr[PH % 3];
         // ---- loads for later row steps and the material of this one
-        V(3) = ld(Vo, s + 3);
-        sr[(PH + 1) & 1] = ld(So, s + 1);
+        if constexpr (!EDGE) {
+            V(3) = pend[PH % 2];  // into the slot of row s-3, dead since row step s-1
+            pend[PH % 2] = ld(Vo, s + 5);
+        }
+        sr[(PH + 2) % 3] = ld(So, s + 2);
         // material through the palette in LDS, looked up where used: stage 1 reads MS of rows s,
         // s+1 and MV of rows s-2, s-1; stage 2 MS of rows s-4, s-3 and MV of rows s-6, s-5
         idr2[(PH + 5) % 6] = idr[(PH + 5) % 6];  // row s-1 moves to the older ring
-        idr[(PH + 2) % 6] = ldm(s + 2);
+        idr[(PH + 3) % 6] = ldm(s + 3);           // into the slot of row s-3, moved on at row step s-2
         const Vec4<T> ms_cur = lpal[idr[PH % 6]], ms_nxt = lpal[idr[(PH + 1) % 6]];
         const Vec4<T> mv_j = lpal[kMaxPalette2 + idr2[(PH + 4) % 6]], mv_j1 = lpal[kMaxPalette2 + idr[(PH + 5) % 6]];
 
@@ -323,7 +339,7 @@ __global__ void __launch_bounds__(BW, 3)
             xv[0][lane + 2] = V(0).x;
             xv[1][lane + 2] = V(0).z;
             xv[2][lane + 2] = V(0).y;
-            T(*xv2)[BW + 4] = lv2[(PH + 3) % 6];
+            T(*xv2)[BW + 4] = lv2[(PH + 3) % R2];
             xv2[0][lane + 2] = V1(-3).x;
             xv2[1][lane + 2] = V1(-3).z;
             xv2[2][lane + 2] = V1(-3).y;
@@ -332,7 +348,7 @@ __global__ void __launch_bounds__(BW, 3)
 
         // ---- receivers: step n at row s-1 (stage 1), step n+1 at row s-5 (stage 2)
         if (rec1 && s - 1 >= zA && s - 1 < zB) gather(trace1, s - 1, lv[(PH + 2) % 3], ls[(PH + 2) % 3]);
-        if (rec2 && s - 5 >= zA && s - 5 < zB) gather(trace2, s - 5, lv2[(PH + 1) % 6], ls2[(PH + 1) % 3]);
+        if (rec2 && s - 5 >= zA && s - 5 < zB) gather(trace2, s - 5, lv2[(PH + 1) % R2], ls2[(PH + 1) % 3]);
 
         // ---- stage 1: stresses of row s (step n), kept for stage 2 in the register ring
         T sxx, p, syy;
@@ -359,6 +375,7 @@ __global__ void __launch_bounds__(BW, 3)
                               pv1, src1);
         else
             V1(-2) = V(-2);
+        if constexpr (EDGE) V(4) = ld(Vo, s + 4);  // into the slot of row s-2, now dead
 
         // ---- stage 2: stresses of row s-4 (step n+1)
         const int r2 = s - 4;
@@ -367,7 +384,7 @@ __global__ void __launch_bounds__(BW, 3)
         T sxx2 = 0, p2 = 0, syy2 = 0;
         Vec4<T> s2 = zero4;
         if (r2 >= zA - 2 && r2 <= zB + 1) {
-            s2 = stress(r2, V1(-6), V1(-5), V1(-4), V1(-3), V1(-2), s1r[(PH + 2) % 6], lv2[(PH + 2) % 6],
+            s2 = stress(r2, V1(-6), V1(-5), V1(-4), V1(-3), V1(-2), s1r[(PH + 2) % 6], lv2[(PH + 2) % R2],
                         lpal[idr2[(PH + 2) % 6]], lpal[idr2[(PH + 3) % 6]], pxm, px_out, pzm, pz_out, srow, src2, sxx2,
                         p2, syy2);
             if (srow) *(GN *)(Sn + off(r2)) = nvec4<T>{s2.x, s2.y, s2.z, s2.w};
