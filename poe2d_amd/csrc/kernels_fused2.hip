@@ -33,33 +33,22 @@
 
 namespace poe2d {
 
-// Three waves per SIMD: at most 168 VGPRs, and LDS for three blocks of 256 threads per CU (the
-// palette copy is cut to kMaxPalette2 entries).  EDGE = false: tiles none of whose points (halos
-// included) lie in a C-PML strip and that hold no receivers; the C-PML and receiver code is then
-// compiled out (the work list puts such tiles in a launch of their own), and the registers this
-// frees hold velocity loads further ahead.  Without the stage-2 receiver gather, the step-(n+1)
-// velocity rows need an LDS ring of 3 instead of 6.
+// One tile of the launch.  EDGE = false: tiles none of whose points (halos included) lie in a
+// C-PML strip and that hold no receivers; the C-PML and receiver code is then compiled out, and
+// the registers this frees hold velocity loads further ahead.  Without the stage-2 receiver
+// gather, the step-(n+1) velocity rows need an LDS ring of 3 instead of 6.  LDS rings, entry lane
+// + 2 of each row: lv 3 step-n velocity rows (vx, wx, vy), ls 3 stage-1 stress rows (sxx, p, sxy),
+// lv2 6 step-(n+1) velocity rows, ls2 3 stage-2 stress rows; lpal the MS then the MV entries.
 template <class T, int BW, bool AVG, bool EDGE>
-__global__ void __launch_bounds__(BW, 3)
-    fused_step2(const StepArgs<T> *__restrict__ models, const WorkItem *__restrict__ work, int nwork, int per_xcd,
-                const long long *__restrict__ base, int k)
+__device__ __forceinline__ void step2_tile(StepArgs<T> A, const int tile, const long long n, T (*lv)[3][BW + 4],
+                                           T (*ls)[3][BW + 4], T (*lv2)[3][BW + 4], T (*ls2)[3][BW + 4],
+                                           Vec4<T> *lpal)
 {
     constexpr int W = fused2_tile_width(BW);
     constexpr uint32_t VB = sizeof(Vec4<T>);
     typedef __attribute__((address_space(1))) nvec4<T> GN;
     typedef __attribute__((address_space(1))) T GT;
-    __shared__ T lv[3][3][BW + 4];   // ring of 3 step-n velocity rows: vx, wx, vy (entry lane + 2)
-    __shared__ T ls[3][3][BW + 4];   // ring of 3 stage-1 stress rows: sxx, p, sxy
     constexpr int R2 = EDGE ? 6 : 3;
-    __shared__ T lv2[R2][3][BW + 4];  // ring of R2 step-(n+1) velocity rows
-    __shared__ T ls2[3][3][BW + 4];  // ring of 3 stage-2 stress rows
-    __shared__ Vec4<T> lpal[2 * kMaxPalette2];  // MS entries, then MV entries
-
-    const int wi = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
-    if (wi >= nwork) return;
-    const WorkItem item = work[wi];
-    StepArgs<T> A = models[item.model];
-    const long long n = *(const kcll *)base + k;  // the launch advances steps n and n + 1
     A.in = (int)((((n - A.pair0) >> 1) + A.pcur) & 1);
     A.out = A.in ^ 1;
     typedef __attribute__((address_space(1))) const T GCT;
@@ -68,7 +57,6 @@ __global__ void __launch_bounds__(BW, 3)
         return (A.src_series && q >= 0 && q < A.src_len) ? ((GCT *)A.src_series)[q] : T(0);
     };
     const T src1 = sample(n), src2 = sample(n + 1);
-    const int tile = item.tile;
     const int tx = tile % A.ntx;
     const int tz = tile / A.ntx;
     const int lane = threadIdx.x;
@@ -172,26 +160,38 @@ __global__ void __launch_bounds__(BW, 3)
                       T &p, T &syy) __attribute__((always_inline)) -> Vec4<T> {
         const bool zr1 = r >= 2 && r < nz - 1;
         const bool zr2 = r >= 1 && r < nz - 1;
+        const bool u1 = xr1 && zr1, u2 = xr2 && zr2;
         const int szs = EDGE ? zstrip(A, r) : -1;
+        // C-PML: the update's old records and row coefficients, all read before any record is written
+        const long long qx = (long long)r * A.wxs + sx, qz = (long long)szs * P + c;
+        T o_dvx = 0, o_dwx = 0, o_dvy = 0, o_dvz = 0, o_dwz = 0, o_dvxz = 0;
+        T az = 0, bz = 0, rkz = 0, azh = 0, bzh = 0;
+        if (sx >= 0) {
+            if (u1) o_dvx = pxo(qx)[PX_DVX], o_dwx = pxo(qx)[PX_DWX];
+            if (u2) o_dvy = pxo(qx)[PX_DVY];
+        }
+        if (szs >= 0) {
+            if (u1) o_dvz = pzo(qz)[PZ_DVZ], o_dwz = pzo(qz)[PZ_DWZ];
+            if (u2) o_dvxz = pzo(qz)[PZ_DVX];
+            az = tab[TB_AZ * A.tn + r], bz = tab[TB_BZ * A.tn + r], rkz = tab[TB_RKZ * A.tn + r];
+            azh = tab[TB_AZH * A.tn + r], bzh = tab[TB_BZH * A.tn + r];
+        }
         T exx_n = so.x, eyy_n = so.y, zeta_n = so.z, sxy_n = so.w;
         p = 0;
         sxx = 0;
         syy = 0;
-        if (xr1 && zr1) {
+        if (u1) {
             T dvx = d4(xv[0][lane + 3], v0.x, xv[0][lane + 4], xv[0][lane + 1], rdx);
             T dwx = d4(xv[1][lane + 3], v0.z, xv[1][lane + 4], xv[1][lane + 1], rdx);
             T dvz = d4(v0.y, vm1.y, vp1.y, vm2.y, rdz);
             T dwz = d4(v0.w, vm1.w, vp1.w, vm2.w, rdz);
             if (sx >= 0) {
-                const long long q = (long long)r * A.wxs + sx;
-                dvx = cpml(dvx, pxo(q) + PX_DVX, pxn_(q) + PX_DVX, keep, axh, bxh, rkxh);
-                dwx = cpml(dwx, pxo(q) + PX_DWX, pxn_(q) + PX_DWX, keep, axh, bxh, rkxh);
+                dvx = cpml_from(dvx, o_dvx, pxn_(qx) + PX_DVX, keep, axh, bxh, rkxh);
+                dwx = cpml_from(dwx, o_dwx, pxn_(qx) + PX_DWX, keep, axh, bxh, rkxh);
             }
             if (szs >= 0) {
-                const long long q = (long long)szs * P + c;
-                const T az = tab[TB_AZ * A.tn + r], bz = tab[TB_BZ * A.tn + r], rkz = tab[TB_RKZ * A.tn + r];
-                dvz = cpml(dvz, pzo(q) + PZ_DVZ, pzn_(q) + PZ_DVZ, keep, az, bz, rkz);
-                dwz = cpml(dwz, pzo(q) + PZ_DWZ, pzn_(q) + PZ_DWZ, keep, az, bz, rkz);
+                dvz = cpml_from(dvz, o_dvz, pzn_(qz) + PZ_DVZ, keep, az, bz, rkz);
+                dwz = cpml_from(dwz, o_dwz, pzn_(qz) + PZ_DWZ, keep, az, bz, rkz);
             }
             exx_n = so.x + dt * dvx;
             eyy_n = so.y + dt * dvz;
@@ -205,19 +205,12 @@ __global__ void __launch_bounds__(BW, 3)
                 syy = lc * exx_n + l2m * eyy_n - al * p;
             }
         }
-        if (xr2 && zr2) {
+        if (u2) {
             T ddx = d4(v0.y, xv[2][lane + 1], xv[2][lane + 3], xv[2][lane], rdx);
             T ddz = d4(vp1.x, v0.x, vp2.x, vm1.x, rdz);
-            if (sx >= 0) {
-                const long long q = (long long)r * A.wxs + sx;
-                ddx = cpml(ddx, pxo(q) + PX_DVY, pxn_(q) + PX_DVY, keep, ax, bx, rkx);
-            }
-            if (szs >= 0) {
-                // half-grid memory coefficients, full-grid K_z divisor (reference :224-226)
-                const long long q = (long long)szs * P + c;
-                ddz = cpml(ddz, pzo(q) + PZ_DVX, pzn_(q) + PZ_DVX, keep, tab[TB_AZH * A.tn + r],
-                           tab[TB_BZH * A.tn + r], tab[TB_RKZ * A.tn + r]);
-            }
+            if (sx >= 0) ddx = cpml_from(ddx, o_dvy, pxn_(qx) + PX_DVY, keep, ax, bx, rkx);
+            // half-grid memory coefficients, full-grid K_z divisor (reference :224-226)
+            if (szs >= 0) ddz = cpml_from(ddz, o_dvxz, pzn_(qz) + PZ_DVX, keep, azh, bzh, rkz);
             const T mu_xy = AVG ? harmonic_mu(ms_cur.w, ms_nxt.w) : ms_nxt.w;
             sxy_n = so.w + mu_xy * (ddx + ddz) * dt;
         }
@@ -231,36 +224,40 @@ __global__ void __launch_bounds__(BW, 3)
                         const Vec4<T> &mv_j1, auto pxo, auto pxn_, auto pzo, auto pzn_, bool keep,
                         T srcv) __attribute__((always_inline)) -> Vec4<T> {
         const int szj = EDGE ? zstrip(A, j) : -1;
-        if (xr4 && j >= 2 && j < nz) {
+        const bool u4 = xr4 && j >= 2 && j < nz, u5 = xr5 && j >= 1 && j < nz - 1;
+        // C-PML: the update's old records and row coefficients, all read before any record is written
+        const long long qx = (long long)j * A.wxs + sx, qz = (long long)szj * P + c;
+        T o_dsxx = 0, o_dpx = 0, o_dsxy = 0, o_dsxyz = 0, o_dsyy = 0, o_dpz = 0;
+        T az = 0, bz = 0, rkz = 0, azh = 0, bzh = 0, rkzh = 0;
+        if (sx >= 0) {
+            if (u4) o_dsxx = pxo(qx)[PX_DSXX], o_dpx = pxo(qx)[PX_DPX];
+            if (u5) o_dsxy = pxo(qx)[PX_DSXY];
+        }
+        if (szj >= 0) {
+            if (u4) o_dsxyz = pzo(qz)[PZ_DSXY];
+            if (u5) o_dsyy = pzo(qz)[PZ_DSYY], o_dpz = pzo(qz)[PZ_DPZ];
+            az = tab[TB_AZ * A.tn + j], bz = tab[TB_BZ * A.tn + j], rkz = tab[TB_RKZ * A.tn + j];
+            azh = tab[TB_AZH * A.tn + j], bzh = tab[TB_BZH * A.tn + j], rkzh = tab[TB_RKZH * A.tn + j];
+        }
+        if (u4) {
             T e1 = d4(xs[0][lane + 2], xs[0][lane + 1], xs[0][lane + 3], xs[0][lane], rdx);
             T e2 = d4(xs[1][lane + 2], xs[1][lane + 1], xs[1][lane + 3], xs[1][lane], rdx);
             T e3 = d4(sxy_0, sxy_m1, sxy_p1, sxy_m2, rdz);
             if (sx >= 0) {
-                const long long q = (long long)j * A.wxs + sx;
-                e1 = cpml(e1, pxo(q) + PX_DSXX, pxn_(q) + PX_DSXX, keep, ax, bx, rkx);
-                e2 = cpml(e2, pxo(q) + PX_DPX, pxn_(q) + PX_DPX, keep, ax, bx, rkx);
+                e1 = cpml_from(e1, o_dsxx, pxn_(qx) + PX_DSXX, keep, ax, bx, rkx);
+                e2 = cpml_from(e2, o_dpx, pxn_(qx) + PX_DPX, keep, ax, bx, rkx);
             }
-            if (szj >= 0) {
-                const long long q = (long long)szj * P + c;
-                e3 = cpml(e3, pzo(q) + PZ_DSXY, pzn_(q) + PZ_DSXY, keep, tab[TB_AZ * A.tn + j],
-                          tab[TB_BZ * A.tn + j], tab[TB_RKZ * A.tn + j]);
-            }
+            if (szj >= 0) e3 = cpml_from(e3, o_dsxyz, pzn_(qz) + PZ_DSXY, keep, az, bz, rkz);
             biot_velocity(v.x, v.z, e1 + e3, e2, mv_j.x, mv_j.y, mv_j.z, mv_j.w, A.inv_dt);
         }
-        if (xr5 && j >= 1 && j < nz - 1) {
+        if (u5) {
             T e1 = d4(xs[2][lane + 3], xs[2][lane + 2], xs[2][lane + 4], xs[2][lane + 1], rdx);
             T e2 = d4(syy_p1, syy_0, syy_p2, syy_m1, rdz);
             T e3 = d4(p_p1, p_0, p_p2, p_m1, rdz);
-            if (sx >= 0) {
-                const long long q = (long long)j * A.wxs + sx;
-                e1 = cpml(e1, pxo(q) + PX_DSXY, pxn_(q) + PX_DSXY, keep, axh, bxh, rkxh);
-            }
+            if (sx >= 0) e1 = cpml_from(e1, o_dsxy, pxn_(qx) + PX_DSXY, keep, axh, bxh, rkxh);
             if (szj >= 0) {
-                const long long q = (long long)szj * P + c;
-                const T azh = tab[TB_AZH * A.tn + j], bzh = tab[TB_BZH * A.tn + j];
-                const T rkzh = tab[TB_RKZH * A.tn + j];
-                e2 = cpml(e2, pzo(q) + PZ_DSYY, pzn_(q) + PZ_DSYY, keep, azh, bzh, rkzh);
-                e3 = cpml(e3, pzo(q) + PZ_DPZ, pzn_(q) + PZ_DPZ, keep, azh, bzh, rkzh);
+                e2 = cpml_from(e2, o_dsyy, pzn_(qz) + PZ_DSYY, keep, azh, bzh, rkzh);
+                e3 = cpml_from(e3, o_dpz, pzn_(qz) + PZ_DPZ, keep, azh, bzh, rkzh);
             }
             T div = e1 + e2;
             if (src_f && j == A.src_z) div += srcv;
@@ -427,15 +424,40 @@ __global__ void __launch_bounds__(BW, 3)
     }
 }
 
-template <class T, bool AVG, bool EDGE>
+// Three waves per SIMD: at most 168 VGPRs, and LDS for three blocks of 256 threads per CU (the
+// palette copy is cut to kMaxPalette2 entries).  Each XCD's share of the work list starts with its
+// edge tiles (tile index stored as ~tile), whose C-PML record traffic makes them latency-bound and
+// several times slower than interior tiles: started first, they overlap with the interior tiles
+// that follow instead of finishing the launch on their own.
+template <class T, int BW, bool AVG>
+__global__ void __launch_bounds__(BW, 3)
+    fused_step2(const StepArgs<T> *__restrict__ models, const WorkItem *__restrict__ work, int nwork, int per_xcd,
+                const long long *__restrict__ base, int k)
+{
+    __shared__ T lv[3][3][BW + 4];
+    __shared__ T ls[3][3][BW + 4];
+    __shared__ T lv2[6][3][BW + 4];
+    __shared__ T ls2[3][3][BW + 4];
+    __shared__ Vec4<T> lpal[2 * kMaxPalette2];
+    const int wi = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+    if (wi >= nwork) return;
+    const WorkItem item = work[wi];
+    const long long n = *(const kcll *)base + k;  // the launch advances steps n and n + 1
+    if (item.tile < 0)
+        step2_tile<T, BW, AVG, true>(models[item.model], ~item.tile, n, lv, ls, lv2, ls2, lpal);
+    else
+        step2_tile<T, BW, AVG, false>(models[item.model], item.tile, n, lv, ls, lv2, ls2, lpal);
+}
+
+template <class T, bool AVG>
 hipError_t launch_fused2_bw(const StepArgs<T> *models, const WorkItem *work, int nwork, int per_xcd, int bw,
                             const long long *base, int k, hipStream_t stream)
 {
     const dim3 grid(8 * per_xcd);
     switch (bw) {
-    case 64: fused_step2<T, 64, AVG, EDGE><<<grid, 64, 0, stream>>>(models, work, nwork, per_xcd, base, k); break;
-    case 128: fused_step2<T, 128, AVG, EDGE><<<grid, 128, 0, stream>>>(models, work, nwork, per_xcd, base, k); break;
-    case 256: fused_step2<T, 256, AVG, EDGE><<<grid, 256, 0, stream>>>(models, work, nwork, per_xcd, base, k); break;
+    case 64: fused_step2<T, 64, AVG><<<grid, 64, 0, stream>>>(models, work, nwork, per_xcd, base, k); break;
+    case 128: fused_step2<T, 128, AVG><<<grid, 128, 0, stream>>>(models, work, nwork, per_xcd, base, k); break;
+    case 256: fused_step2<T, 256, AVG><<<grid, 256, 0, stream>>>(models, work, nwork, per_xcd, base, k); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -443,17 +465,14 @@ hipError_t launch_fused2_bw(const StepArgs<T> *models, const WorkItem *work, int
 
 template <class T>
 hipError_t launch_fused2(const StepArgs<T> *models, const WorkItem *work, int nwork, int per_xcd, int bw, bool avg,
-                         bool edge, const long long *base, int k, hipStream_t stream)
+                         const long long *base, int k, hipStream_t stream)
 {
     if (nwork < 1 || per_xcd < 1 || 8LL * per_xcd < nwork) return hipErrorInvalidValue;
-    if (edge)
-        return avg ? launch_fused2_bw<T, true, true>(models, work, nwork, per_xcd, bw, base, k, stream)
-                   : launch_fused2_bw<T, false, true>(models, work, nwork, per_xcd, bw, base, k, stream);
-    return avg ? launch_fused2_bw<T, true, false>(models, work, nwork, per_xcd, bw, base, k, stream)
-               : launch_fused2_bw<T, false, false>(models, work, nwork, per_xcd, bw, base, k, stream);
+    return avg ? launch_fused2_bw<T, true>(models, work, nwork, per_xcd, bw, base, k, stream)
+               : launch_fused2_bw<T, false>(models, work, nwork, per_xcd, bw, base, k, stream);
 }
 
-template hipError_t launch_fused2<float>(const StepArgs<float> *, const WorkItem *, int, int, int, bool, bool,
+template hipError_t launch_fused2<float>(const StepArgs<float> *, const WorkItem *, int, int, int, bool,
                                          const long long *, int, hipStream_t);
 
 }  // namespace poe2d

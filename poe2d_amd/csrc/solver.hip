@@ -138,7 +138,7 @@ struct StepList {
     bool palette = false;
     bool avg = false;  // POE2D_MATERIAL_AVERAGED (all members alike)
     bool two = false;  // two-step kernel: each launch advances two steps
-    int nint = 0;      // two-step kernel: items [0, nint) are interior tiles, the rest edge tiles
+    int nint = 0;      // two-step kernel: interior tiles in the list
     int precision = POE2D_F32;
     // graph of kGraphSteps launches plus the base advance, and the launch it was captured for
     hipGraphExec_t exec = nullptr;
@@ -502,7 +502,7 @@ poe2d_status build_list(StepList &L, const std::vector<poe2d_solver *> &members,
         if (m->grid.staggered_material != material)
             return fail(POE2D_E_INVALID, "run: batch members differ in staggered material");
     std::vector<StepArgs<T>> args(nm);
-    std::vector<WorkItem> work, edge_work;
+    std::vector<WorkItem> work;
     std::vector<int> ptr;
     std::vector<int2> ent;
     std::vector<size_t> ptr_off(nm), ent_off(nm);
@@ -538,8 +538,8 @@ poe2d_status build_list(StepList &L, const std::vector<poe2d_solver *> &members,
         ptr.insert(ptr.end(), cnt.begin(), cnt.end());
         for (int r : order) ent.push_back(int2{hx[r], r});
         if (two) {
-            // interior tiles first: no C-PML strip point within the rows and columns the kernel
-            // computes for the tile, and no receivers in its rows
+            // edge tiles (stored as ~tile): a C-PML strip point within the rows and columns the
+            // kernel computes for the tile, or receivers in its rows
             const Model *md = s->m;
             const int nz = s->grid.nz;
             for (int t = 0; t < args[i].ntiles; ++t) {
@@ -551,7 +551,7 @@ poe2d_status build_list(StepList &L, const std::vector<poe2d_solver *> &members,
                 const bool zs = r0 < md->ztop || r1 >= md->zbot;
                 const size_t p0 = (size_t)tx * nrow + zA + 1, p1 = (size_t)tx * nrow + zB + 1;
                 const bool rec = s->nrec > 0 && cnt[p1 - 1] > cnt[p0 - 1];
-                (xs || zs || rec ? edge_work : work).push_back(WorkItem{i, t});
+                work.push_back(WorkItem{i, xs || zs || rec ? ~t : t});
             }
         }
         args[i].rec_n = s->nrec;
@@ -561,8 +561,17 @@ poe2d_status build_list(StepList &L, const std::vector<poe2d_solver *> &members,
         args[i].rec_mask = s->mask;
         args[i].traces = (T *)s->traces;
     }
-    const int nint = (int)work.size();
-    work.insert(work.end(), edge_work.begin(), edge_work.end());
+    int nint = (int)work.size();
+    if (two) {
+        // each XCD's share (a band of consecutive tiles) starts with its edge tiles, which are the
+        // slow ones: they then overlap with the band's interior tiles
+        const int per = (int)((work.size() + 7) / 8);
+        for (size_t b0 = 0; b0 < work.size(); b0 += per) {
+            const auto b1 = work.begin() + std::min(work.size(), b0 + per);
+            std::stable_partition(work.begin() + b0, b1, [](const WorkItem &w) { return w.tile < 0; });
+        }
+        nint = (int)std::count_if(work.begin(), work.end(), [](const WorkItem &w) { return w.tile >= 0; });
+    }
     poe2d_status st;
     if ((st = grow(&L.models, &L.cap_models, args.size() * sizeof(StepArgs<T>)))) return st;
     void *w = L.work;
@@ -611,16 +620,9 @@ hipError_t launch_list_chunk(const StepList &L, int count, hipStream_t stream)
 {
     if (L.two) {
         if constexpr (sizeof(T) == 4) {
-            // interior tiles, then the tiles with C-PML strips or receivers, per step pair
-            const int nedge = L.nwork - L.nint;
             for (int k = 0; k < count; ++k) {
-                hipError_t e = hipSuccess;
-                if (L.nint > 0)
-                    e = launch_fused2<T>((const StepArgs<T> *)L.models, L.work, L.nint, (L.nint + 7) / 8, L.bw, L.avg,
-                                         false, L.base, 2 * k, stream);
-                if (e == hipSuccess && nedge > 0)
-                    e = launch_fused2<T>((const StepArgs<T> *)L.models, L.work + L.nint, nedge, (nedge + 7) / 8, L.bw,
-                                         L.avg, true, L.base, 2 * k, stream);
+                const hipError_t e = launch_fused2<T>((const StepArgs<T> *)L.models, L.work, L.nwork, L.per_xcd, L.bw,
+                                                      L.avg, L.base, 2 * k, stream);
                 if (e != hipSuccess) return e;
             }
             return launch_base_advance(L.base, 2 * count, stream);

@@ -181,12 +181,22 @@ __device__ __forceinline__ T d4(T f1, T f0, T f2, T fm, T rh)
 
 // C-PML correction of a derivative d: psi_new = b psi_old + a d (stored when `store`), returns
 // d / K + psi_new, with rk = 1/K.
+template <class T, class PN>
+__device__ __forceinline__ T cpml_from(T d, T psi_old, PN psi_n, bool store, T a, T b, T rk)
+{
+    const T s = b * psi_old + a * d;
+    if (store) *psi_n = s;
+    return d * rk + s;
+}
+
+// The same, reading psi_old from psi_o.  Kernels whose old and new records are different memory
+// read every old record of an update first and call cpml_from: the compiler cannot tell the two
+// record sets apart and would otherwise hold each read until the write before it is done, one
+// memory latency per record.
 template <class T, class PO, class PN>
 __device__ __forceinline__ T cpml(T d, PO psi_o, PN psi_n, bool store, T a, T b, T rk)
 {
-    const T s = b * *psi_o + a * d;
-    if (store) *psi_n = s;
-    return d * rk + s;
+    return cpml_from(d, T(*psi_o), psi_n, store, a, b, rk);
 }
 
 // Quotient used by the velocity update: IEEE division in FP64; in FP32 the hardware reciprocal
@@ -271,11 +281,11 @@ inline int fused_tile_width(int bw) { return bw - 8; }
 // side), and the launch; n = *base + k is the first of the two steps.
 __host__ __device__ constexpr int fused2_tile_width(int bw) { return bw - 12; }
 constexpr int kMaxPalette2 = 32;  // palette entries the two-step kernel keeps in LDS
-// edge: the items are tiles touching a C-PML strip or holding receivers (the others run a variant
-// without that code).
+// Items whose tile field is negative (~tile) are edge tiles: tiles touching a C-PML strip or
+// holding receivers (the others run a variant without that code).
 template <class T>
 hipError_t launch_fused2(const StepArgs<T> *models, const WorkItem *work, int nwork, int per_xcd, int bw, bool avg,
-                         bool edge, const long long *base, int k, hipStream_t stream);
+                         const long long *base, int k, hipStream_t stream);
 // Rows and columns around a two-step tile whose points the kernel computes (halos of both steps and
 // the over-run of its row loop): a tile with a C-PML strip point in this range is an edge tile.
 constexpr int kFused2RowsBefore = 10, kFused2RowsAfter = 12, kFused2ColsBefore = 6;
