@@ -26,6 +26,10 @@ void launch_derived(const StepArgs<T> &A, int buf, int what, T *out, hipStream_t
 template <class T>
 void launch_extract(const StepArgs<T> &A, int g, int comp, T *out, hipStream_t stream);
 
+// Velocity-update constants (biot_velocity_cn) of the npal MV entries of an FP32 palette into its
+// PAL_CN section.
+void launch_cn_palette(Vec4<float> *pal, int npal, float inv_dt, hipStream_t stream);
+
 // Largest |vx|, |vy| of buffer `buf` (non-finite counts as +inf) into result[0].
 template <class T>
 void launch_absmax(const StepArgs<T> &A, int buf, float *partials, int nblocks, float *result,

@@ -126,6 +126,23 @@ void launch_absmax(const StepArgs<T> &A, int buf, float *partials, int nblocks, 
     absmax_final<<<1, 256, 0, stream>>>(partials, nblocks, result);
 }
 
+__global__ void __launch_bounds__(256) cn_palette_kernel(Vec4<float> *pal, int npal, float inv_dt)
+{
+    const int e = (int)threadIdx.x;
+    if (e >= npal) return;
+    const Vec4<float> mv = pal[PAL_MV * kMaxPalette + e];
+    float co, c1, c2;
+    cn_coeffs(mv.x, mv.y, mv.z, mv.w, inv_dt, co, c1, c2);
+    const float rhofb = mv.y * mv.w;
+    pal[PAL_CN * kMaxPalette + e] = Vec4<float>{c2, __builtin_amdgcn_rcpf(c1), rhofb, __builtin_amdgcn_rcpf(co)};
+}
+
+void launch_cn_palette(Vec4<float> *pal, int npal, float inv_dt, hipStream_t stream)
+{
+    static_assert(kMaxPalette <= 256, "one thread per palette entry");
+    cn_palette_kernel<<<1, 256, 0, stream>>>(pal, npal, inv_dt);
+}
+
 #define POE2D_AUX_INSTANTIATE(T)                                                                      \
     template void launch_gather<T>(const StepArgs<T> &, int, int, const Receivers &, T *, hipStream_t); \
     template void launch_derived<T>(const StepArgs<T> &, int, int, T *, hipStream_t);                 \

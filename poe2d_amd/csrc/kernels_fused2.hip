@@ -38,7 +38,8 @@ namespace poe2d {
 // the registers this frees hold velocity loads further ahead.  Without the stage-2 receiver
 // gather, the step-(n+1) velocity rows need an LDS ring of 3 instead of 6.  LDS rings, entry lane
 // + 2 of each row: lv 3 step-n velocity rows (vx, wx, vy), ls 3 stage-1 stress rows (sxx, p, sxy),
-// lv2 6 step-(n+1) velocity rows, ls2 3 stage-2 stress rows; lpal the MS then the MV entries.
+// lv2 6 step-(n+1) velocity rows, ls2 3 stage-2 stress rows; lpal the MS, the MV and the
+// velocity-constant entries (palette sections PAL_MS, PAL_MV, PAL_CN, kMaxPalette2 entries each).
 template <class T, int BW, bool AVG, bool EDGE>
 __device__ __forceinline__ void step2_tile(StepArgs<T> A, const int tile, const long long n, T (*lv)[3][BW + 4],
                                            T (*ls)[3][BW + 4], T (*lv2)[3][BW + 4], T (*ls2)[3][BW + 4],
@@ -105,10 +106,10 @@ __device__ __forceinline__ void step2_tile(StepArgs<T> A, const int tile, const 
     auto ldm = [&](int row) -> int { return mid[colm + (uint32_t)(min(row, last_row) + kPadRows) * (uint32_t)P]; };
     {
         const GN *const pal = (const GN *)A.pal;
-        for (int i = lane; i < 2 * kMaxPalette2; i += BW) {
+        for (int i = lane; i < PAL_SECTIONS * kMaxPalette2; i += BW) {
             const int e = i % kMaxPalette2;
             if (e < A.npal) {
-                const nvec4<T> v = pal[(i < kMaxPalette2 ? 0 : kMaxPalette) + e];
+                const nvec4<T> v = pal[(i / kMaxPalette2) * kMaxPalette + e];
                 lpal[i] = Vec4<T>{v.x, v.y, v.z, v.w};
             }
         }
@@ -155,8 +156,8 @@ __device__ __forceinline__ void step2_tile(StepArgs<T> A, const int tile, const 
     // and written through pn_* when `keep`).  Returns the new (e_xx, e_yy, zeta, sigma_xy) and
     // sets sxx, p, syy.
     auto stress = [&](int r, const Vec4<T> &vm2, const Vec4<T> &vm1, const Vec4<T> &v0, const Vec4<T> &vp1,
-                      const Vec4<T> &vp2, const Vec4<T> &so, const T (*xv)[BW + 4], const Vec4<T> &ms_cur,
-                      const Vec4<T> &ms_nxt, auto pxo, auto pxn_, auto pzo, auto pzn_, bool keep, T srcv, T &sxx,
+                      const Vec4<T> &vp2, const Vec4<T> &so, const T (*xv)[BW + 4], int id_cur, int id_nxt,
+                      auto pxo, auto pxn_, auto pzo, auto pzn_, bool keep, T srcv, T &sxx,
                       T &p, T &syy) __attribute__((always_inline)) -> Vec4<T> {
         const bool zr1 = r >= 2 && r < nz - 1;
         const bool zr2 = r >= 1 && r < nz - 1;
@@ -196,6 +197,7 @@ __device__ __forceinline__ void step2_tile(StepArgs<T> A, const int tile, const 
             exx_n = so.x + dt * dvx;
             eyy_n = so.y + dt * dvz;
             zeta_n = (so.z - dwx * dt) - dwz * dt;
+            const Vec4<T> ms_cur = lpal[PAL_MS * kMaxPalette2 + id_cur];
             p = pore_pressure(exx_n, eyy_n, zeta_n, ms_cur.x, ms_cur.y);
             if (src_p && r == A.src_z) p += srcv;
             if (xr3) {
@@ -211,7 +213,8 @@ __device__ __forceinline__ void step2_tile(StepArgs<T> A, const int tile, const 
             if (sx >= 0) ddx = cpml_from(ddx, o_dvy, pxn_(qx) + PX_DVY, keep, ax, bx, rkx);
             // half-grid memory coefficients, full-grid K_z divisor (reference :224-226)
             if (szs >= 0) ddz = cpml_from(ddz, o_dvxz, pzn_(qz) + PZ_DVX, keep, azh, bzh, rkz);
-            const T mu_xy = AVG ? harmonic_mu(ms_cur.w, ms_nxt.w) : ms_nxt.w;
+            const T mu_nxt = lpal[PAL_MS * kMaxPalette2 + id_nxt].w;
+            const T mu_xy = AVG ? harmonic_mu(lpal[PAL_MS * kMaxPalette2 + id_cur].w, mu_nxt) : mu_nxt;
             sxy_n = so.w + mu_xy * (ddx + ddz) * dt;
         }
         return Vec4<T>{exx_n, eyy_n, zeta_n, sxy_n};
@@ -220,9 +223,8 @@ __device__ __forceinline__ void step2_tile(StepArgs<T> A, const int tile, const 
     // One velocity update of row j at this lane: v the step's old velocities of row j, xs the
     // new stress row j (x neighbours), sxy of rows j-2 .. j+1 and syy, p of rows j-1 .. j+2.
     auto velocity = [&](int j, Vec4<T> v, const T (*xs)[BW + 4], T sxy_m2, T sxy_m1, T sxy_0, T sxy_p1, T syy_m1,
-                        T syy_0, T syy_p1, T syy_p2, T p_m1, T p_0, T p_p1, T p_p2, const Vec4<T> &mv_j,
-                        const Vec4<T> &mv_j1, auto pxo, auto pxn_, auto pzo, auto pzn_, bool keep,
-                        T srcv) __attribute__((always_inline)) -> Vec4<T> {
+                        T syy_0, T syy_p1, T syy_p2, T p_m1, T p_0, T p_p1, T p_p2, int id_j, int id_j1, auto pxo,
+                        auto pxn_, auto pzo, auto pzn_, bool keep, T srcv) __attribute__((always_inline)) -> Vec4<T> {
         const int szj = EDGE ? zstrip(A, j) : -1;
         const bool u4 = xr4 && j >= 2 && j < nz, u5 = xr5 && j >= 1 && j < nz - 1;
         // C-PML: the update's old records and row coefficients, all read before any record is written
@@ -248,7 +250,8 @@ __device__ __forceinline__ void step2_tile(StepArgs<T> A, const int tile, const 
                 e2 = cpml_from(e2, o_dpx, pxn_(qx) + PX_DPX, keep, ax, bx, rkx);
             }
             if (szj >= 0) e3 = cpml_from(e3, o_dsxyz, pzn_(qz) + PZ_DSXY, keep, az, bz, rkz);
-            biot_velocity(v.x, v.z, e1 + e3, e2, mv_j.x, mv_j.y, mv_j.z, mv_j.w, A.inv_dt);
+            const Vec4<T> mv_j = lpal[PAL_MV * kMaxPalette2 + id_j];
+            biot_velocity_cn(v.x, v.z, e1 + e3, e2, mv_j.x, mv_j.y, mv_j.z, lpal[PAL_CN * kMaxPalette2 + id_j]);
         }
         if (u5) {
             T e1 = d4(xs[2][lane + 3], xs[2][lane + 2], xs[2][lane + 4], xs[2][lane + 1], rdx);
@@ -261,8 +264,13 @@ __device__ __forceinline__ void step2_tile(StepArgs<T> A, const int tile, const 
             }
             T div = e1 + e2;
             if (src_f && j == A.src_z) div += srcv;
-            const Vec4<T> mvy = AVG ? mean_rows(mv_j, mv_j1) : mv_j1;
-            biot_velocity(v.y, v.w, div, e3, mvy.x, mvy.y, mvy.z, mvy.w, A.inv_dt);
+            if constexpr (AVG) {
+                const Vec4<T> mvy = mean_rows(lpal[PAL_MV * kMaxPalette2 + id_j], lpal[PAL_MV * kMaxPalette2 + id_j1]);
+                biot_velocity(v.y, v.w, div, e3, mvy.x, mvy.y, mvy.z, mvy.w, A.inv_dt);
+            } else {
+                const Vec4<T> mv_j1 = lpal[PAL_MV * kMaxPalette2 + id_j1];
+                biot_velocity_cn(v.y, v.w, div, e3, mv_j1.x, mv_j1.y, mv_j1.z, lpal[PAL_CN * kMaxPalette2 + id_j1]);
+            }
         }
         return v;
     };
@@ -327,8 +335,6 @@ __device__ __forceinline__ void step2_tile(StepArgs<T> A, const int tile, const 
         // s+1 and MV of rows s-2, s-1; stage 2 MS of rows s-4, s-3 and MV of rows s-6, s-5
         idr2[(PH + 5) % 6] = idr[(PH + 5) % 6];  // row s-1 moves to the older ring
         idr[(PH + 3) % 6] = ldm(s + 3);           // into the slot of row s-3, moved on at row step s-2
-        const Vec4<T> ms_cur = lpal[idr[PH % 6]], ms_nxt = lpal[idr[(PH + 1) % 6]];
-        const Vec4<T> mv_j = lpal[kMaxPalette2 + idr2[(PH + 4) % 6]], mv_j1 = lpal[kMaxPalette2 + idr[(PH + 5) % 6]];
 
         // ---- x neighbours through LDS: step-n velocities of row s, step-(n+1) velocities of row s-3
         {
@@ -349,7 +355,7 @@ __device__ __forceinline__ void step2_tile(StepArgs<T> A, const int tile, const 
 
         // ---- stage 1: stresses of row s (step n), kept for stage 2 in the register ring
         T sxx, p, syy;
-        const Vec4<T> s1 = stress(s, V(-2), V(-1), V(0), V(1), V(2), so, lv[PH % 3], ms_cur, ms_nxt, px_in, pxm,
+        const Vec4<T> s1 = stress(s, V(-2), V(-1), V(0), V(1), V(2), so, lv[PH % 3], idr[PH % 6], idr[(PH + 1) % 6], px_in, pxm,
                                   pz_in, pzm, ps1, src1, sxx, p, syy);
         s1r[PH % 6] = s1;
         {
@@ -368,8 +374,8 @@ __device__ __forceinline__ void step2_tile(StepArgs<T> A, const int tile, const 
         // ---- stage 1: velocities of row s-2 (step n -> n+1); stage 2 reads rows [zA-4, zB+3]
         if (s - 2 >= zA - 4 && s - 2 <= zB + 3)
             V1(-2) = velocity(s - 2, V(-2), ls[(PH + 1) % 3], SXY(-4), SXY(-3), SXY(-2), SXY(-1), SYY(-3), SYY(-2),
-                              SYY(-1), SYY(0), PP(-3), PP(-2), PP(-1), PP(0), mv_j, mv_j1, px_in, pxm, pz_in, pzm,
-                              pv1, src1);
+                              SYY(-1), SYY(0), PP(-3), PP(-2), PP(-1), PP(0), idr2[(PH + 4) % 6], idr[(PH + 5) % 6], px_in, pxm,
+                              pz_in, pzm, pv1, src1);
         else
             V1(-2) = V(-2);
         if constexpr (EDGE) V(4) = ld(Vo, s + 4);  // into the slot of row s-2, now dead
@@ -382,7 +388,7 @@ __device__ __forceinline__ void step2_tile(StepArgs<T> A, const int tile, const 
         Vec4<T> s2 = zero4;
         if (r2 >= zA - 2 && r2 <= zB + 1) {
             s2 = stress(r2, V1(-6), V1(-5), V1(-4), V1(-3), V1(-2), s1r[(PH + 2) % 6], lv2[(PH + 2) % R2],
-                        lpal[idr2[(PH + 2) % 6]], lpal[idr2[(PH + 3) % 6]], pxm, px_out, pzm, pz_out, srow, src2, sxx2,
+                        idr2[(PH + 2) % 6], idr2[(PH + 3) % 6], pxm, px_out, pzm, pz_out, srow, src2, sxx2,
                         p2, syy2);
             if (srow) *(GN *)(Sn + off(r2)) = nvec4<T>{s2.x, s2.y, s2.z, s2.w};
         }
@@ -405,8 +411,7 @@ __device__ __forceinline__ void step2_tile(StepArgs<T> A, const int tile, const 
             const bool vrow = own;
             const Vec4<T> v2 = velocity(j2, V1(-6), ls2[PH % 3], SXY2(-8), SXY2(-7), SXY2(-6), SXY2(-5), SYY2(-7),
                                         SYY2(-6), SYY2(-5), SYY2(-4), PP2(-7), PP2(-6), PP2(-5), PP2(-4),
-                                        lpal[kMaxPalette2 + idr2[PH % 6]], lpal[kMaxPalette2 + idr2[(PH + 1) % 6]],
-                                        pxm, px_out, pzm, pz_out, vrow, src2);
+                                        idr2[PH % 6], idr2[(PH + 1) % 6], pxm, px_out, pzm, pz_out, vrow, src2);
             if (vrow) *(GN *)(Vn + off(j2)) = nvec4<T>{v2.x, v2.y, v2.z, v2.w};
         }
         ++s;
@@ -426,9 +431,10 @@ __device__ __forceinline__ void step2_tile(StepArgs<T> A, const int tile, const 
 
 // Three waves per SIMD: at most 168 VGPRs, and LDS for three blocks of 256 threads per CU (the
 // palette copy is cut to kMaxPalette2 entries).  Each XCD's share of the work list starts with its
-// edge tiles (tile index stored as ~tile), whose C-PML record traffic makes them latency-bound and
-// several times slower than interior tiles: started first, they overlap with the interior tiles
-// that follow instead of finishing the launch on their own.
+// edge tiles (tile index stored as ~tile), whose chains of C-PML record accesses make them
+// latency-bound and about twice as long as interior tiles: started first, they overlap with the
+// interior tiles that follow instead of finishing the launch on their own.  Items with a negative
+// model index pad the shares to equal length.
 template <class T, int BW, bool AVG>
 __global__ void __launch_bounds__(BW, 3)
     fused_step2(const StepArgs<T> *__restrict__ models, const WorkItem *__restrict__ work, int nwork, int per_xcd,
@@ -438,10 +444,11 @@ __global__ void __launch_bounds__(BW, 3)
     __shared__ T ls[3][3][BW + 4];
     __shared__ T lv2[6][3][BW + 4];
     __shared__ T ls2[3][3][BW + 4];
-    __shared__ Vec4<T> lpal[2 * kMaxPalette2];
+    __shared__ Vec4<T> lpal[PAL_SECTIONS * kMaxPalette2];
     const int wi = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
     if (wi >= nwork) return;
     const WorkItem item = work[wi];
+    if (item.model < 0) return;
     const long long n = *(const kcll *)base + k;  // the launch advances steps n and n + 1
     if (item.tile < 0)
         step2_tile<T, BW, AVG, true>(models[item.model], ~item.tile, n, lv, ls, lv2, ls2, lpal);

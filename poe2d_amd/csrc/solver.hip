@@ -95,7 +95,7 @@ struct Model {
     double cp_max = 0;
     void *mat = nullptr;               // MS, MV groups
     unsigned char *mid = nullptr;      // material index per point (palette mode)
-    void *pal = nullptr;               // 2 * kMaxPalette Vec4 entries
+    void *pal = nullptr;               // PAL_SECTIONS * kMaxPalette Vec4 entries
     int npal = 0;                      // palette entries, 0 when the grid has more than kMaxPalette
     void *tab = nullptr;
     int *xmap = nullptr, *zmap = nullptr;
@@ -359,10 +359,10 @@ poe2d_status upload_coeffs(Model *md, const poe2d_coeffs *c)
             idx[md->origin() + (size_t)z * md->pitch + x] = (unsigned char)last_id;
         }
     if (pal.size() > (size_t)kMaxPalette) return POE2D_OK;  // no palette: the groups serve every schedule
-    std::vector<Vec4<T>> hp(2 * kMaxPalette, Vec4<T>{T(0), T(0), T(0), T(0)});
+    std::vector<Vec4<T>> hp(PAL_SECTIONS * kMaxPalette, Vec4<T>{T(0), T(0), T(0), T(0)});
     for (size_t e = 0; e < pal.size(); ++e) {
-        hp[e] = Vec4<T>{pal[e][0], pal[e][1], pal[e][2], pal[e][3]};
-        hp[kMaxPalette + e] = Vec4<T>{pal[e][4], pal[e][5], pal[e][6], pal[e][7]};
+        hp[PAL_MS * kMaxPalette + e] = Vec4<T>{pal[e][0], pal[e][1], pal[e][2], pal[e][3]};
+        hp[PAL_MV * kMaxPalette + e] = Vec4<T>{pal[e][4], pal[e][5], pal[e][6], pal[e][7]};
     }
     void *q = nullptr;
     poe2d_status st = dev_alloc(&q, md->group_points());
@@ -372,6 +372,13 @@ poe2d_status upload_coeffs(Model *md, const poe2d_coeffs *c)
     HIP_TRY(hipMemcpy(md->mid, idx.data(), idx.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(md->pal, hp.data(), hp.size() * sizeof(Vec4<T>), hipMemcpyHostToDevice));
     md->npal = (int)pal.size();
+    if constexpr (sizeof(T) == 4) {
+        // the velocity update's constants of each entry, computed on the device as the kernels
+        // would compute them per point (the same expressions and reciprocal)
+        launch_cn_palette((Vec4<float> *)md->pal, md->npal, (float)(1.0 / g.dt), nullptr);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipDeviceSynchronize());
+    }
     return POE2D_OK;
 }
 
@@ -562,15 +569,23 @@ poe2d_status build_list(StepList &L, const std::vector<poe2d_solver *> &members,
         args[i].traces = (T *)s->traces;
     }
     int nint = (int)work.size();
+    int per_xcd = (int)((work.size() + 7) / 8);
     if (two) {
-        // each XCD's share (a band of consecutive tiles) starts with its edge tiles, which are the
-        // slow ones: they then overlap with the band's interior tiles
-        const int per = (int)((work.size() + 7) / 8);
-        for (size_t b0 = 0; b0 < work.size(); b0 += per) {
-            const auto b1 = work.begin() + std::min(work.size(), b0 + per);
-            std::stable_partition(work.begin() + b0, b1, [](const WorkItem &w) { return w.tile < 0; });
+        // Edge tiles take about twice as long as interior ones and gather in the bands of the z
+        // strips: each XCD's share of the launch gets every eighth edge tile, first, and then a
+        // band of consecutive interior tiles, so that the XCDs finish together and the edge tiles
+        // overlap with interior work.  Shares are padded to equal length with empty items.
+        std::vector<WorkItem> edge, inter;
+        for (const WorkItem &w : work) (w.tile < 0 ? edge : inter).push_back(w);
+        nint = (int)inter.size();
+        const int ne = (int)((edge.size() + 7) / 8), ni = (int)((inter.size() + 7) / 8);
+        per_xcd = ne + ni;
+        work.assign((size_t)8 * per_xcd, WorkItem{-1, 0});
+        for (int x = 0; x < 8; ++x) {
+            size_t o = (size_t)x * per_xcd;
+            for (size_t e = x; e < edge.size(); e += 8) work[o++] = edge[e];
+            for (size_t q = (size_t)x * ni; q < std::min(inter.size(), (size_t)(x + 1) * ni); ++q) work[o++] = inter[q];
         }
-        nint = (int)std::count_if(work.begin(), work.end(), [](const WorkItem &w) { return w.tile >= 0; });
     }
     poe2d_status st;
     if ((st = grow(&L.models, &L.cap_models, args.size() * sizeof(StepArgs<T>)))) return st;
@@ -604,7 +619,7 @@ poe2d_status build_list(StepList &L, const std::vector<poe2d_solver *> &members,
     L.nmodels = nm;
     L.nint = two ? nint : (int)work.size();
     L.nwork = (int)work.size();
-    L.per_xcd = (L.nwork + 7) / 8;
+    L.per_xcd = per_xcd;
     L.bw = bw;
     L.palette = palette;
     L.avg = material == POE2D_MATERIAL_AVERAGED;

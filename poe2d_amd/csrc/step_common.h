@@ -74,6 +74,9 @@ enum {
     NPSI_Z = 6
 };
 constexpr int kMaxPalette = 256;  // distinct (MS, MV) tuples a one-byte material index can select
+// palette sections of kMaxPalette Vec4 entries each: MS, MV, and (FP32 grids) the velocity
+// update's constants of each MV entry (see biot_velocity_cn)
+enum { PAL_MS = 0, PAL_MV = 1, PAL_CN = 2, PAL_SECTIONS = 3 };
 
 // 1-D C-PML profiles, one table each, `tn` entries apart.  The K tables hold 1/K (K = 1 outside
 // the layers, and wherever K_max = 1).
@@ -209,18 +212,52 @@ __device__ __forceinline__ double quot(double a, double b) { return a / b; }
 //   [rho rho_f; rho_f m] d/dt [v; w] = [div; -grad - b w],
 // with the viscous term b w taken Crank-Nicolson ((w_old + w_new) / 2), as the reference's vx and
 // vz kernels do (seismic_CPML_2D_poroelastic_withCUDA.cu:293-311).
+// The material's constants co = (rho m - rho_f^2) / dt, c1 = co + rho b / 2, c2 = co - rho b / 2,
+// and the two numerators, shared by every form of the update so that all of them round alike.
+template <class T>
+__device__ __forceinline__ void cn_coeffs(T rho, T rhof, T m, T b, T inv_dt, T &co, T &c1, T &c2)
+{
+    co = (rho * m - rhof * rhof) * inv_dt;
+    const T half_rb = rho * b * T(0.5);
+    c1 = co + half_rb;
+    c2 = co - half_rb;
+}
+template <class T>
+__device__ __forceinline__ T cn_wnum(T c2, T wold, T div, T grad, T rho, T rhof)
+{
+    return c2 * wold + (-rhof * div - rho * grad);
+}
+template <class T>
+__device__ __forceinline__ T cn_vnum(T div, T grad, T wavg, T m, T rhof, T rhofb)
+{
+    return m * div + rhof * grad + rhofb * wavg;
+}
 template <class T>
 __device__ __forceinline__ void biot_velocity(T &v, T &w, T div, T grad, T rho, T rhof, T m, T b,
                                               T inv_dt)
 {
-    const T co = (rho * m - rhof * rhof) * inv_dt;
-    const T half_rb = rho * b * T(0.5);
-    const T c1 = co + half_rb;
-    const T c2 = co - half_rb;
+    T co, c1, c2;
+    cn_coeffs(rho, rhof, m, b, inv_dt, co, c1, c2);
     const T wold = w;
-    const T wnew = quot(c2 * wold + (-rhof * div - rho * grad), c1);
+    const T wnew = quot(cn_wnum(c2, wold, div, grad, rho, rhof), c1);
     const T wavg = (wold + wnew) * T(0.5);
-    v = v + quot(m * div + rhof * grad + rhof * b * wavg, co);
+    v = v + quot(cn_vnum(div, grad, wavg, m, rhof, rhof * b), co);
+    w = wnew;
+}
+
+// The FP32 update with a palette entry's constants dv = (c2, 1/c1, rho_f b, 1/co), as
+// cn_palette_kernel computes them (cn_coeffs, the hardware reciprocal quot uses): equal bit for
+// bit to biot_velocity on the entry's (rho, rho_f, m, b), without its per-point divisions.  The
+// products with the reciprocals are rounded on their own (times_recip), as quot's are, so that
+// no multiply-add forms across them.
+__device__ __forceinline__ float times_recip(float a, float r) { return a * r; }
+__device__ __forceinline__ void biot_velocity_cn(float &v, float &w, float div, float grad, float rho, float rhof,
+                                                 float m, const Vec4<float> &dv)
+{
+    const float wold = w;
+    const float wnew = times_recip(cn_wnum(dv.x, wold, div, grad, rho, rhof), dv.y);
+    const float wavg = (wold + wnew) * 0.5f;
+    v = v + times_recip(cn_vnum(div, grad, wavg, m, rhof, dv.z), dv.w);
     w = wnew;
 }
 
