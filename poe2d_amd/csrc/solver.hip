@@ -1247,6 +1247,8 @@ poe2d_status poe2d_reset(poe2d_solver *s)
     return POE2D_OK;
 }
 
+}  // extern "C"
+
 namespace {
 
 // Tile shapes found by the autotuners, per grid class (device, precision, kernel, palette use and
@@ -1283,6 +1285,8 @@ void tune_store(const std::string &key, int w, int h)
 }
 
 }  // namespace
+
+extern "C" {
 
 poe2d_status poe2d_autotune(poe2d_solver *s, int nsteps, int *block_w, int *tile_rows)
 {
