@@ -452,7 +452,11 @@ def main():
                                     f"(104 B x {npts} points + 48 B x strip points, {cols} strip columns, "
                                     f"{rows} strip rows)"),
                      "hbm_gbs": None if traffic is None else round(traffic / (step_ms / 1e3) / 1e9, 1),
-                     "traffic_source": traffic_src},
+                     "traffic_source": traffic_src,
+                     "note": ("algorithmic bytes: 104 B per cell-update (+ C-PML strips) whatever the schedule; "
+                              "the two-step kernel performs two cell-updates per point for each pass over HBM, "
+                              "so frac above 1 is its gain over the one-pass-per-step roofline, and hbm_gbs is "
+                              "the measured traffic rate") if schedule == 3 else None},
     }
     if args.cpu_steps > 0:
         from oracle.pyoracle import material_biot
