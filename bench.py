@@ -262,7 +262,8 @@ def surrogate_bench(args):
 
     ntr, nval, ncand = args.sur_train, args.sur_val, args.sur_cand
     params = S.draw_params(ntr + nval, seed=3)
-    gathers, solve = S.simulate(params, geom=geom)
+    gathers, solve = S.simulate(params, geom=geom, progress=lambda d, n: d % 1024 == 0 and print(
+        f"[surrogate] solved {d} of {n}", file=sys.stderr, flush=True))
     print(f"[surrogate] solver alone: {solve.n} models in {solve.wall_seconds:.1f} s "
           f"({solve.models_per_second:.1f}/s; GPU {solve.gpu_models_per_second:.1f}/s)", file=sys.stderr, flush=True)
     sur = S.Surrogate(geom, members=5, max_batch=1024)
@@ -277,7 +278,8 @@ def surrogate_bench(args):
     screens = []
     for q in quantiles:
         thr = S.spread_threshold(verr, vspread, 0.05, q)
-        res = sur.screen(cand, thr)
+        res = sur.screen(cand, thr, progress=lambda d, n: d % 1024 == 0 and print(
+            f"[surrogate] re-solved {d} of {n}", file=sys.stderr, flush=True))
         je = res.judge_errors
         ffb = len(res.fallback) / res.n
         value = res.models_per_second
@@ -329,8 +331,9 @@ def main():
     ap.add_argument("--sur-train", type=int, default=4096,
                     help="surrogate: training models (0: time the network alone)")
     ap.add_argument("--sur-val", type=int, default=512, help="surrogate: held-out models for the threshold")
-    ap.add_argument("--sur-cand", type=int, default=4096, help="surrogate: candidates screened")
-    ap.add_argument("--sur-steps", type=int, default=15000, help="surrogate: training steps per member")
+    ap.add_argument("--sur-cand", type=int, default=16384,
+                    help="surrogate: candidates screened (configuration 4: 16 384)")
+    ap.add_argument("--sur-steps", type=int, default=30000, help="surrogate: training steps per member")
     ap.add_argument("--sur-quantiles", default="0.5,0.99",
                     help="surrogate: error quantiles after fallback held within 5 %% by the threshold, one "
                          "screening pass each (the first gives the line's value)")

@@ -125,10 +125,11 @@ class SolveTiming:
         return self.n / self.gpu_seconds
 
 
-def simulate(params, seeds=None, geom=CONFIG3, batch=64, workers=8):
+def simulate(params, seeds=None, geom=CONFIG3, batch=64, workers=8, progress=None):
     """Gathers [n, 3, nrec, nsamp] (v_x, v_y, p) of the models, solved in batches of `batch`
     handles advanced together while host threads build the next models; returns (gathers,
-    SolveTiming).  seeds: patch-layout seed per model (LAYOUT_SEED for all when None)."""
+    SolveTiming).  seeds: patch-layout seed per model (LAYOUT_SEED for all when None); progress:
+    called with (models done, n) after each batch."""
     t0 = time.perf_counter()
     n = len(params)
     seeds = np.full(n, LAYOUT_SEED) if seeds is None else np.asarray(seeds)
@@ -155,6 +156,8 @@ def simulate(params, seeds=None, geom=CONFIG3, batch=64, workers=8):
             finally:
                 for s in solvers:
                     s.close()
+            if progress:
+                progress(min(i + batch, n), n)
     return out, SolveTiming(n, time.perf_counter() - t0, gpu_ms * 1e-3)
 
 
@@ -261,16 +264,18 @@ class Surrogate:
         err = self.errors(pred, gathers)
         return spread_threshold(err, spread, bound, quantile), err, spread
 
-    def screen(self, params, threshold, seeds=None, judge_frac=0.05, seed=7):
+    def screen(self, params, threshold, seeds=None, judge_frac=0.05, seed=7, progress=None):
         """Predict every candidate, re-solve those whose spread exceeds `threshold`, then re-solve
-        a random `judge_frac` of the others to measure the surrogate's error (timed apart)."""
+        a random `judge_frac` of the others to measure the surrogate's error (timed apart).
+        progress: passed to simulate for both sets of re-solves."""
         t0 = time.perf_counter()
         pred, spread = self.predict(params)
         t_pred = time.perf_counter() - t0
         fallback = np.nonzero(spread > threshold)[0]
         fb = SolveTiming(0, 0.0, 0.0)
         if len(fallback):
-            _, fb = simulate(params[fallback], None if seeds is None else seeds[fallback], self.geom)
+            _, fb = simulate(params[fallback], None if seeds is None else seeds[fallback], self.geom,
+                             progress=progress)
         t_screen = time.perf_counter() - t0
         accepted = np.setdiff1d(np.arange(len(params)), fallback)
         rng = np.random.default_rng(seed)
@@ -279,7 +284,8 @@ class Surrogate:
         err = np.zeros(0)
         jt = SolveTiming(0, 0.0, 0.0)
         if nj:
-            true, jt = simulate(params[judged], None if seeds is None else seeds[judged], self.geom)
+            true, jt = simulate(params[judged], None if seeds is None else seeds[judged], self.geom,
+                                progress=progress)
             err = self.errors(pred[judged], true)
         return ScreenResult(len(params), fallback, judged, err, t_screen, t_pred, fb, jt)
 
