@@ -266,7 +266,7 @@ def surrogate_bench(args):
         f"[surrogate] solved {d} of {n}", file=sys.stderr, flush=True))
     print(f"[surrogate] solver alone: {solve.n} models in {solve.wall_seconds:.1f} s "
           f"({solve.models_per_second:.1f}/s; GPU {solve.gpu_models_per_second:.1f}/s)", file=sys.stderr, flush=True)
-    sur = S.Surrogate(geom, members=5, max_batch=1024)
+    sur = S.Surrogate(geom, members=5, max_batch=1024, receivers=bool(args.sur_receivers))
     t0 = time.perf_counter()
     sur.fit(params[:ntr], gathers[:ntr], args.sur_steps,
             log=lambda k, l: print(f"[surrogate] member {k}: loss {l[-20:].mean():.5f}", file=sys.stderr, flush=True))
@@ -304,9 +304,11 @@ def surrogate_bench(args):
         "value": first["models_per_s"], "unit": "models/s", "n_gpus": 1, "steps": 1, "warmup": 0,
         "ms_per_step": round(first["screen_s"] * 1e3, 1), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "bf16 network (FP32 accumulate), f32 solver", "data": "synthetic",
-        "config": {"workload": f"{ncand} candidates screened by a 5-member 4-512-512-512-{sizes[-1]} GELU "
+        "config": {"workload": f"{ncand} candidates screened by a 5-member "
+                               f"{sur.net_sizes[0]}-512-512-512-{sizes[-1]} GELU "
                                f"ensemble trained {args.sur_steps} steps (batch 256) on {ntr} solver gathers "
-                               f"(401^2, 3000 steps / 6, 101 receivers x 3)",
+                               f"(401^2, 3000 steps / 6, 101 receivers x 3)"
+                               + (", inputs: 4 parameters + gas at each receiver" if sur.receivers else ""),
                    "step": "one screening pass: prediction + re-solves of the high-spread candidates",
                    "solver_alone_models_per_s": round(solve.models_per_second, 2),
                    "solver_alone_gpu_models_per_s": round(solve.gpu_models_per_second, 2),
@@ -334,6 +336,9 @@ def main():
     ap.add_argument("--sur-cand", type=int, default=16384,
                     help="surrogate: candidates screened (configuration 4: 16 384)")
     ap.add_argument("--sur-steps", type=int, default=30000, help="surrogate: training steps per member")
+    ap.add_argument("--sur-receivers", type=int, default=1,
+                    help="surrogate: 1 = per-receiver gas inputs besides the four parameters, 0 = the "
+                         "four parameters alone")
     ap.add_argument("--sur-quantiles", default="0.5,0.99",
                     help="surrogate: error quantiles after fallback held within 5 %% by the threshold, one "
                          "screening pass each (the first gives the line's value)")

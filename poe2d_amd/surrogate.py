@@ -12,9 +12,10 @@ which the four inputs do not say: two layouts with the same parameters differ by
 relative L2 (pore pressure at a receiver drops about 40-fold when a gas patch covers it), far
 beyond what any surrogate of the parameters could reach.
 
-The network (4 -> 512 -> 512 -> 512 -> outputs, GELU, bf16 on the matrix cores, 5 members) is
-trained on gathers the solver itself produced, each component scaled by its RMS over the training
-set.  Screening predicts every candidate; a candidate whose ensemble spread exceeds the threshold
+The network (inputs -> 512 -> 512 -> 512 -> outputs, GELU, bf16 on the matrix cores, 5 members)
+takes the four parameters and, by default, one input per receiver telling whether gas covers it
+(receiver_gas, from the fixed layout), and is trained on gathers the solver itself produced, each
+component scaled by its RMS over the training set.  Screening predicts every candidate; a candidate whose ensemble spread exceeds the threshold
 is re-solved with the batched solver (fallback), and a random 5 % of the accepted ones are
 re-solved as well to judge the surrogate (timed apart from the screening).  Errors are relative L2
 over the whole gather after the per-component scaling; after fallback, a re-solved model's error
@@ -63,13 +64,35 @@ def patch_count(params, geom=CONFIG3):
     return p[:, 0] * area / (np.pi * p[:, 1] ** 2)
 
 
-def features(params, geom=CONFIG3):
+def receiver_gas(params, geom=CONFIG3, workers=8):
+    """[n, nrec] float32: 1 where the model's gas patches (the fixed layout, painted for its gas
+    fraction and radius) cover the receiver, else 0.  A function of the parameters: the mask the
+    solver's model is built from, read at the receivers."""
+    ix, iz = receiver_points(geom)
+    n, dx = geom["n"], geom["dx"]
+
+    def one(p):
+        gas, radius = float(p[0]), float(p[1])
+        if gas <= 0:
+            return np.zeros(len(ix), np.float32)
+        return patchy_mask(n, n, dx, dx, gas, LAYOUT_SEED, radius, radius)[iz, ix].astype(np.float32)
+
+    p = np.asarray(params, np.float64).reshape(-1, 4)
+    if len(p) == 0:
+        return np.zeros((0, len(ix)), np.float32)
+    with ThreadPoolExecutor(workers) as pool:
+        return np.stack(list(pool.map(one, p)))
+
+
+def features(params, geom=CONFIG3, receivers=False):
     """Network inputs: log(1 + nominal patch count), log radius, porosity and log permeability,
     each mapped linearly onto [-1, 1] over the parameter ranges, times INPUT_SCALE.  The patch
     count replaces the gas fraction (the two determine each other given the radius): patch k of
     the fixed layout is painted once the count passes about k and covers a receiver once the
     radius passes its distance, so the jumps a receiver sees sit along the input axes.  Wider
-    inputs let the first layer form those jumps sharply."""
+    inputs let the first layer form those jumps sharply.  With `receivers`, one more input per
+    receiver: +INPUT_SCALE where gas covers it (receiver_gas), -INPUT_SCALE where brine does; pore
+    pressure there differs about 40-fold between the two."""
     p = np.asarray(params, np.float64)
     lo_r, hi_r = np.log(RANGES["radius"][0]), np.log(RANGES["radius"][1])
     lo_k, hi_k = np.log(RANGES["kappa"][0]), np.log(RANGES["kappa"][1])
@@ -78,6 +101,8 @@ def features(params, geom=CONFIG3):
                   2.0 * (np.log(p[:, 1]) - lo_r) / (hi_r - lo_r) - 1.0,
                   (p[:, 2] - 0.25) / 0.1,
                   2.0 * (np.log(p[:, 3]) - lo_k) / (hi_k - lo_k) - 1.0], axis=1)
+    if receivers:
+        f = np.concatenate([f, 2.0 * receiver_gas(p, geom) - 1.0], axis=1)
     return (INPUT_SCALE * f).astype(np.float32)
 
 
@@ -206,14 +231,17 @@ class ScreenResult:
 
 
 class Surrogate:
-    """Ensemble network from features(params, geom) to a model's gather.  The network's targets are the
-    gathers with each component divided by its RMS over the training set."""
+    """Ensemble network from features(params, geom, receivers) to a model's gather.  The network's
+    targets are the gathers with each component divided by its RMS over the training set.
+    receivers: add the per-receiver gas inputs (features)."""
 
-    def __init__(self, geom=CONFIG3, members=5, hidden=512, max_batch=1024, seed=1):
+    def __init__(self, geom=CONFIG3, members=5, hidden=512, max_batch=1024, seed=1, receivers=True):
         self.geom = dict(geom)
+        self.receivers = bool(receivers)
         self.shape = gather_shape(geom)
         self.nout = int(np.prod(self.shape))
-        self.net = Network([4, hidden, hidden, hidden, self.nout], members=members, max_batch=max_batch, seed=seed)
+        self.net_sizes = [4 + (geom["nrec"] if self.receivers else 0), hidden, hidden, hidden, self.nout]
+        self.net = Network(self.net_sizes, members=members, max_batch=max_batch, seed=seed)
         self.scale = np.ones(self.shape[0], np.float32)    # RMS of each component of the gathers
 
     def _scaled(self, g):
@@ -226,7 +254,7 @@ class Surrogate:
         g = np.asarray(gathers, np.float32)
         rms = np.sqrt(np.mean(g.astype(np.float64) ** 2, axis=(0, 2, 3)))
         self.scale = np.where(rms > 0, rms, 1.0).astype(np.float32)
-        self.net.set_dataset(features(params, self.geom), self._scaled(g))
+        self.net.set_dataset(features(params, self.geom, self.receivers), self._scaled(g))
         rng = np.random.default_rng(seed)
         n = len(params)
         cuts = np.linspace(0, steps, chunks + 1).round().astype(int)
@@ -247,7 +275,7 @@ class Surrogate:
 
     def predict(self, params):
         """(gathers [n, 3, nrec, nsamp], ensemble spread [n] in units of the scaled targets)."""
-        mean, spread = self.net.predict(features(params, self.geom))
+        mean, spread = self.net.predict(features(params, self.geom, self.receivers))
         g = mean.reshape((len(params),) + self.shape)
         g *= self.scale[None, :, None, None]
         return g, spread

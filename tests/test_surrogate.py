@@ -111,3 +111,23 @@ def test_mixture_quantile():
     full = np.concatenate([e, np.zeros(1000)])
     for q in (0.3, 0.5, 0.9, 0.99):
         assert abs(S.mixture_quantile(e, 0.2, q) - np.quantile(full, q)) < 2e-3
+
+
+def test_receiver_inputs_follow_the_model_material():
+    # the per-receiver gas input is +INPUT_SCALE exactly where the solver's model holds gas at the
+    # receiver (the gas fluid modulus is ~40 times the brine's smaller), -INPUT_SCALE elsewhere
+    params = S.draw_params(12, seed=11)
+    params[0, 0] = 0.0  # one model without gas
+    ix, iz = S.receiver_points(S.SMALL)
+    f = S.features(params, S.SMALL, receivers=True)
+    assert f.shape == (12, 4 + S.SMALL["nrec"])
+    np.testing.assert_array_equal(f[:, :4], S.features(params, S.SMALL))
+    seen_gas = 0
+    for k, p in enumerate(params):
+        M = S.model_problem(p, geom=S.SMALL).coeffs["M"][iz, ix]
+        M_brine = S.model_problem([0.0, *p[1:]], geom=S.SMALL).coeffs["M"][iz, ix]
+        gas_here = M < 0.5 * M_brine
+        np.testing.assert_array_equal(f[k, 4:], np.where(gas_here, S.INPUT_SCALE, -S.INPUT_SCALE))
+        seen_gas += int(gas_here.sum())
+    assert seen_gas > 0
+    assert np.all(f[0, 4:] == -S.INPUT_SCALE)
