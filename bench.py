@@ -308,7 +308,7 @@ def surrogate_bench(args):
                                f"{sur.net_sizes[0]}-512-512-512-{sizes[-1]} GELU "
                                f"ensemble trained {args.sur_steps} steps (batch 256) on {ntr} solver gathers "
                                f"(401^2, 3000 steps / 6, 101 receivers x 3)"
-                               + (", inputs: 4 parameters + gas at each receiver" if sur.receivers else ""),
+                               + (", inputs: 4 parameters + gas at each receiver, on its path from the source and at the source" if sur.receivers else ""),
                    "step": "one screening pass: prediction + re-solves of the high-spread candidates",
                    "solver_alone_models_per_s": round(solve.models_per_second, 2),
                    "solver_alone_gpu_models_per_s": round(solve.gpu_models_per_second, 2),

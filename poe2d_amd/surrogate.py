@@ -64,24 +64,40 @@ def patch_count(params, geom=CONFIG3):
     return p[:, 0] * area / (np.pi * p[:, 1] ** 2)
 
 
-def receiver_gas(params, geom=CONFIG3, workers=8):
-    """[n, nrec] float32: 1 where the model's gas patches (the fixed layout, painted for its gas
-    fraction and radius) cover the receiver, else 0.  A function of the parameters: the mask the
-    solver's model is built from, read at the receivers."""
+PATH_SAMPLES = 64
+
+
+def layout_inputs(params, geom=CONFIG3, workers=8):
+    """[n, 2 nrec + 1] float32 from the model's gas mask (the fixed layout, painted for its gas
+    fraction and radius; the mask the solver's model is built from): 1 where gas covers receiver
+    k, else 0 (columns [0, nrec)); the share of gas along the straight segment from the source to
+    receiver k, PATH_SAMPLES points (columns [nrec, 2 nrec)); 1 where gas covers the source (last
+    column).  Functions of the parameters."""
     ix, iz = receiver_points(geom)
     n, dx = geom["n"], geom["dx"]
+    c = n // 2
+    t = (np.arange(PATH_SAMPLES) + 0.5) / PATH_SAMPLES
+    px = np.rint(c + np.outer(ix - c, t)).astype(np.intp)
+    pz = np.rint(c + np.outer(iz - c, t)).astype(np.intp)
+    width = 2 * len(ix) + 1
 
     def one(p):
         gas, radius = float(p[0]), float(p[1])
         if gas <= 0:
-            return np.zeros(len(ix), np.float32)
-        return patchy_mask(n, n, dx, dx, gas, LAYOUT_SEED, radius, radius)[iz, ix].astype(np.float32)
+            return np.zeros(width, np.float32)
+        m = patchy_mask(n, n, dx, dx, gas, LAYOUT_SEED, radius, radius)
+        return np.concatenate([m[iz, ix], m[pz, px].mean(axis=1), [m[c, c]]]).astype(np.float32)
 
     p = np.asarray(params, np.float64).reshape(-1, 4)
     if len(p) == 0:
-        return np.zeros((0, len(ix)), np.float32)
+        return np.zeros((0, width), np.float32)
     with ThreadPoolExecutor(workers) as pool:
         return np.stack(list(pool.map(one, p)))
+
+
+def receiver_gas(params, geom=CONFIG3, workers=8):
+    """[n, nrec] float32: 1 where the model's gas patches cover the receiver, else 0."""
+    return layout_inputs(params, geom, workers)[:, :geom["nrec"]]
 
 
 def features(params, geom=CONFIG3, receivers=False):
@@ -90,9 +106,10 @@ def features(params, geom=CONFIG3, receivers=False):
     count replaces the gas fraction (the two determine each other given the radius): patch k of
     the fixed layout is painted once the count passes about k and covers a receiver once the
     radius passes its distance, so the jumps a receiver sees sit along the input axes.  Wider
-    inputs let the first layer form those jumps sharply.  With `receivers`, one more input per
-    receiver: +INPUT_SCALE where gas covers it (receiver_gas), -INPUT_SCALE where brine does; pore
-    pressure there differs about 40-fold between the two."""
+    inputs let the first layer form those jumps sharply.  With `receivers`, the layout_inputs: per
+    receiver +INPUT_SCALE where gas covers it, -INPUT_SCALE where brine does (pore pressure there
+    differs about 40-fold between the two), then per receiver the gas share of its path from the
+    source and whether gas covers the source, mapped from [0, 1] onto [-INPUT_SCALE, INPUT_SCALE]."""
     p = np.asarray(params, np.float64)
     lo_r, hi_r = np.log(RANGES["radius"][0]), np.log(RANGES["radius"][1])
     lo_k, hi_k = np.log(RANGES["kappa"][0]), np.log(RANGES["kappa"][1])
@@ -102,7 +119,7 @@ def features(params, geom=CONFIG3, receivers=False):
                   (p[:, 2] - 0.25) / 0.1,
                   2.0 * (np.log(p[:, 3]) - lo_k) / (hi_k - lo_k) - 1.0], axis=1)
     if receivers:
-        f = np.concatenate([f, 2.0 * receiver_gas(p, geom) - 1.0], axis=1)
+        f = np.concatenate([f, 2.0 * layout_inputs(p, geom) - 1.0], axis=1)
     return (INPUT_SCALE * f).astype(np.float32)
 
 
@@ -240,7 +257,7 @@ class Surrogate:
         self.receivers = bool(receivers)
         self.shape = gather_shape(geom)
         self.nout = int(np.prod(self.shape))
-        self.net_sizes = [4 + (geom["nrec"] if self.receivers else 0), hidden, hidden, hidden, self.nout]
+        self.net_sizes = [4 + (2 * geom["nrec"] + 1 if self.receivers else 0), hidden, hidden, hidden, self.nout]
         self.net = Network(self.net_sizes, members=members, max_batch=max_batch, seed=seed)
         self.scale = np.ones(self.shape[0], np.float32)    # RMS of each component of the gathers
 

@@ -120,14 +120,35 @@ def test_receiver_inputs_follow_the_model_material():
     params[0, 0] = 0.0  # one model without gas
     ix, iz = S.receiver_points(S.SMALL)
     f = S.features(params, S.SMALL, receivers=True)
-    assert f.shape == (12, 4 + S.SMALL["nrec"])
+    nrec = S.SMALL["nrec"]
+    assert f.shape == (12, 4 + 2 * nrec + 1)
     np.testing.assert_array_equal(f[:, :4], S.features(params, S.SMALL))
     seen_gas = 0
     for k, p in enumerate(params):
         M = S.model_problem(p, geom=S.SMALL).coeffs["M"][iz, ix]
         M_brine = S.model_problem([0.0, *p[1:]], geom=S.SMALL).coeffs["M"][iz, ix]
         gas_here = M < 0.5 * M_brine
-        np.testing.assert_array_equal(f[k, 4:], np.where(gas_here, S.INPUT_SCALE, -S.INPUT_SCALE))
+        np.testing.assert_array_equal(f[k, 4:4 + nrec], np.where(gas_here, S.INPUT_SCALE, -S.INPUT_SCALE))
+        # a receiver under gas has gas at the end of its path; inputs stay within the scale
+        assert np.all(f[k, 4 + nrec:4 + 2 * nrec][gas_here] > -S.INPUT_SCALE)
+        assert np.all(np.abs(f[k]) <= S.INPUT_SCALE + 1e-6)
         seen_gas += int(gas_here.sum())
     assert seen_gas > 0
     assert np.all(f[0, 4:] == -S.INPUT_SCALE)
+
+
+def test_path_inputs_count_the_gas_on_the_segment():
+    # brute force on one model: walk the source-receiver segment through the mask
+    p = S.draw_params(40, seed=12)
+    p = p[p[:, 0] > 0.2][:1]
+    g = S.SMALL
+    lay = S.layout_inputs(p, g)[0]
+    mask = S.patchy_mask(g["n"], g["n"], g["dx"], g["dx"], float(p[0, 0]), S.LAYOUT_SEED, float(p[0, 1]), float(p[0, 1]))
+    ix, iz = S.receiver_points(g)
+    c = g["n"] // 2
+    for k in (0, len(ix) // 2, len(ix) - 1):
+        share = np.mean([mask[int(np.rint(c + (iz[k] - c) * (i + 0.5) / S.PATH_SAMPLES)),
+                              int(np.rint(c + (ix[k] - c) * (i + 0.5) / S.PATH_SAMPLES))]
+                         for i in range(S.PATH_SAMPLES)])
+        assert lay[g["nrec"] + k] == pytest.approx(share)
+    assert lay[-1] == float(mask[c, c])
