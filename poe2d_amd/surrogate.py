@@ -25,6 +25,7 @@ CONFIG3 is the geometry of configuration 3: 401 x 401 points at 0.5 m, 3000 step
 decimated by 6 to 500 samples, 101 receivers at x = 50 + 3k, 40 points below the source (3 x 101 x
 500 = 151 500 outputs).  SMALL is a reduced geometry with the same layout for quick tests.
 """
+import ctypes
 import time
 from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass
@@ -76,23 +77,38 @@ def layout_inputs(params, geom=CONFIG3, workers=8):
     ix, iz = receiver_points(geom)
     n, dx = geom["n"], geom["dx"]
     c = n // 2
+    nrec = len(ix)
     t = (np.arange(PATH_SAMPLES) + 0.5) / PATH_SAMPLES
     px = np.rint(c + np.outer(ix - c, t)).astype(np.intp)
     pz = np.rint(c + np.outer(iz - c, t)).astype(np.intp)
-    width = 2 * len(ix) + 1
-
-    def one(p):
-        gas, radius = float(p[0]), float(p[1])
-        if gas <= 0:
-            return np.zeros(width, np.float32)
-        m = patchy_mask(n, n, dx, dx, gas, LAYOUT_SEED, radius, radius)
-        return np.concatenate([m[iz, ix], m[pz, px].mean(axis=1), [m[c, c]]]).astype(np.float32)
-
+    # flat indices into the (n+1) x (n+1) mask: receivers, path samples, source
+    flat = np.concatenate([iz * (n + 1) + ix, (pz * (n + 1) + px).ravel(), [c * (n + 1) + c]])
     p = np.asarray(params, np.float64).reshape(-1, 4)
-    if len(p) == 0:
-        return np.zeros((0, width), np.float32)
+    samples = np.zeros((len(p), len(flat)), np.uint8)  # the mask at the flat indices, per model
+    lib = _abi.load_library()
+
+    def run(lo, hi):
+        buf = np.zeros((n + 1) * (n + 1), np.uint8)
+        ptr = buf.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+        realised = ctypes.c_double()
+        for k in range(lo, hi):
+            gas, radius = float(p[k, 0]), float(p[k, 1])
+            if gas <= 0:
+                continue
+            _abi.check(lib, lib.poe2d_patchy_mask(n, n, dx, dx, gas, ctypes.c_uint64(LAYOUT_SEED), radius, radius,
+                                                  ptr, ctypes.byref(realised)), "patchy_mask")
+            np.take(buf, flat, out=samples[k])
+
+    cuts = np.linspace(0, len(p), workers + 1).round().astype(int)
     with ThreadPoolExecutor(workers) as pool:
-        return np.stack(list(pool.map(one, p)))
+        for f in [pool.submit(run, cuts[w], cuts[w + 1]) for w in range(workers)]:
+            f.result()
+    out = np.empty((len(p), 2 * nrec + 1), np.float32)
+    out[:, :nrec] = samples[:, :nrec]
+    out[:, nrec:2 * nrec] = samples[:, nrec:nrec * (PATH_SAMPLES + 1)].reshape(len(p), nrec, PATH_SAMPLES).mean(
+        axis=2, dtype=np.float32)
+    out[:, 2 * nrec] = samples[:, -1]
+    return out
 
 
 def receiver_gas(params, geom=CONFIG3, workers=8):
