@@ -219,10 +219,11 @@ def surrogate_bench(args):
     to 500 samples, 101 receivers x (vx, vy, p) = 151 500 outputs), with the model counts cut to fit
     a few minutes (the options say how many).
 
-    The network part times the full-shape ensemble member (4 -> 512 -> 512 -> 512 -> 151 500, bf16
-    on the matrix cores) with hipEvents: one forward pass over an inference batch of 1024 and one
-    training step (forward, backward, Adam) over a batch of 256; FLOPs are 2 x 78.09 M MACs per
-    sample forward and 3 x that per training sample.  The pipeline part draws models, solves them
+    The network part times the full-shape ensemble member of the screening network (207 -> 512 ->
+    512 -> 512 -> 151 500 with the layout inputs, 4 -> ... without; bf16 on the matrix cores) with
+    hipEvents: one forward pass over an inference batch of 1024 and one training step (forward,
+    backward, Adam) over a batch of 256; FLOPs are 2 x the MACs per sample forward (78.2 M with
+    the layout inputs) and 3 x that per training sample.  The pipeline part draws models, solves them
     with the batched solver (solver alone: models/s over every candidate), trains the 5-member
     ensemble, calibrates the spread threshold on held-out models, screens candidates (prediction +
     spread-triggered re-solves, the models/s of the surrogate) and re-solves a random 5 % of the
@@ -231,7 +232,8 @@ def surrogate_bench(args):
     from poe2d_amd.mlp import Network
 
     geom = S.CONFIG3
-    sizes = [4, 512, 512, 512, int(np.prod(S.gather_shape(geom)))]
+    nin = 4 + (2 * geom["nrec"] + 1 if args.sur_receivers else 0)
+    sizes = [nin, 512, 512, 512, int(np.prod(S.gather_shape(geom)))]
     macs = sum(a * b for a, b in zip(sizes[:-1], sizes[1:]))
     with Network(sizes, members=1, max_batch=1024) as net:
         net.time(256, 2)
@@ -244,7 +246,7 @@ def surrogate_bench(args):
 
     net_roofline = {"bound": "mfma", "achieved": round(fwd_tflops, 1), "peak": MFMA_BF16_PEAK_TFLOPS,
                     "unit": "TFLOP/s", "frac": round(fwd_tflops / MFMA_BF16_PEAK_TFLOPS, 4), "traffic": None,
-                    "per_launch": "one member's forward pass over 1024 samples: 2 x 78.09 M MACs each",
+                    "per_launch": f"one member's forward pass over 1024 samples: 2 x {macs / 1e6:.2f} M MACs each",
                     "train_step_tflops": round(train_tflops, 1),
                     "train_step_frac": round(train_tflops / MFMA_BF16_PEAK_TFLOPS, 4),
                     "forward_ms": round(fwd_ms, 4), "train_step_ms": round(train_ms, 4)}
@@ -254,8 +256,8 @@ def surrogate_bench(args):
             "unit": "TFLOP/s", "n_gpus": 1, "steps": 10, "warmup": 1, "ms_per_step": round(fwd_ms, 4),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "bf16 (FP32 accumulate)", "data": "synthetic",
-            "config": {"workload": f"one ensemble member 4-512-512-512-{sizes[-1]} GELU; forward over 1024 "
-                                   "samples, training step over 256",
+            "config": {"workload": f"one ensemble member {sizes[0]}-512-512-512-{sizes[-1]} GELU; forward over "
+                                   "1024 samples, training step over 256",
                        "step": "one forward pass"},
             "roofline": net_roofline}), flush=True)
         return
