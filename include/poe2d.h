@@ -167,10 +167,11 @@ poe2d_status poe2d_set_receivers(poe2d_solver *s, const poe2d_receivers *rec);
  *     read through a one-byte material index per point.
  *   schedule 1: a stress launch and a velocity launch per step, intermediates through memory.
  *   schedule 2: as schedule 0, always reading the per-point coefficient arrays.
- *   schedule 3: two time steps per launch (FP32 grids with a material palette; tiles of
- *     block_w - 12 columns, block_w 64, 128 or 256, by tile_rows rows, multiples of 6 best), the
- *     intermediate state kept on chip; an odd last step of a run takes one schedule-0 launch.
- *     FP64 handles and grids without a palette run schedule 0.
+ *   schedule 3: two time steps per launch (FP32 grids with a material palette of at most 28
+ *     entries; tiles of block_w - 12 columns, block_w 64, 128 or 256, by tile_rows rows,
+ *     multiples of 6 best), the intermediate state kept on chip; an odd last step of a run takes
+ *     one schedule-0 launch.  FP64 handles and grids with no palette or a larger one run
+ *     schedule 0 (block_w 64 as 128), and poe2d_autotune then tunes that kernel.
  * Tile heights do not change a bit; schedules and block widths agree to rounding.  The choice
  * only changes speed. */
 poe2d_status poe2d_set_schedule(poe2d_solver *s, int schedule, int block_w, int tile_rows);

@@ -22,6 +22,9 @@ STATUS_NAMES = {
 F32 = 32
 F64 = 64
 
+# palette entries up to which schedule 3 (two steps per launch) runs; larger palettes run schedule 0
+MAX_PALETTE_TWO_STEP = 28
+
 STRESS_AS_WRITTEN = 0
 STRESS_DRAINED = 1
 MATERIAL_AS_WRITTEN = 0

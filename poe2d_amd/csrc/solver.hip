@@ -704,6 +704,18 @@ poe2d_status stability_check(poe2d_solver *s)
     return POE2D_OK;
 }
 
+// The kernel a handle's steps run on: schedule 3 takes the two-step kernel on FP32 grids whose
+// palette fits that kernel's LDS copy, and the single-step kernel (as schedule 0, 64-thread blocks
+// widened to 128) otherwise.
+bool two_step_kernel(const poe2d_solver *s)
+{
+    return s->schedule == SCHED_FUSED2 && s->precision == POE2D_F32 && s->m->npal > 0 && s->m->npal <= kMaxPalette2;
+}
+int effective_schedule(const poe2d_solver *s)
+{
+    return s->schedule == SCHED_FUSED2 && !two_step_kernel(s) ? SCHED_FUSED : s->schedule;
+}
+
 template <class T>
 poe2d_status run_steps(poe2d_solver *s, int nsteps)
 {
@@ -724,7 +736,7 @@ poe2d_status run_steps(poe2d_solver *s, int nsteps)
         }
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(s->ev1, s->stream));
-    } else if (sizeof(T) == 4 && s->schedule == SCHED_FUSED2 && s->m->npal > 0 && s->m->npal <= kMaxPalette2) {
+    } else if (sizeof(T) == 4 && two_step_kernel(s)) {
         // step pairs on the two-step kernel, an odd last step on the single-step kernel
         const int pairs = nsteps / 2, rem = nsteps & 1;
         if (!s->psi_mid && (st = dev_alloc(&s->psi_mid, std::max<size_t>(s->m->psi_half_elems(), 1) * s->rs)))
@@ -743,7 +755,8 @@ poe2d_status run_steps(poe2d_solver *s, int nsteps)
     } else {
         if ((st = upload_source<T>(s, s->step, nsteps))) return st;
         const bool palette = (s->schedule == SCHED_FUSED || s->schedule == SCHED_FUSED2) && s->m->npal > 0;
-        if ((st = build_list<T>(s->list, {s}, s->block_w, s->tile_rows, palette, s->stream))) return st;
+        const int bw = s->schedule == SCHED_FUSED2 ? std::max(s->block_w, 128) : s->block_w;
+        if ((st = build_list<T>(s->list, {s}, bw, s->tile_rows, palette, s->stream))) return st;
         HIP_TRY(hipEventRecord(s->ev0, s->stream));
         if ((st = run_list<T>(s->list, s->step, nsteps, s->graphs, s->stream))) return st;
         HIP_TRY(hipEventRecord(s->ev1, s->stream));
@@ -856,11 +869,11 @@ poe2d_status batch_run_steps(poe2d_batch *b, int nsteps)
         poe2d_status st = upload_source<T>(s, n0, nsteps);
         if (st) return st;
         palette = palette && (s->schedule == SCHED_FUSED || s->schedule == SCHED_FUSED2) && s->m->npal > 0;
-        two = two && s->schedule == SCHED_FUSED2 && s->m->npal > 0 && s->m->npal <= kMaxPalette2;
+        two = two && two_step_kernel(s);
     }
-    if (two ? b->block_w == 512 : b->block_w == 64)
-        return fail(POE2D_E_INVALID, two ? "batch_run: the two-step kernel takes block widths 64, 128 or 256"
-                                         : "batch_run: the single-step kernel takes block widths 128, 256 or 512");
+    // (the single-step kernel widens 64-thread blocks to 128, below)
+    if (two && b->block_w == 512)
+        return fail(POE2D_E_INVALID, "batch_run: the two-step kernel takes block widths 64, 128 or 256");
     const int pairs = two ? nsteps / 2 : 0, rem = two ? nsteps & 1 : nsteps;
     poe2d_status st;
     if (pairs) {
@@ -1294,7 +1307,7 @@ poe2d_status poe2d_autotune(poe2d_solver *s, int nsteps, int *block_w, int *tile
     if (st) return st;
     if (s->schedule == SCHED_TWOPASS) return fail(POE2D_E_INVALID, "autotune: the two-pass schedule has no tile shape");
     if (s->step != 0) return fail(POE2D_E_STATE, "autotune: needs a handle at step 0 (call poe2d_reset first)");
-    const std::string key = tune_key({s}, s->schedule);
+    const std::string key = tune_key({s}, effective_schedule(s));
     if (nsteps >= 0) {
         int w, h;
         if (tune_lookup(key, &w, &h)) {
@@ -1308,7 +1321,7 @@ poe2d_status poe2d_autotune(poe2d_solver *s, int nsteps, int *block_w, int *tile
     nsteps = nsteps == 0 ? 10 : std::abs(nsteps);
     // Tile heights of 6k + 2 rows (6k for the two-step kernel) make whole groups of the kernels'
     // 6-step row loops.
-    const bool two = s->schedule == SCHED_FUSED2;
+    const bool two = two_step_kernel(s);
     const std::vector<int> widths = two ? std::vector<int>{64, 128, 256} : std::vector<int>{128, 256, 512};
     const std::vector<int> heights = two ? std::vector<int>{6, 12, 18, 24, 30, 42, 60}
                                          : std::vector<int>{8, 14, 20, 26, 32, 44, 62, 98};
@@ -1430,7 +1443,7 @@ poe2d_status poe2d_batch_autotune(poe2d_batch *b, int nsteps, int *block_w, int 
     for (poe2d_solver *s : b->members)
         if (s->step != 0) return fail(POE2D_E_STATE, "batch_autotune: needs every member at step 0");
     int kinds = 0;
-    for (poe2d_solver *s : b->members) kinds |= 1 << s->schedule;
+    for (poe2d_solver *s : b->members) kinds |= 1 << effective_schedule(s);
     const std::string key = "batch/" + tune_key(b->members, kinds);
     if (nsteps >= 0) {
         int w, h;
@@ -1444,8 +1457,7 @@ poe2d_status poe2d_batch_autotune(poe2d_batch *b, int nsteps, int *block_w, int 
     }
     nsteps = nsteps == 0 ? 10 : std::abs(nsteps);
     bool two = b->precision == POE2D_F32;
-    for (poe2d_solver *s : b->members)
-        two = two && s->schedule == SCHED_FUSED2 && s->m->npal > 0 && s->m->npal <= kMaxPalette2;
+    for (poe2d_solver *s : b->members) two = two && two_step_kernel(s);
     const std::vector<int> widths = two ? std::vector<int>{64, 128, 256} : std::vector<int>{128, 256, 512};
     const std::vector<int> heights = two ? std::vector<int>{6, 12, 18, 24, 30, 42, 60}
                                          : std::vector<int>{8, 14, 20, 26, 32, 44, 62, 98};

@@ -317,7 +317,7 @@ inline int fused_tile_width(int bw) { return bw - 8; }
 // The two-step kernel (FP32, palette grids): output columns per tile (halo of 6 columns on each
 // side), and the launch; n = *base + k is the first of the two steps.
 __host__ __device__ constexpr int fused2_tile_width(int bw) { return bw - 12; }
-constexpr int kMaxPalette2 = 32;  // palette entries the two-step kernel keeps in LDS
+constexpr int kMaxPalette2 = 28;  // palette entries the two-step kernel keeps in LDS (blocks of 64 threads: 12 per CU)
 // Items whose tile field is negative (~tile) are edge tiles: tiles touching a C-PML strip or
 // holding receivers (the others run a variant without that code).
 template <class T>

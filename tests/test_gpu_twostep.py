@@ -130,6 +130,37 @@ def test_fp64_and_no_palette_fall_back():
             s.set_schedule(3, 512, 12)
 
 
+def _layered(nlayers, n=160, nt=61):
+    # horizontal bands of distinct porosity: nlayers palette entries
+    fields = biot_fields(n, n, None, BRINE, GAS)
+    rows = np.linspace(0, n + 1, nlayers + 1).round().astype(int)
+    for k in range(nlayers):
+        fields["phi"][rows[k]:rows[k + 1], :] = 0.18 + 0.004 * k
+    coeffs = material_biot(**fields)
+    ix = np.arange(25, n - 24, 9, dtype=np.int32)
+    rec = ReceiverSpec(ix=ix, iz=np.full(len(ix), 100, np.int32))
+    return Problem(nx=n, nz=n, dx=0.5, dz=0.5, dt=5e-5, nt=nt, coeffs=coeffs,
+                   source=SourceSpec(kind=_abi.SRC_PRESSURE_DGAUSS, ix=n // 2, iz=n // 2, amp=1e10), receivers=rec,
+                   npml=20, f0=80.0)
+
+
+@pytest.mark.parametrize("extra", [0, 1])
+def test_palette_size_limit(extra):
+    # at the two-step kernel's palette limit it runs; one entry more and the run takes the
+    # single-step kernel: either way the results equal schedule 0's bit for bit
+    prob = _layered(_abi.MAX_PALETTE_TWO_STEP + extra)
+    with Solver(prob) as s:
+        assert s.material_palette() == _abi.MAX_PALETTE_TWO_STEP + extra
+        s.set_schedule(3, 64, 12)
+        s.run()
+        g3 = s.seismograms()
+    with Solver(prob) as s:
+        s.set_schedule(0, 256, 14)
+        s.run()
+        assert np.array_equal(s.seismograms(), g3)
+    assert np.abs(g3).max() > 0
+
+
 def test_autotune_two_step():
     prob = _problem(nt=20)
     with Solver(prob) as s:
