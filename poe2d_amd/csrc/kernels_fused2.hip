@@ -38,8 +38,9 @@ namespace poe2d {
 // the registers this frees hold velocity loads further ahead.  Without the stage-2 receiver
 // gather, the step-(n+1) velocity rows need an LDS ring of 3 instead of 6.  LDS rings, entry lane
 // + 2 of each row: lv 3 step-n velocity rows (vx, wx, vy), ls 3 stage-1 stress rows (sxx, p, sxy),
-// lv2 6 step-(n+1) velocity rows, ls2 3 stage-2 stress rows; lpal the MS, the MV and the
-// velocity-constant entries (palette sections PAL_MS, PAL_MV, PAL_CN, kMaxPalette2 entries each).
+// lv2 6 step-(n+1) velocity rows (the first R2 in use), ls2 3 stage-2 stress rows; lpal the MS,
+// the MV and the velocity-constant entries (palette sections PAL_MS, PAL_MV, PAL_CN, kMaxPalette2
+// entries each).
 template <class T, int BW, bool AVG, bool EDGE>
 __device__ __forceinline__ void step2_tile(StepArgs<T> A, const int tile, const long long n, T (*lv)[3][BW + 4],
                                            T (*ls)[3][BW + 4], T (*lv2)[3][BW + 4], T (*ls2)[3][BW + 4],
