@@ -145,7 +145,12 @@ def model_problem(p, seed=LAYOUT_SEED, geom=CONFIG3):
     mask = patchy_mask(n, n, dx, dx, gas, seed, radius, radius) if gas > 0 else None
     rock = dict(BRINE, phi=phi, kappa=kappa)
     fluid = dict(GAS, phi=phi, kappa=kappa)
-    coeffs = material_biot(**biot_fields(n, n, mask, rock, fluid))
+    # the model has two materials: their coefficients (per point, so the same bits as over the
+    # whole grid), then placed by the mask
+    pair = material_biot(**biot_fields(1, 0, np.array([[False, True]]), rock, fluid))
+    shape = (n + 1, n + 1)
+    coeffs = {k: (np.where(mask, v[0, 1], v[0, 0]) if mask is not None else np.full(shape, v[0, 0]))
+              for k, v in pair.items()}
     c = n // 2
     ix, iz = receiver_points(geom)
     rec = ReceiverSpec(ix=ix, iz=iz, decimate=geom["dec"])

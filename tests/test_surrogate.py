@@ -152,3 +152,21 @@ def test_path_inputs_count_the_gas_on_the_segment():
                          for i in range(S.PATH_SAMPLES)])
         assert lay[g["nrec"] + k] == pytest.approx(share)
     assert lay[-1] == float(mask[c, c])
+
+
+def test_model_coefficients_equal_the_whole_grid_evaluation():
+    # a model's two materials, evaluated once and placed by the mask, give the bits the material
+    # front end gives over the whole grid
+    from poe2d_amd.configs import BRINE, GAS, biot_fields
+    from poe2d_amd.material import material_biot
+    g = S.SMALL
+    n = g["n"]
+    for p in list(S.draw_params(3, seed=13)) + [[0.0, 5.0, 0.2, 1e-12]]:
+        gas, radius, phi, kappa = (float(v) for v in p)
+        mask = S.patchy_mask(n, n, g["dx"], g["dx"], gas, S.LAYOUT_SEED, radius, radius) if gas > 0 else None
+        whole = material_biot(**biot_fields(n, n, mask, dict(BRINE, phi=phi, kappa=kappa),
+                                            dict(GAS, phi=phi, kappa=kappa)))
+        built = S.model_problem(p, geom=g).coeffs
+        assert set(built) == set(whole)
+        for k in whole:
+            assert built[k].shape == whole[k].shape and np.array_equal(built[k], whole[k]), k
