@@ -13,9 +13,9 @@ relative L2 (pore pressure at a receiver drops about 40-fold when a gas patch co
 beyond what any surrogate of the parameters could reach.
 
 The network (inputs -> 512 -> 512 -> 512 -> outputs, GELU, bf16 on the matrix cores, 5 members)
-takes the four parameters and, by default, one input per receiver telling whether gas covers it
-(receiver_gas, from the fixed layout), and is trained on gathers the solver itself produced, each
-component scaled by its RMS over the training set.  Screening predicts every candidate; a candidate whose ensemble spread exceeds the threshold
+takes the four parameters and, by default, inputs read off the model's gas mask (layout_inputs:
+gas at each receiver, on its path from the source and at the source), and is trained on gathers
+the solver itself produced, each component scaled by its RMS over the training set.  Screening predicts every candidate; a candidate whose ensemble spread exceeds the threshold
 is re-solved with the batched solver (fallback), and a random 5 % of the accepted ones are
 re-solved as well to judge the surrogate (timed apart from the screening).  Errors are relative L2
 over the whole gather after the per-component scaling; after fallback, a re-solved model's error
@@ -109,11 +109,6 @@ def layout_inputs(params, geom=CONFIG3, workers=8):
         axis=2, dtype=np.float32)
     out[:, 2 * nrec] = samples[:, -1]
     return out
-
-
-def receiver_gas(params, geom=CONFIG3, workers=8):
-    """[n, nrec] float32: 1 where the model's gas patches cover the receiver, else 0."""
-    return layout_inputs(params, geom, workers)[:, :geom["nrec"]]
 
 
 def features(params, geom=CONFIG3, receivers=False):
