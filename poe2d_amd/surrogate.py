@@ -15,9 +15,10 @@ beyond what any surrogate of the parameters could reach.
 The network (inputs -> 512 -> 512 -> 512 -> outputs, GELU, bf16 on the matrix cores, 5 members)
 takes the four parameters and, by default, inputs read off the model's gas mask (layout_inputs:
 gas at each receiver, on its path from the source and at the source), and is trained on gathers
-the solver itself produced, each component scaled by its RMS over the training set.  Screening predicts every candidate; a candidate whose ensemble spread exceeds the threshold
-is re-solved with the batched solver (fallback), and a random 5 % of the accepted ones are
-re-solved as well to judge the surrogate (timed apart from the screening).  Errors are relative L2
+the solver itself produced, each component scaled by its RMS over the training set.  Screening
+predicts every candidate; a candidate whose ensemble spread exceeds the threshold is re-solved
+with the batched solver (fallback), and a random 5 % of the accepted ones are re-solved as well to
+judge the surrogate (timed apart from the screening).  Errors are relative L2
 over the whole gather after the per-component scaling; after fallback, a re-solved model's error
 is zero.
 
