@@ -161,17 +161,19 @@ poe2d_status poe2d_set_receivers(poe2d_solver *s, const poe2d_receivers *rec);
 
 /* Choose how a time step is executed on this handle (no counterpart in the reference, whose
  * six launches per step are fixed at seismic_CPML_2D_poroelastic_withCUDA.cu:1628-1634).
- *   schedule 0 (default): one fused launch per step; tiles of block_w - 8 columns (block_w =
- *     128, 256 or 512 threads, default 256) by tile_rows rows (default 14; see poe2d_autotune).  When the grid holds
- *     at most 256 distinct coefficient tuples (see poe2d_material_palette) the coefficients are
- *     read through a one-byte material index per point.
+ *   schedule 0 (default of FP64 handles): one fused launch per step; tiles of block_w - 8 columns
+ *     (block_w = 128, 256 or 512 threads, default 256) by tile_rows rows (default 14; see
+ *     poe2d_autotune).  When the grid holds at most 256 distinct coefficient tuples (see
+ *     poe2d_material_palette) the coefficients are read through a one-byte material index per
+ *     point.
  *   schedule 1: a stress launch and a velocity launch per step, intermediates through memory.
  *   schedule 2: as schedule 0, always reading the per-point coefficient arrays.
- *   schedule 3: two time steps per launch (FP32 grids with a material palette of at most 28
- *     entries; tiles of block_w - 12 columns, block_w 64, 128 or 256, by tile_rows rows,
- *     multiples of 6 best), the intermediate state kept on chip; an odd last step of a run takes
- *     one schedule-0 launch.  FP64 handles and grids with no palette or a larger one run
- *     schedule 0 (block_w 64 as 128), and poe2d_autotune then tunes that kernel.
+ *   schedule 3 (default of FP32 handles, block_w 256, tile_rows 24): two time steps per launch
+ *     (FP32 grids with a material palette of at most 28 entries; tiles of block_w - 12 columns,
+ *     block_w 64, 128 or 256, by tile_rows rows, multiples of 6 best), the intermediate state kept
+ *     on chip; an odd last step of a run takes one schedule-0 launch.  FP64 handles and grids with
+ *     no palette or a larger one run schedule 0 (block_w 64 as 128), and poe2d_autotune then
+ *     tunes that kernel.
  * Tile heights do not change a bit; schedules and block widths agree to rounding.  The choice
  * only changes speed. */
 poe2d_status poe2d_set_schedule(poe2d_solver *s, int schedule, int block_w, int tile_rows);

@@ -833,6 +833,12 @@ poe2d_solver *new_handle(Model *md, poe2d_status *st)
     s->precision = md->precision;
     s->rs = md->rs;
     s->grid = md->grid;
+    if (s->precision == POE2D_F32) {
+        // FP32 handles start on schedule 3: the two-step kernel where the grid's palette fits it,
+        // the single-step kernel elsewhere, the same results either way
+        s->schedule = SCHED_FUSED2;
+        s->tile_rows = 24;
+    }
     auto bail = [&](poe2d_status e) -> poe2d_solver * {
         *st = e;
         free_handle(s);
