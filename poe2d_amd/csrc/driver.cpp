@@ -37,6 +37,7 @@ struct Options {
     std::string input1 = "input1.txt", input2 = "input2.txt", outdir = ".";
     int precision = 32;
     std::string tile = "auto";
+    int schedule = -1;  // -1: the handle's default (3 for FP32, 0 for FP64)
     bool quiet = false;
     int stress_law = POE2D_STRESS_AS_WRITTEN;
     int material = POE2D_MATERIAL_AS_WRITTEN;
@@ -69,7 +70,8 @@ void usage()
         "  --samp N             snapshot interval in steps (5000)\n"
         "  --outdir D           where the output files go (.)\n"
         "  --precision 32|64    arithmetic of the time loop (32)\n"
-        "  --tile auto|BWxTH    tile shape of the step kernel (auto: autotune)\n"
+        "  --schedule 0|1|2|3   step schedule (poe2d_set_schedule; default 3 in FP32, 0 in FP64)\n"
+        "  --tile auto|BWxTH    tile shape of the schedule's kernel (auto: autotune)\n"
         "  --stress-law written|drained   lambda_c as the reference writes it, or Biot's drained\n"
         "                       lambda in the normal stresses (written)\n"
         "  --material written|averaged    staggered-point material as written, or harmonic mu for\n"
@@ -110,6 +112,7 @@ bool parse(int argc, char **argv, Options &o)
         else if (k == "--outdir") o.outdir = val();
         else if (k == "--precision") o.precision = std::atoi(val());
         else if (k == "--tile") o.tile = val();
+        else if (k == "--schedule") o.schedule = std::atoi(val());
         else if (k == "--quiet") o.quiet = true;
         else if (k == "--stress-law") {
             const std::string v = val();
@@ -269,13 +272,19 @@ int main(int argc, char **argv)
     rec.nsamp = nstep;
     if (poe2d_set_receivers(s, &rec) != POE2D_OK) return die("receivers");
 
+    const int sched = o.schedule >= 0 ? o.schedule : (o.precision == 64 ? 0 : 3);
     if (o.tile == "auto") {
         int bw = 0, th = 0;
-        if (poe2d_autotune(s, 5, &bw, &th) != POE2D_OK) return die("autotune");
-        if (!o.quiet) std::printf("tile %dx%d (autotuned)\n", bw, th);
+        if (poe2d_set_schedule(s, sched, 256, sched == 3 ? 24 : 14) != POE2D_OK) return die("schedule");
+        if (sched == 1) {
+            if (!o.quiet) std::printf("two-pass schedule (no tile shape)\n");
+        } else {
+            if (poe2d_autotune(s, 5, &bw, &th) != POE2D_OK) return die("autotune");
+            if (!o.quiet) std::printf("schedule %d, tile %dx%d (autotuned)\n", sched, bw, th);
+        }
     } else {
         int bw = 0, th = 0;
-        if (std::sscanf(o.tile.c_str(), "%dx%d", &bw, &th) != 2 || poe2d_set_schedule(s, 0, bw, th) != POE2D_OK)
+        if (std::sscanf(o.tile.c_str(), "%dx%d", &bw, &th) != 2 || poe2d_set_schedule(s, sched, bw, th) != POE2D_OK)
             return die("tile");
     }
 
