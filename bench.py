@@ -86,15 +86,43 @@ def recorded_traffic(tile, n, schedule=0):
         return None, None
 
 
-def cpu_baseline(prob, steps):
-    from oracle.pyoracle import OracleSolver
-    s = OracleSolver(prob, threads=1)
-    s.set_reference_scans(122, 680, prob.nz // 2 + 100, prob.nz // 2 + 200)
+def trace_rel_l2(g, o, floor=1e-3):
+    """Per-trace relative L2 distance ||g - o|| / max(||o||, floor x the largest trace norm of the
+    component) over the last axis; g, o shaped [ncomp, ntrace, nsamp]."""
+    g = np.asarray(g, np.float64)
+    o = np.asarray(o, np.float64)
+    num = np.linalg.norm(g - o, axis=-1)
+    den = np.linalg.norm(o, axis=-1)
+    return num / np.maximum(np.maximum(den, floor * den.max(axis=-1, keepdims=True)), 1e-300)
+
+
+def compulsory_bytes_per_step(npts, strip_points, schedule, palette):
+    """Bytes per time step that the schedule cannot avoid: each launch reads and writes the V and S
+    groups (64 B per point, FP32) and reads the one-byte material index (palette) or both coefficient
+    groups (32 B); the two-step kernel does this once per two steps.  Every step reads and writes the
+    six memory variables of each strip record (48 B per strip point and strip)."""
+    per_point = 64.0 + (1.0 if palette else 32.0)
+    per_launch_steps = 2 if schedule == 3 else 1
+    return per_point * npts / per_launch_steps + 48.0 * strip_points
+
+
+def cpu_leg(prob, n, steps, gpu_traces):
+    """The oracle on the host: (1) the CPU baseline, its reference-structured form (five sweeps plus
+    three full-grid receiver scans per step, one thread) timed over `steps` steps of the same model;
+    (2) as the checker, the first steps of the GPU's source run (gpu_traces, [3, nrec, k]) against the
+    FP32 oracle run for k steps on all host threads.  Returns (baseline, parity or None)."""
+    from oracle.pyoracle import OracleSolver, material_biot
+    from poe2d_amd.configs import config1
+
+    cprob = config1(nt=steps + 1, n=n, material_fn=material_biot)
+    s = OracleSolver(cprob, threads=1)
+    s.set_reference_scans(122, 680, cprob.nz // 2 + 100, cprob.nz // 2 + 200)
     s.run(1)  # untimed first step (page-in)
     t = time.perf_counter()
     s.run(steps)
     dt = time.perf_counter() - t
-    npts = (prob.nx + 1) * (prob.nz + 1)
+    s.close()
+    npts = (cprob.nx + 1) * (cprob.nz + 1)
     cpu_model = ""
     try:
         for line in open("/proc/cpuinfo"):
@@ -103,10 +131,29 @@ def cpu_baseline(prob, steps):
                 break
     except OSError:
         pass
-    return {"value": npts * steps / dt / 1e9, "unit": "Gcell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"{steps} time steps of the {prob.nx + 1}x{prob.nz + 1} configuration-1 model "
-                      f"({dt:.1f} s), reference-structured oracle (5 sweeps + 3 full-grid receiver scans), "
-                      f"1 thread; host {cpu_model or platform.processor()}, nproc {os.cpu_count()}"}
+    baseline = {"value": npts * steps / dt / 1e9, "unit": "Gcell-updates/s", "cores": 1, "kind": "port",
+                "sample": f"{steps} time steps of the {cprob.nx + 1}x{cprob.nz + 1} configuration-1 model "
+                          f"({dt:.1f} s), reference-structured oracle (5 sweeps + 3 full-grid receiver scans), "
+                          f"1 thread; host {cpu_model or platform.processor()}, nproc {os.cpu_count()}"}
+    if gpu_traces is None:
+        return baseline, None
+    k = gpu_traces.shape[-1]
+    threads = min(16, os.cpu_count() or 1)
+    t = time.perf_counter()
+    o = OracleSolver(prob, threads=threads)  # the GPU's own problem: identical coefficients
+    o.run(k)
+    o32 = o.seismograms()[..., :k]
+    o.close()
+    err = trace_rel_l2(gpu_traces, o32)
+    reached = int((np.linalg.norm(o32, axis=-1) > 1e-3 * np.linalg.norm(o32, axis=-1).max(axis=-1, keepdims=True)).sum())
+    print(f"[bench] oracle {k} steps on {threads} threads in {time.perf_counter() - t:.1f} s: max per-trace "
+          f"rel L2 {err.max():.3e} ({reached} traces above 1e-3 of their component's largest)",
+          file=sys.stderr, flush=True)
+    return baseline, {"max": float(err.max()),
+                      "what": f"max over the 801 x 3 traces of the per-trace relative L2 distance between the GPU "
+                              f"and the FP32 oracle (the reference's CPU loop restated) over the first {k} steps "
+                              f"of the source run ({reached} traces carry signal by then); oracle on {threads} "
+                              f"host threads"}
 
 
 def copy_probe(mib, reps=20):
@@ -125,27 +172,18 @@ def copy_probe(mib, reps=20):
     ms = e0.elapsed_time(e1) / reps
     moved = 2 * n * 4
     del src, dst
-    import ctypes
-    from poe2d_amd import _abi
-    lib = _abi.load_library()
-    best = [ctypes.c_double() for _ in range(3)]
-    _abi.check(lib, lib.poe2d_stream_probe(mib << 20, 10, *[ctypes.byref(b) for b in best]), "stream_probe")
     print(json.dumps({"copy_probe_mib": mib, "bytes_read_per_copy": n * 4, "bytes_written_per_copy": n * 4,
-                      "ms_per_copy": round(ms, 4), "GBps_read_plus_write": round(moved / ms / 1e6, 1),
-                      "stream_probe_GBps": {"copy": round(best[0].value, 1), "read": round(best[1].value, 1),
-                                            "write": round(best[2].value, 1)}}), flush=True)
+                      "ms_per_copy": round(ms, 4), "GBps_read_plus_write": round(moved / ms / 1e6, 1)}), flush=True)
 
 
-def batch_bench(args):
-    """Configuration 2: batch A (64 Ricker shots over one 1201^2 patchy model, clones sharing the
-    model) or batch B (32 patchy models, sides uniform in [300, 1500] points), FP32, advanced in
-    lock-step by one launch per time step with the loop in HIP graphs; tile shape autotuned.
-    value = sum over members of points x nt / device time of one batch run."""
+def build_batch(which, nt):
+    """Configuration 2's batch A (64 Ricker shots over one 1201^2 patchy model, clones sharing the
+    model) or batch B (32 patchy models, sides uniform in [300, 1500] points): (members, workload,
+    algorithmic bytes per point-update, points)."""
     from poe2d_amd.configs import config2a, config2b
-    from poe2d_amd.solver import Batch, Solver
+    from poe2d_amd.solver import Solver
 
-    nt = args.nt if args.nt_given else 3000
-    if args.workload == "batchA":
+    if which == "batchA":
         base, shots, rec = config2a(nt=nt)
         first = Solver(base)
         members = [first] + [first.clone(source=sh, receivers=rec, nt=nt) for sh in shots[1:]]
@@ -158,10 +196,18 @@ def batch_bench(args):
         workload = f"biot-cpml {len(probs)} patchy models, sides U[300,1500], gas U[0,0.3], {nt} steps, 101 receivers each"
         bytes_per_point = BYTES_PER_POINT
     npts = sum((m.problem.nx + 1) * (m.problem.nz + 1) for m in members)
+    return members, workload, bytes_per_point, npts
+
+
+def time_batch(members, nt, warmup, reps, schedules=(0, 3), label="batch"):
+    """Autotune the batch on each schedule (one and two steps per launch), keep the faster, then
+    time `reps` runs of nt steps after `warmup` untimed ones (device time, hipEvents on the batch's
+    stream).  Returns (median ms per run, schedule, tile, traces finite)."""
+    from poe2d_amd.solver import Batch
+
     with Batch(members) as b:
-        # autotune the batch on each kernel (one and two steps per launch), keep the faster
         best = None
-        for sched in (0, 3):
+        for sched in schedules:
             for m in members:
                 m.reset()
                 m.set_schedule(sched, 256, 12 if sched == 3 else 8)
@@ -170,7 +216,7 @@ def batch_bench(args):
             b.set_schedule(*t)
             b.run(20)
             ms, n = b.last_run_ms()
-            print(f"[bench] batch schedule {sched}: autotuned {t[0]}x{t[1]}, {ms / n * 1e3:.1f} us per step",
+            print(f"[bench] {label} schedule {sched}: autotuned {t[0]}x{t[1]}, {ms / n * 1e3:.1f} us per step",
                   file=sys.stderr, flush=True)
             if best is None or ms / n < best[0]:
                 best = (ms / n, sched, t)
@@ -185,13 +231,25 @@ def batch_bench(args):
             b.run(nt)
             return b.last_run_ms()[0]
 
-        for _ in range(args.warmup):
+        for _ in range(warmup):
             one_run()
-        dev_ms = [one_run() for _ in range(args.steps)]
+        dev_ms = [one_run() for _ in range(reps)]
         finite = all(bool(np.isfinite(m.seismograms()).all()) for m in members)
+    return statistics.median(dev_ms), schedule, tile, finite
+
+
+SCHEDULE_NAMES = {0: "one step per launch", 3: "two steps per launch"}
+
+
+def batch_bench(args):
+    """Configuration 2: batch A or batch B, FP32, advanced in lock-step by one work-list launch per
+    time step (per two steps on the two-step kernel) with the loop in HIP graphs; tile shape and
+    kernel autotuned.  value = sum over members of points x nt / device time of one batch run."""
+    nt = args.nt if args.nt_given else 3000
+    members, workload, bytes_per_point, npts = build_batch(args.workload, nt)
+    ms, schedule, tile, finite = time_batch(members, nt, args.warmup, args.steps, label=args.workload)
     for m in members:
         m.close()
-    ms = statistics.median(dev_ms)
     gcell = npts * nt / (ms / 1e3) / 1e9
     achieved = npts * bytes_per_point / (ms / 1e3 / nt) / 1e9
     print(json.dumps({
@@ -201,7 +259,7 @@ def batch_bench(args):
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": workload, "members": len(members), "points": npts, "nt": nt,
                    "tile": f"{tile[0]}x{tile[1]} (autotuned)", "step": "one batch run of nt steps",
-                   "schedule": {0: "one step per launch", 3: "two steps per launch"}[schedule],
+                   "schedule": SCHEDULE_NAMES[schedule],
                    "launch": "one work-list launch per time step (per two steps with the two-step kernel), "
                              "32-step HIP graphs", "traces_finite": finite},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -211,7 +269,55 @@ def batch_bench(args):
     }), flush=True)
 
 
+def secondary_batches(nt, reps=2):
+    """Configuration 2's batches, timed briefly for the default line: nt steps per run, one warm-up
+    run, median of `reps`."""
+    out = {}
+    for which in ("batchA", "batchB"):
+        members, workload, bytes_per_point, npts = build_batch(which, nt)
+        ms, schedule, tile, finite = time_batch(members, nt, 1, reps, label=which)
+        for m in members:
+            m.close()
+        step_s = ms / 1e3 / nt
+        out[which] = {"workload": workload, "members": len(members), "points": npts, "nt": nt,
+                      "gcell_updates_per_s": round(npts / step_s / 1e9, 2), "us_per_step": round(step_s * 1e6, 1),
+                      "schedule": SCHEDULE_NAMES[schedule], "tile": f"{tile[0]}x{tile[1]} (autotuned)",
+                      "algorithmic_B_per_point": round(bytes_per_point, 2),
+                      "frac": round(npts * bytes_per_point / step_s / 1e9 / HBM_PEAK_GBS, 4),
+                      "traces_finite": finite}
+    return out
+
+
 MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def network_timing(receivers=True):
+    """One ensemble member of the screening network at its full shape (207 -> 512 -> 512 -> 512 ->
+    151 500 with the layout inputs, 4 -> ... without; bf16 on the matrix cores), timed with hipEvents:
+    one forward pass over an inference batch of 1024 and one training step (forward, backward,
+    Adam) over a batch of 256.  FLOPs are 2 x the MACs per sample forward and 3 x that per training
+    sample.  Returns (roofline dict, layer sizes)."""
+    from poe2d_amd import surrogate as S
+    from poe2d_amd.mlp import Network
+
+    geom = S.CONFIG3
+    nin = 4 + (2 * geom["nrec"] + 1 if receivers else 0)
+    sizes = [nin, 512, 512, 512, int(np.prod(S.gather_shape(geom)))]
+    macs = sum(a * b for a, b in zip(sizes[:-1], sizes[1:]))
+    with Network(sizes, members=1, max_batch=1024) as net:
+        net.time(256, 2)
+        fwd_ms, _ = net.time(1024, 10)
+        _, train_ms = net.time(256, 10)
+    fwd_tflops = 2 * macs * 1024 / (fwd_ms / 1e3) / 1e12
+    train_tflops = 3 * 2 * macs * 256 / (train_ms / 1e3) / 1e12
+    print(f"[bench] network {'-'.join(map(str, sizes))}: forward(1024) {fwd_ms:.3f} ms = {fwd_tflops:.0f} TFLOP/s; "
+          f"train step(256) {train_ms:.3f} ms = {train_tflops:.0f} TFLOP/s", file=sys.stderr, flush=True)
+    return ({"bound": "mfma", "achieved": round(fwd_tflops, 1), "peak": MFMA_BF16_PEAK_TFLOPS,
+             "unit": "TFLOP/s", "frac": round(fwd_tflops / MFMA_BF16_PEAK_TFLOPS, 4), "traffic": None,
+             "per_launch": f"one member's forward pass over 1024 samples: 2 x {macs / 1e6:.2f} M MACs each",
+             "train_step_tflops": round(train_tflops, 1),
+             "train_step_frac": round(train_tflops / MFMA_BF16_PEAK_TFLOPS, 4),
+             "forward_ms": round(fwd_ms, 4), "train_step_ms": round(train_ms, 4)}, sizes)
 
 
 def surrogate_bench(args):
@@ -229,27 +335,10 @@ def surrogate_bench(args):
     spread-triggered re-solves, the models/s of the surrogate) and re-solves a random 5 % of the
     accepted ones to judge it (timed apart)."""
     from poe2d_amd import surrogate as S
-    from poe2d_amd.mlp import Network
 
     geom = S.CONFIG3
-    nin = 4 + (2 * geom["nrec"] + 1 if args.sur_receivers else 0)
-    sizes = [nin, 512, 512, 512, int(np.prod(S.gather_shape(geom)))]
-    macs = sum(a * b for a, b in zip(sizes[:-1], sizes[1:]))
-    with Network(sizes, members=1, max_batch=1024) as net:
-        net.time(256, 2)
-        fwd_ms, _ = net.time(1024, 10)
-        _, train_ms = net.time(256, 10)
-    fwd_tflops = 2 * macs * 1024 / (fwd_ms / 1e3) / 1e12
-    train_tflops = 3 * 2 * macs * 256 / (train_ms / 1e3) / 1e12
-    print(f"[surrogate] network: forward(1024) {fwd_ms:.3f} ms = {fwd_tflops:.0f} TFLOP/s; "
-          f"train step(256) {train_ms:.3f} ms = {train_tflops:.0f} TFLOP/s", file=sys.stderr, flush=True)
-
-    net_roofline = {"bound": "mfma", "achieved": round(fwd_tflops, 1), "peak": MFMA_BF16_PEAK_TFLOPS,
-                    "unit": "TFLOP/s", "frac": round(fwd_tflops / MFMA_BF16_PEAK_TFLOPS, 4), "traffic": None,
-                    "per_launch": f"one member's forward pass over 1024 samples: 2 x {macs / 1e6:.2f} M MACs each",
-                    "train_step_tflops": round(train_tflops, 1),
-                    "train_step_frac": round(train_tflops / MFMA_BF16_PEAK_TFLOPS, 4),
-                    "forward_ms": round(fwd_ms, 4), "train_step_ms": round(train_ms, 4)}
+    net_roofline, sizes = network_timing(bool(args.sur_receivers))
+    fwd_ms, fwd_tflops = net_roofline["forward_ms"], net_roofline["achieved"]
     if args.sur_train == 0:
         print(json.dumps({
             "metric": "surrogate network TFLOP/s (bf16 forward pass, batch 1024)", "value": round(fwd_tflops, 1),
@@ -344,7 +433,15 @@ def main():
     ap.add_argument("--sur-quantiles", default="0.5,0.99",
                     help="surrogate: error quantiles after fallback held within 5 %% by the threshold, one "
                          "screening pass each (the first gives the line's value)")
-    ap.add_argument("--cpu-steps", type=int, default=12, help="time steps of the CPU baseline sample (0: skip)")
+    ap.add_argument("--cpu-steps", type=int, default=12,
+                    help="time steps of the CPU baseline sample (0: skip the CPU leg, the oracle comparison "
+                         "included)")
+    ap.add_argument("--parity-steps", type=int, default=200,
+                    help="steps of the source run compared with the oracle on all host threads (0: skip)")
+    ap.add_argument("--fp64", type=int, default=1, help="1: solve again in FP64 and report the FP32 error")
+    ap.add_argument("--secondary", type=int, default=1,
+                    help="1: add configuration 2's batches (briefly) and the network timing to the line")
+    ap.add_argument("--batch-nt", type=int, default=300, help="steps per run of the secondary batch timings")
     ap.add_argument("--tile", default="auto", metavar="BWxTH",
                     help="tile shape of the step kernel (block width x tile rows); 'auto' runs the "
                          "library's autotuner first")
@@ -353,8 +450,7 @@ def main():
                          "faster after autotuning both")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="instead of the benchmark: time a plain device copy of MIB MiB (torch; the copy "
-                         "that scales the FETCH_SIZE / WRITE_SIZE counters), then the library's stream "
-                         "probe (best copy / read / write rates over MIB MiB buffers)")
+                         "that scales the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     args.nt_given = args.nt is not None
     if args.nt is None:
@@ -373,7 +469,7 @@ def main():
         batch_bench(args)
         return
 
-    from poe2d_amd import F32
+    from poe2d_amd import F32, F64
     from poe2d_amd.configs import config1
     from poe2d_amd.solver import Solver
 
@@ -422,13 +518,32 @@ def main():
         dev_ms.append(ms)
     seis = solver.seismograms()
     finite = bool(np.isfinite(seis).all())
+    parity_traces = None
+    if args.parity_steps > 0:
+        # the first parity_steps steps again, for the comparison with the oracle below
+        solver.reset()
+        solver.run(args.parity_steps)
+        parity_traces = solver.seismograms()[..., :args.parity_steps]
     solver.close()
+
+    # the same solve in FP64 (the FP64 build of the same kernels): the error the FP32 result carries
+    fp64_err = None
+    if args.fp64:
+        with Solver(prob, precision=F64) as s64:
+            s64.run(args.nt)
+            seis64 = s64.seismograms()
+            fp64_ms = s64.last_run_ms()[0]
+        fp64_err = float(trace_rel_l2(seis, seis64).max())
+        print(f"[bench] FP64 solve {fp64_ms:.1f} ms; FP32 vs FP64 max per-trace rel L2 {fp64_err:.3e}",
+              file=sys.stderr, flush=True)
+        del seis64
 
     ms_solve = statistics.median(dev_ms)
     gcell = npts * args.nt / (ms_solve / 1e3) / 1e9
     step_ms = ms_solve / args.nt
     achieved = bytes_step / (step_ms / 1e3) / 1e9
     traffic, traffic_src = recorded_traffic(tile, args.n, schedule)
+    compulsory = compulsory_bytes_per_step(npts, cols * (prob.nz + 1) + rows * (prob.nx + 1), schedule, palette > 0)
     out = {
         "metric": "Gcell-updates/s (FP32 Biot C-PML time loop, 4001x4001, 6000 steps)",
         "value": round(gcell * 1.0, 3),
@@ -442,17 +557,23 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
+        "fp64_rel_l2_max": None if fp64_err is None else float(f"{fp64_err:.4g}"),
+        "parity_rel_l2_max": None,
         "config": {"workload": prob.meta["workload"] + f", {args.nt} steps, "
                                f"{prob.receivers.n} receivers x (vx, vy, p)",
                    "grid": [prob.nx + 1, prob.nz + 1], "nt": args.nt, "npml": prob.npml,
                    "gas_fraction": prob.meta.get("gas_fraction_realised"),
                    "step": "one full solve", "wall_ms_median": round(statistics.median(wall_ms), 3),
                    "traces_finite": finite, "tile": f"{tile[0]}x{tile[1]} ({tile_note})",
-                   "schedule": {0: "one step per launch", 3: "two steps per launch"}[schedule],
-                   "material_palette": palette},
+                   "schedule": SCHEDULE_NAMES[schedule], "material_palette": palette,
+                   "fp64_rel_l2_max": "max over the 801 x 3 traces of the per-trace relative L2 distance between "
+                                      "this FP32 solve and the FP64 build's solve of the same model, all 6000 steps"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None if traffic is None else round(traffic),
+                     "frac_measured": None if traffic is None else round(traffic / (step_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "frac_compulsory": round(compulsory / (step_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "compulsory_bytes_per_step": round(compulsory),
                      "kernel": ("fused_step (one launch per time step, 32-step HIP graphs)" if schedule == 0 else
                                 "fused_step2 (one launch per two time steps, 32-step HIP graphs); per step: "
                                 "half a launch"),
@@ -463,15 +584,22 @@ def main():
                                     f"{rows} strip rows)"),
                      "hbm_gbs": None if traffic is None else round(traffic / (step_ms / 1e3) / 1e9, 1),
                      "traffic_source": traffic_src,
-                     "note": ("algorithmic bytes: 104 B per cell-update (+ C-PML strips) whatever the schedule; "
-                              "the two-step kernel performs two cell-updates per point for each pass over HBM, "
-                              "so frac above 1 is its gain over the one-pass-per-step roofline, and hbm_gbs is "
-                              "the measured traffic rate") if schedule == 3 else None},
+                     "note": ("frac: algorithmic bytes, 104 B per cell-update (+ C-PML strips) whatever the "
+                              "schedule (the two-step kernel performs two cell-updates per point for each pass "
+                              "over HBM, so it can pass 1); frac_measured: the HBM traffic the counters read per "
+                              "step (traffic) over the same time; frac_compulsory: the bytes this schedule cannot "
+                              "avoid per step (state read and written once per launch, the one-byte material "
+                              "index or the coefficients, C-PML records) over the same time")},
     }
+    if args.secondary:
+        out["secondary"] = {"configuration_2": secondary_batches(args.batch_nt),
+                            "surrogate_network": network_timing(True)[0]}
     if args.cpu_steps > 0:
-        from oracle.pyoracle import material_biot
-        cprob = config1(nt=args.cpu_steps + 1, n=args.n, material_fn=material_biot)
-        out["cpu_baseline"] = cpu_baseline(cprob, args.cpu_steps)
+        baseline, parity = cpu_leg(prob, args.n, args.cpu_steps, parity_traces)
+        out["cpu_baseline"] = baseline
+        if parity is not None:
+            out["parity_rel_l2_max"] = float(f"{parity['max']:.4g}")
+            out["config"]["parity_rel_l2_max"] = parity["what"]
     print(json.dumps(out), flush=True)
 
 

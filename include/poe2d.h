@@ -205,6 +205,13 @@ poe2d_status poe2d_seismograms(poe2d_solver *s, void *out, size_t out_bytes);
  * Replaces the per-chunk D2H of vz/vx (:1642-1645) behind the .vzo/.vxo snapshots. */
 poe2d_status poe2d_snapshot(poe2d_solver *s, int field, void *out, size_t out_bytes);
 
+/* Overwrite one state field (VX .. SXY; ZETA is the total fluid content) of the current state
+ * from (nz+1) rows of (nx+1) values in the handle's precision: an initial condition other than
+ * the reference's zero state (zero_kernel, :387-422).  Points the reference's updates never
+ * reach keep what is set here, as in the reference's arrays; fields derived from the strains (p,
+ * sxx, syy) are refused.  No counterpart in the reference's interface. */
+poe2d_status poe2d_set_field(poe2d_solver *s, int field, const void *in, size_t in_bytes);
+
 /* Full state (wavefield, C-PML memory, step counter, recorded traces) for bitwise
  * checkpoint/resume.  No counterpart in the reference. */
 poe2d_status poe2d_checkpoint_size(poe2d_solver *s, size_t *bytes);
@@ -316,13 +323,6 @@ poe2d_status poe2d_patchy_mask(int nx, int nz, double dx, double dz, double gas_
  * (radius r points, rows x cols layer, 1 inside a disk).  type = 1..6. */
 poe2d_status poe2d_patchy_lattice(int type, int rows, int cols, int r, uint8_t *layer);
 
-/* Achievable bandwidth of plain streams on the current device (a measurement helper with no
- * counterpart in the reference): grid-stride float4 copy, read and write kernels over buffers of
- * `bytes`, best of several grid sizes, each timed over `reps` launches; GB/s counts bytes read
- * plus bytes written. */
-poe2d_status poe2d_stream_probe(size_t bytes, int reps, double *copy_gbps, double *read_gbps,
-                                double *write_gbps);
-
 /* ---- Surrogate network (no counterpart in the reference) ----
  * An ensemble of fully connected networks sizes[0] -> ... -> sizes[nlayers] with GELU (erf form)
  * between layers, bf16 weights and activations, FP32 accumulation on the matrix cores, FP32 master
@@ -336,6 +336,9 @@ poe2d_status poe2d_mlp_create(int nlayers, const int *sizes, int members, int ma
                               poe2d_mlp **out);
 poe2d_status poe2d_mlp_get_params(poe2d_mlp *m, int member, int layer, float *w, float *b);
 poe2d_status poe2d_mlp_set_params(poe2d_mlp *m, int member, int layer, const float *w, const float *b);
+/* The bf16 copies of a layer's weights that the products read: w16[out][in] (forward) and its
+ * transpose w16t[in][out] (the backward G W product); either may be null. */
+poe2d_status poe2d_mlp_get_bf16(poe2d_mlp *m, int member, int layer, uint16_t *w16, uint16_t *w16t);
 /* y[n][sizes[nlayers]] = network `member` applied to x[n][sizes[0]]. */
 poe2d_status poe2d_mlp_forward(poe2d_mlp *m, int member, int n, const float *x, float *y);
 /* Loss and gradients (gw[l]: [out][in], gb[l]: [out]; null entries skipped) of member on a batch

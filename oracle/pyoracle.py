@@ -66,12 +66,6 @@ class OracleSolver(HandleSolver):
         n = threads if threads is not None else min(16, os.cpu_count() or 1)
         self._check(self._lib.poe2d_oracle_set_threads(self._h, int(n)), "set_threads")
 
-    def set_field(self, fieldname, values):
-        f = _abi.FIELD_NAMES[fieldname] if isinstance(fieldname, str) else fieldname
-        a = np.ascontiguousarray(values, dtype=self.real)
-        assert a.shape == (self.problem.nz + 1, self.problem.nx + 1)
-        self._check(self._lib.poe2d_oracle_set_field(self._h, f, a.ctypes.data, a.nbytes), "set_field")
-
     def set_reference_scans(self, panj, panj2, a0, a1):
         self._check(self._lib.poe2d_oracle_set_reference_scans(self._h, panj, panj2, a0, a1),
                     "set_reference_scans")

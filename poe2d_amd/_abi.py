@@ -130,10 +130,10 @@ def _declare(lib):
         "poe2d_batch_last_run_ms": ([vp, c_double_p, c_int_p], i),
         "poe2d_batch_autotune": ([vp, i, c_int_p, c_int_p], i),
         "poe2d_batch_destroy": ([vp], None),
-        "poe2d_stream_probe": ([sz, i, c_double_p, c_double_p, c_double_p], i),
         "poe2d_mlp_create": ([i, c_int_p, i, i, ctypes.c_uint64, ctypes.POINTER(vp)], i),
         "poe2d_mlp_get_params": ([vp, i, i, c_float_p, c_float_p], i),
         "poe2d_mlp_set_params": ([vp, i, i, c_float_p, c_float_p], i),
+        "poe2d_mlp_get_bf16": ([vp, i, i, vp, vp], i),
         "poe2d_mlp_forward": ([vp, i, i, c_float_p, c_float_p], i),
         "poe2d_mlp_gradients": ([vp, i, i, c_float_p, c_float_p, c_float_p, ctypes.POINTER(c_float_p),
                                  ctypes.POINTER(c_float_p)], i),
@@ -153,6 +153,7 @@ def _declare(lib):
         "poe2d_step_count": ([vp, c_int_p], i),
         "poe2d_seismograms": ([vp, vp, sz], i),
         "poe2d_snapshot": ([vp, i, vp, sz], i),
+        "poe2d_set_field": ([vp, i, vp, sz], i),
         "poe2d_checkpoint_size": ([vp, ctypes.POINTER(sz)], i),
         "poe2d_checkpoint": ([vp, vp, sz], i),
         "poe2d_restore": ([vp, vp, sz], i),

@@ -26,6 +26,10 @@ void launch_derived(const StepArgs<T> &A, int buf, int what, T *out, hipStream_t
 template <class T>
 void launch_extract(const StepArgs<T> &A, int g, int comp, T *out, hipStream_t stream);
 
+// A dense (nz+1) x (nx+1) array into component `comp` of group `g` (pads untouched).
+template <class T>
+void launch_insert(const StepArgs<T> &A, int g, int comp, const T *in, hipStream_t stream);
+
 // Velocity-update constants (biot_velocity_cn) of the npal MV entries of an FP32 palette into its
 // PAL_CN section.
 void launch_cn_palette(Vec4<float> *pal, int npal, float inv_dt, hipStream_t stream);

@@ -63,6 +63,20 @@ __global__ void extract_kernel(StepArgs<T> A, int g, int comp, T *out)
 }
 
 template <class T>
+__global__ void insert_kernel(StepArgs<T> A, int g, int comp, const T *in)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int z = blockIdx.y;
+    if (x > A.nx || z > A.nz) return;
+    Vec4<T> &v = state_group(A, g)[(long long)z * A.pitch + x];
+    const T a = in[(size_t)z * (A.nx + 1) + x];
+    if (comp == 0) v.x = a;
+    else if (comp == 1) v.y = a;
+    else if (comp == 2) v.z = a;
+    else v.w = a;
+}
+
+template <class T>
 __global__ void __launch_bounds__(256) absmax_blocks(StepArgs<T> A, int buf, float *partial)
 {
     __shared__ float red[256];
@@ -120,6 +134,12 @@ void launch_extract(const StepArgs<T> &A, int g, int comp, T *out, hipStream_t s
 }
 
 template <class T>
+void launch_insert(const StepArgs<T> &A, int g, int comp, const T *in, hipStream_t stream)
+{
+    insert_kernel<T><<<dim3((A.nx + 1 + 255) / 256, A.nz + 1), 256, 0, stream>>>(A, g, comp, in);
+}
+
+template <class T>
 void launch_absmax(const StepArgs<T> &A, int buf, float *partials, int nblocks, float *result, hipStream_t stream)
 {
     absmax_blocks<T><<<nblocks, 256, 0, stream>>>(A, buf, partials);
@@ -147,6 +167,7 @@ void launch_cn_palette(Vec4<float> *pal, int npal, float inv_dt, hipStream_t str
     template void launch_gather<T>(const StepArgs<T> &, int, int, const Receivers &, T *, hipStream_t); \
     template void launch_derived<T>(const StepArgs<T> &, int, int, T *, hipStream_t);                 \
     template void launch_extract<T>(const StepArgs<T> &, int, int, T *, hipStream_t);                 \
+    template void launch_insert<T>(const StepArgs<T> &, int, int, const T *, hipStream_t);            \
     template void launch_absmax<T>(const StepArgs<T> &, int, float *, int, float *, hipStream_t);
 POE2D_AUX_INSTANTIATE(float)
 POE2D_AUX_INSTANTIATE(double)

@@ -401,6 +401,20 @@ poe2d_status poe2d_mlp_get_params(poe2d_mlp *m, int member, int layer, float *w,
     return POE2D_OK;
 }
 
+poe2d_status poe2d_mlp_get_bf16(poe2d_mlp *m, int member, int layer, uint16_t *w16, uint16_t *w16t)
+{
+    poe2d_status st = check_mlp(m);
+    if (st || (st = check_member(m, member))) return st;
+    if (layer < 0 || layer >= m->L) return fail(POE2D_E_INVALID, "mlp_get_bf16: no such layer");
+    Layer &l = m->members[member].layers[layer];
+    HIP_TRY(hipStreamSynchronize(m->stream));
+    if (w16) HIP_TRY(hipMemcpy2D(w16, l.in * sizeof(uint16_t), l.w16, l.inp * sizeof(uint16_t), l.in * sizeof(uint16_t),
+                                 l.out, hipMemcpyDeviceToHost));
+    if (w16t) HIP_TRY(hipMemcpy2D(w16t, l.out * sizeof(uint16_t), l.w16t, l.outp * sizeof(uint16_t),
+                                  l.out * sizeof(uint16_t), l.in, hipMemcpyDeviceToHost));
+    return POE2D_OK;
+}
+
 poe2d_status poe2d_mlp_set_params(poe2d_mlp *m, int member, int layer, const float *w, const float *b)
 {
     poe2d_status st = check_mlp(m);

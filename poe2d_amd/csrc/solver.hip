@@ -1194,6 +1194,30 @@ poe2d_status poe2d_snapshot(poe2d_solver *s, int field, void *out, size_t bytes)
     return s->precision == POE2D_F32 ? field_to_host<float>(s, field, out) : field_to_host<double>(s, field, out);
 }
 
+poe2d_status poe2d_set_field(poe2d_solver *s, int field, const void *in, size_t bytes)
+{
+    poe2d_status st = check_handle(s);
+    if (st) return st;
+    if (!in) return fail(POE2D_E_INVALID, "set_field: null input");
+    if (field < POE2D_FIELD_VX || field > POE2D_FIELD_SXY)
+        return fail(POE2D_E_INVALID, "set_field: only the state fields VX .. SXY can be set");
+    const size_t need = (size_t)(s->grid.nx + 1) * (s->grid.nz + 1) * s->rs;
+    if (bytes != need) return fail(POE2D_E_INVALID, "set_field: input must hold (nx+1)(nz+1) values");
+    if (!s->dense && (st = dev_alloc_zero(&s->dense, need, s->stream))) return st;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipMemcpy(s->dense, in, need, hipMemcpyHostToDevice));
+    // V = (vx, vy, wx, wy), S = (e_xx, e_yy, zeta, sxy) of the current state buffer
+    const int g = field < POE2D_FIELD_EXX ? group_v(s->cur) : group_s(s->cur);
+    const int comp = field & 3;
+    if (s->precision == POE2D_F32)
+        launch_insert<float>(step_args<float>(s, s->cur, s->cur), g, comp, (const float *)s->dense, s->stream);
+    else
+        launch_insert<double>(step_args<double>(s, s->cur, s->cur), g, comp, (const double *)s->dense, s->stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return POE2D_OK;
+}
+
 poe2d_status poe2d_checkpoint_size(poe2d_solver *s, size_t *bytes)
 {
     if (!s || !bytes) return fail(POE2D_E_INVALID, "checkpoint_size: null argument");

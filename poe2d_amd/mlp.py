@@ -36,6 +36,18 @@ class Network:
             out.append((w, b))
         return out
 
+    def get_bf16(self, member=0):
+        """The bf16 weight copies the products read, per layer: (w16 [out][in], w16t [in][out]) as
+        uint16 bit patterns."""
+        out = []
+        for l in range(self.nlayers):
+            w16 = np.zeros((self.sizes[l + 1], self.sizes[l]), np.uint16)
+            w16t = np.zeros((self.sizes[l], self.sizes[l + 1]), np.uint16)
+            _abi.check(self._lib, self._lib.poe2d_mlp_get_bf16(self._h, member, l, w16.ctypes.data, w16t.ctypes.data),
+                       "mlp_get_bf16")
+            out.append((w16, w16t))
+        return out
+
     def set_params(self, params, member=0):
         for l, (w, b) in enumerate(params):
             w = np.ascontiguousarray(w, np.float32)

@@ -179,6 +179,14 @@ class HandleSolver:
         self._check(self._fn("seismograms")(self._h, out.ctypes.data, out.nbytes), "seismograms")
         return out
 
+    def set_field(self, fieldname, values):
+        """Overwrite one state field (vx .. sxy) of the current state with an (nz+1, nx+1) array."""
+        f = _abi.FIELD_NAMES[fieldname] if isinstance(fieldname, str) else fieldname
+        a = np.ascontiguousarray(values, dtype=self.real)
+        if a.shape != (self.problem.nz + 1, self.problem.nx + 1):
+            raise ValueError(f"set_field: shape {a.shape}, need {(self.problem.nz + 1, self.problem.nx + 1)}")
+        self._check(self._fn("set_field")(self._h, f, a.ctypes.data, a.nbytes), "set_field")
+
     def snapshot(self, fieldname):
         f = _abi.FIELD_NAMES[fieldname] if isinstance(fieldname, str) else fieldname
         out = np.zeros((self.problem.nz + 1, self.problem.nx + 1), dtype=self.real)

@@ -258,6 +258,8 @@ class ScreenResult:
     predict_seconds: float
     fallback_timing: SolveTiming
     judge_timing: SolveTiming
+    gathers: np.ndarray = None  # [n, 3, nrec, nsamp]: the surrogate's prediction where accepted,
+                                # the solver's gather where re-solved for its spread
 
     @property
     def models_per_second(self):
@@ -328,16 +330,18 @@ class Surrogate:
 
     def screen(self, params, threshold, seeds=None, judge_frac=0.05, seed=7, progress=None):
         """Predict every candidate, re-solve those whose spread exceeds `threshold`, then re-solve
-        a random `judge_frac` of the others to measure the surrogate's error (timed apart).
+        a random `judge_frac` of the others to measure the surrogate's error (timed apart).  The
+        result's gathers hold, per candidate, the prediction or (re-solved) the solver's gather.
         progress: passed to simulate for both sets of re-solves."""
         t0 = time.perf_counter()
         pred, spread = self.predict(params)
         t_pred = time.perf_counter() - t0
         fallback = np.nonzero(spread > threshold)[0]
         fb = SolveTiming(0, 0.0, 0.0)
+        gathers = pred  # re-solved rows replaced in place; the judged rows are accepted ones
         if len(fallback):
-            _, fb = simulate(params[fallback], None if seeds is None else seeds[fallback], self.geom,
-                             progress=progress)
+            gathers[fallback], fb = simulate(params[fallback], None if seeds is None else seeds[fallback],
+                                             self.geom, progress=progress)
         t_screen = time.perf_counter() - t0
         accepted = np.setdiff1d(np.arange(len(params)), fallback)
         rng = np.random.default_rng(seed)
@@ -349,7 +353,7 @@ class Surrogate:
             true, jt = simulate(params[judged], None if seeds is None else seeds[judged], self.geom,
                                 progress=progress)
             err = self.errors(pred[judged], true)
-        return ScreenResult(len(params), fallback, judged, err, t_screen, t_pred, fb, jt)
+        return ScreenResult(len(params), fallback, judged, err, t_screen, t_pred, fb, jt, gathers)
 
     def close(self):
         self.net.close()

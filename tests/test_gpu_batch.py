@@ -117,3 +117,78 @@ def test_batch_errors_and_autotune():
         assert a.step_count() == b.step_count() == 13
     a.close()
     b.close()
+
+
+def _parity_vs_oracle(prob, g, label):
+    """The member's traces against the FP32 oracle, with an FP64 GPU run of the same problem as the
+    FP64 reference for FP32-noisy traces."""
+    from .test_gpu_parity import assert_parity
+    with Solver(prob, precision=_abi.F64) as s:
+        s.run()
+        g64 = s.seismograms()
+    assert_parity(g, prob, label, o64=g64)
+
+
+def test_config2a_batch_at_shape():
+    """Configuration 2, batch A at its shape: 64 Ricker shots over one 1201 x 1201 patchy model
+    (clones sharing the model), advanced in lock-step on the default schedule for 301 steps
+    (150 step pairs and one single step).  Shots 0 and 63 equal the same shots run alone bitwise;
+    shot 0 matches the oracle."""
+    import dataclasses
+    from poe2d_amd.configs import config2a
+    nt = 301
+    base, shots, rec = config2a(nt=nt)
+    first = Solver(base)
+    members = [first] + [first.clone(source=sh, receivers=rec, nt=nt) for sh in shots[1:]]
+    assert len(members) == 64 and base.nx == base.nz == 1200
+    with Batch(members) as b:
+        say("config2a: 64 shots x 1201^2, 301 steps in one batch")
+        b.run(nt)
+        ms, n = b.last_run_ms()
+        say(f"config2a: {n} steps in {ms:.1f} ms = {64 * base.npoints * n / ms / 1e6:.1f} Gcell-updates/s")
+    results = {i: state(members[i]) for i in (0, 63)}
+    for m in members:
+        m.close()
+    for i in (0, 63):
+        p = dataclasses.replace(base, source=shots[i], receivers=rec, nt=nt)
+        with Solver(p) as s:
+            s.run()
+            alone = state(s)
+        same = all(np.array_equal(x, y) for x, y in zip(results[i], alone))
+        say(f"config2a shot {i}: batch equals the shot alone bitwise: {same}")
+        assert np.abs(alone[0]).max() > 0
+        for x, y in zip(results[i], alone):
+            assert np.array_equal(x, y)
+    _parity_vs_oracle(dataclasses.replace(base, source=shots[0], receivers=rec, nt=nt), results[0][0],
+                      "config2a shot 0")
+
+
+def test_config2b_batch_at_shape():
+    """Configuration 2, batch B at its shape: 32 patchy models with sides uniform in [300, 1500]
+    points and gas fraction uniform in [0, 0.3], in one lock-step batch for 301 steps.  Models 0 and
+    31 equal the same models run alone bitwise; model 5 matches the oracle."""
+    from poe2d_amd.configs import config2b
+    nt = 301
+    probs = config2b(nt=nt)
+    assert len(probs) == 32
+    sides = [(p.nx + 1, p.nz + 1) for p in probs]
+    say(f"config2b: sides {min(min(s) for s in sides)} .. {max(max(s) for s in sides)}, "
+        f"{sum(p.npoints for p in probs)} points")
+    members = [Solver(p) for p in probs]
+    with Batch(members) as b:
+        b.run(nt)
+        ms, n = b.last_run_ms()
+        say(f"config2b: {n} steps in {ms:.1f} ms = {sum(p.npoints for p in probs) * n / ms / 1e6:.1f} Gcell-updates/s")
+    results = {i: state(members[i]) for i in (0, 5, 31)}
+    for m in members:
+        m.close()
+    for i in (0, 31):
+        with Solver(probs[i]) as s:
+            s.run()
+            alone = state(s)
+        same = all(np.array_equal(x, y) for x, y in zip(results[i], alone))
+        say(f"config2b model {i} ({sides[i][0]}x{sides[i][1]}): batch equals the model alone bitwise: {same}")
+        assert np.abs(alone[0]).max() > 0
+        for x, y in zip(results[i], alone):
+            assert np.array_equal(x, y)
+    _parity_vs_oracle(probs[5], results[5][0], f"config2b model 5 ({sides[5][0]}x{sides[5][1]})")

@@ -91,3 +91,78 @@ def test_training_fits_a_smooth_map_and_spread_tracks_disagreement():
             net.set_params(p, member)
         _, same = net.predict(x[:64], want_mean=False)
         assert np.all(same <= 1e-6)
+
+
+def test_screening_network_shape_forward_and_gradients():
+    """The screening network at its full shape, 207 -> 512 -> 512 -> 512 -> 151 500 (four model
+    parameters and 203 layout inputs in; 101 receivers x 3 components x 500 samples out), on a batch
+    of 32 samples: forward pass, loss and every layer's gradients against the FP64 reference."""
+    rng = np.random.default_rng(13)
+    sizes = [207, 512, 512, 512, 151500]
+    params = random_params(sizes, rng)
+    x = rng.uniform(-1, 1, (32, sizes[0])).astype(np.float32)
+    t = rng.normal(size=(32, sizes[-1])).astype(np.float32)
+    with Network(sizes, members=1, max_batch=32) as net:
+        net.set_params(params)
+        y = net.forward(x)
+        y_ref, _ = forward(params, x, emulate=True)
+        ey = rel(y, y_ref)
+        loss, grads = net.gradients(x, t)
+    loss_ref, grads_ref = loss_and_grads(params, x, t, emulate=True)
+    say(f"207-512-512-512-151500: forward rel {ey:.2e}; loss {loss:.6f} vs {loss_ref:.6f}")
+    assert ey < 1e-4
+    assert abs(loss - loss_ref) <= 1e-4 * loss_ref
+    for l, ((gw, gb), (rw, rb)) in enumerate(zip(grads, grads_ref)):
+        ew, eb = rel(gw, rw), rel(gb, rb)
+        say(f"  layer {l} ({sizes[l]} -> {sizes[l + 1]}): dW rel {ew:.2e}, db rel {eb:.2e}")
+        assert ew < 2e-3 and eb < 2e-3
+
+
+def test_adam_matches_the_bias_corrected_update():
+    """Five Adam steps of one member against the bias-corrected update (beta 0.9 / 0.999, eps 1e-8)
+    restated in FP64 from the GPU's own gradients of each step's batch (the gradients themselves are
+    checked against the reference above); after every step the bf16 copy of each layer equals the
+    round-to-nearest-even of its master weights and the transposed copy equals its transpose, bit for
+    bit."""
+    rng = np.random.default_rng(21)
+    sizes = [4, 96, 64, 160, 300]
+    params = random_params(sizes, rng)
+    x = rng.uniform(-1, 1, (96, sizes[0])).astype(np.float32)
+    t = rng.normal(size=(96, sizes[-1])).astype(np.float32)
+    lr, b1, b2, eps = 1e-3, 0.9, 0.999, 1e-8
+    with Network(sizes, members=2, max_batch=64) as net:
+        net.set_params(params, member=1)
+        net.set_dataset(x, t)
+        untouched = net.get_params(0)
+        w = [(w.astype(np.float64), b.astype(np.float64)) for w, b in params]
+        m = [(np.zeros_like(a), np.zeros_like(b)) for a, b in w]
+        v = [(np.zeros_like(a), np.zeros_like(b)) for a, b in w]
+        for step in range(1, 6):
+            idx = rng.permutation(96)[:64].astype(np.int32)
+            _, grads = net.gradients(x[idx], t[idx], member=1)
+            net.train(idx, 64, lr, member=1)
+            got = net.get_params(1)
+            copies = net.get_bf16(1)
+            c1, c2 = 1.0 - b1 ** step, 1.0 - b2 ** step
+            for l in range(len(sizes) - 1):
+                for k in range(2):
+                    g = grads[l][k].astype(np.float64)
+                    mk = b1 * m[l][k] + (1 - b1) * g
+                    vk = b2 * v[l][k] + (1 - b2) * g * g
+                    upd = lr * (mk / c1) / (np.sqrt(vk / c2) + eps)
+                    ref = got[l][k].astype(np.float64)
+                    # restated from this step's own start point, so errors do not accumulate
+                    start = w[l][k]
+                    err = np.abs(ref - (start - upd)).max() / max(np.abs(upd).max(), 1e-30)
+                    assert err < 1e-3, (step, l, k, err)
+                    m[l] = (mk, m[l][1]) if k == 0 else (m[l][0], mk)
+                    v[l] = (vk, v[l][1]) if k == 0 else (v[l][0], vk)
+                    w[l] = (ref, w[l][1]) if k == 0 else (w[l][0], ref)
+                w16, w16t = copies[l]
+                expect = (bf16(got[l][0]).astype(np.float32).view(np.uint32) >> 16).astype(np.uint16)
+                assert np.array_equal(w16, expect), (step, l)
+                assert np.array_equal(w16t, w16.T), (step, l)
+            say(f"Adam step {step}: master weights within 1e-3 of the update, bf16 copies exact")
+        # the other member's weights are untouched
+        for (a, c), (b, d) in zip(untouched, net.get_params(0)):
+            assert np.array_equal(a, b) and np.array_equal(c, d)

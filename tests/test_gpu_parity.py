@@ -9,7 +9,7 @@ import pytest
 from oracle import pyoracle
 from oracle.pyoracle import OracleSolver
 from poe2d_amd import _abi
-from poe2d_amd.configs import BRINE, GAS, biot_fields, config0, config1
+from poe2d_amd.configs import BRINE, GAS, biot_fields, config0
 from poe2d_amd.solver import Problem, ReceiverSpec, Solver, SourceSpec
 
 from .helpers import random_patchy_mask, trace_rel_l2
@@ -24,27 +24,32 @@ def say(msg):
     print(f"[gpu-test] {msg}", flush=True)
 
 
-def assert_parity(g, prob, label, o32=None):
-    """Per-trace bound 1e-4 against the FP32 oracle.  Traces on which FP32 arithmetic itself is
-    noisy (near-nodal traces, e.g. v_x on the axis of a pressure source, whose value is a small
-    difference of large symmetric fields) are identified by the FP32 oracle's own distance to
-    the FP64 oracle: where that exceeds 5e-5, no FP32 evaluation order can promise 1e-4, and the
-    GPU result must instead be no further from the FP64 oracle than the reference's FP32
-    arithmetic is (factor 2).  Returns (max error on regular traces, number of noisy traces)."""
+def assert_parity(g, prob, label, o32=None, o64=None, exempt=0):
+    """Per-trace bound 1e-4 against the FP32 oracle.  A trace on which FP32 arithmetic itself is
+    noisy (a near-nodal trace, e.g. v_x on the axis of a pressure source, whose value is a small
+    difference of large symmetric fields) is recognised by the FP32 oracle's own distance to the
+    FP64 oracle: where that exceeds 5e-5, no FP32 evaluation order can promise 1e-4 against
+    another FP32 evaluation.  Such a trace may miss the 1e-4 bound only if the GPU result is no
+    further from the FP64 oracle than 1.25 x the FP32 oracle is; the number of traces that use
+    this exemption must equal `exempt`.  o64 may be an FP64 GPU result of the same problem (the
+    FP64 kernels match the FP64 oracle to ~1e-14).  Returns (max error, exempt count)."""
     o32 = run_oracle(prob, F32) if o32 is None else o32
-    o64 = run_oracle(prob, F64)
+    o64 = run_oracle(prob, F64) if o64 is None else o64
     err = trace_rel_l2(g, o32)
     noise = trace_rel_l2(o32, o64)
-    noisy = noise > 5e-5
-    reg = err[~noisy].max() if (~noisy).any() else 0.0
     g64 = trace_rel_l2(g, o64)
-    say(f"{label}: max rel L2 vs FP32 oracle = {err.max():.3e} (regular traces {reg:.3e}); "
-        f"{int(noisy.sum())} FP32-noisy traces, GPU-vs-FP64 {g64[noisy].max() if noisy.any() else 0:.3e} "
-        f"vs oracle-FP32-vs-FP64 {noise[noisy].max() if noisy.any() else 0:.3e}")
+    noisy = noise > 5e-5
+    used = noisy & (err > TOL32)
+    names = [f"{'vx vy p'.split()[c]}[{r}]: {err[c, r]:.2e} vs FP32, {g64[c, r]:.2e} vs FP64, oracle FP32 "
+             f"{noise[c, r]:.2e} vs FP64 (ratio {g64[c, r] / noise[c, r]:.2f})" for c, r in np.argwhere(used)]
+    say(f"{label}: max rel L2 vs FP32 oracle = {err.max():.3e}, regular traces "
+        f"{err[~noisy].max() if (~noisy).any() else 0:.3e}; {int(noisy.sum())} FP32-noisy traces, "
+        f"{int(used.sum())} exempt" + "".join(f"; {n}" for n in names))
     assert np.isfinite(g).all()
-    assert reg <= TOL32, (label, reg)
-    assert (g64[noisy] <= 2.0 * noise[noisy] + 1e-5).all(), label
-    return reg, int(noisy.sum())
+    assert (err[~noisy] <= TOL32).all(), (label, float(err[~noisy].max()))
+    assert (g64[used] <= 1.25 * noise[used]).all(), label
+    assert int(used.sum()) == exempt, (label, int(used.sum()), exempt)
+    return float(err.max()), int(used.sum())
 
 
 def run_gpu(prob, precision=F32, nsteps=None):
@@ -77,6 +82,11 @@ def test_parity_config0_full_fp32():
     assert_parity(g, prob, "config0")
 
 
+# traces outside the 1e-4 bound whose FP32 noise floor exceeds it: v_x on the source column of
+# the homogeneous pressure-source case (receiver 15, x = 120)
+EXEMPT = {"force_homog": 0, "pressure_homog": 1, "force_patchy": 0, "pressure_patchy_tcut": 0}
+
+
 @pytest.mark.parametrize("case", ["force_homog", "pressure_homog", "force_patchy", "pressure_patchy_tcut"])
 def test_parity_variants_fp32(case):
     n, nt = 240, 1200
@@ -91,16 +101,19 @@ def test_parity_variants_fp32(case):
                    receivers=rec, npml=20, f0=80.0)
     say(f"variant {case} {n + 1}^2 nt={nt}")
     g = run_gpu(prob)
-    assert_parity(g, prob, f"variant {case}")
+    assert_parity(g, prob, f"variant {case}", exempt=EXEMPT[case])
 
 
-def test_parity_fp64():
-    say("FP64 kernels vs FP64 oracle 201^2 nt=800")
-    prob = config0(nt=800, n=200)
+def test_parity_config0_full_fp64():
+    """Configuration 0 in FP64 as specified (401 x 401 points, 3000 steps): the FP64 kernels
+    against the FP64 oracle."""
+    say("config0 401x401 nt=3000 FP64 kernels vs FP64 oracle")
+    prob = config0()
     g = run_gpu(prob, F64)
     o = run_oracle(prob, F64)
     err = trace_rel_l2(g, o)
-    say(f"FP64 max rel L2 = {err.max():.3e}")
+    say(f"config0 FP64 max rel L2 = {err.max():.3e}")
+    assert np.abs(g).max() > 0
     assert err.max() <= 1e-9
 
 

@@ -67,8 +67,17 @@ def test_screening_gains_over_the_solver_alone(trained):
     assert len(res.judged) == max(1, round(0.25 * n_acc))
     assert not set(res.judged) & set(res.fallback)
     assert res.models_per_second > 1.5 * timing.models_per_second
-    # the threshold holds the median error after fallback near 5 % (sampling margin) ...
-    assert p50 <= 0.065
+    # the threshold holds the median error after fallback within the 5 % acceptance bound ...
+    assert p50 <= 0.05
+    # the screen returns every candidate's gather: the solver's own for the re-solved ones
+    assert res.gathers.shape == (res.n,) + S.gather_shape(GEOM)
+    for k in res.fallback[:3]:
+        with Solver(S.model_problem(cand[k], geom=GEOM)) as s:
+            s.run()
+            assert np.array_equal(s.seismograms(), res.gathers[k])
+    acc = np.setdiff1d(np.arange(res.n), res.fallback)[:2]
+    pred, _ = sur.predict(cand[acc])
+    assert np.array_equal(pred, res.gathers[acc])
     # ... and the spread sends the worse models to the solver
     assert np.median(res.judge_errors) < np.median(err)
     # the surrogate's own rate, fallback aside
