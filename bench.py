@@ -199,15 +199,16 @@ def build_batch(which, nt):
     return members, workload, bytes_per_point, npts
 
 
-def time_batch(members, nt, warmup, reps, schedules=(0, 3), label="batch"):
-    """Autotune the batch on each schedule (one and two steps per launch), keep the faster, then
-    time `reps` runs of nt steps after `warmup` untimed ones (device time, hipEvents on the batch's
-    stream).  Returns (median ms per run, schedule, tile, traces finite)."""
+def time_batch(members, nt, warmup, reps, schedules=(0, 3), label="batch", fixed=None):
+    """Autotune the batch on each schedule (one and two steps per launch), keep the faster (or take
+    fixed = (schedule, (block width, tile rows))), then time `reps` runs of nt steps after `warmup`
+    untimed ones (device time, hipEvents on the batch's stream).  Returns (median ms per run,
+    schedule, tile, traces finite)."""
     from poe2d_amd.solver import Batch
 
     with Batch(members) as b:
-        best = None
-        for sched in schedules:
+        best = None if fixed is None else (0.0, fixed[0], fixed[1])
+        for sched in (schedules if fixed is None else ()):
             for m in members:
                 m.reset()
                 m.set_schedule(sched, 256, 12 if sched == 3 else 8)
@@ -247,7 +248,10 @@ def batch_bench(args):
     kernel autotuned.  value = sum over members of points x nt / device time of one batch run."""
     nt = args.nt if args.nt_given else 3000
     members, workload, bytes_per_point, npts = build_batch(args.workload, nt)
-    ms, schedule, tile, finite = time_batch(members, nt, args.warmup, args.steps, label=args.workload)
+    fixed = None
+    if args.tile != "auto":
+        fixed = (3 if args.schedule == "auto" else int(args.schedule), tuple(int(v) for v in args.tile.lower().split("x")))
+    ms, schedule, tile, finite = time_batch(members, nt, args.warmup, args.steps, label=args.workload, fixed=fixed)
     for m in members:
         m.close()
     gcell = npts * nt / (ms / 1e3) / 1e9
@@ -258,7 +262,8 @@ def batch_bench(args):
         "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": workload, "members": len(members), "points": npts, "nt": nt,
-                   "tile": f"{tile[0]}x{tile[1]} (autotuned)", "step": "one batch run of nt steps",
+                   "tile": f"{tile[0]}x{tile[1]} ({'autotuned' if fixed is None else 'fixed'})",
+                   "step": "one batch run of nt steps",
                    "schedule": SCHEDULE_NAMES[schedule],
                    "launch": "one work-list launch per time step (per two steps with the two-step kernel), "
                              "32-step HIP graphs", "traces_finite": finite},
