@@ -360,6 +360,14 @@ poe2d_status poe2d_mlp_time(poe2d_mlp *m, int batch, int iters, double *forward_
 /* C[M][N] = A[M][K] B[N][K]^T with A and B rounded to bf16 (round to nearest even) and FP32
  * accumulation on the matrix cores, ksplit >= 1 K partitions (a test and measurement helper). */
 poe2d_status poe2d_gemm_bf16(int M, int N, int K, const float *A, const float *B, float *C, int ksplit);
+/* The same product without allocating: no counterpart in the reference; for callers that cannot
+ * allocate device memory (tight serving loops, harnesses that forbid hipMalloc).  Operand staging,
+ * the product and the split-K partials live in a fixed 32 MiB device scratch that each GPU holds
+ * from library load until the process ends; operands needing more (4 M K + 4 N K + 2 (M + N) Kp
+ * + 4 M N + 4 ksplit M N bytes, Kp = K rounded up to 32, each part rounded up to 256) are refused
+ * with POE2D_E_INVALID.  Results equal poe2d_gemm_bf16's bit for bit.  Every call on a GPU shares
+ * that GPU's scratch: calls that overlap on one GPU (from several host threads) race. */
+poe2d_status poe2d_gemm_bf16_noalloc(int M, int N, int K, const float *A, const float *B, float *C, int ksplit);
 void poe2d_mlp_destroy(poe2d_mlp *m);
 
 /* ---- Reference-format input and output (host only; no device needed) ---- */

@@ -142,6 +142,7 @@ def _declare(lib):
         "poe2d_mlp_predict": ([vp, i, c_float_p, c_float_p, c_float_p], i),
         "poe2d_mlp_time": ([vp, i, i, c_double_p, c_double_p], i),
         "poe2d_gemm_bf16": ([i, i, i, c_float_p, c_float_p, c_float_p, i], i),
+        "poe2d_gemm_bf16_noalloc": ([i, i, i, c_float_p, c_float_p, c_float_p, i], i),
         "poe2d_mlp_destroy": ([vp], None),
         "poe2d_read_input1": ([ctypes.c_char_p, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p], i),
         "poe2d_read_input2": ([ctypes.c_char_p, c_double_p], i),

@@ -15,6 +15,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -208,6 +209,46 @@ __global__ void row_spread(const float *__restrict__ mean, const float *__restri
 }
 
 int blocks(size_t n) { return (int)std::min<size_t>((n + 255) / 256, 1u << 20); }
+
+// rows x K FP32 values into rows x Kp bf16 values (round to nearest even), zeros in columns K..Kp-1.
+__global__ void pack_bf16_rows(const float *__restrict__ src, int rows, int K, int Kp, uint16_t *__restrict__ dst)
+{
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= (size_t)rows * Kp) return;
+    const int r = (int)(i / Kp), k = (int)(i % Kp);
+    dst[i] = k < K ? bf16_rne(src[(size_t)r * K + k]) : (uint16_t)0;
+}
+
+// Device scratch of poe2d_gemm_bf16_noalloc: operand staging (FP32 as copied from the host, then
+// bf16), the product and the split-K partials.  Each device the library runs on holds its own copy
+// from module load until the process ends.  Host code must not pass `scratch` itself to a kernel or
+// copy (in host code the name is a stand-in, not a device address): it uses the address
+// hipGetSymbolAddress returns for the current device, kept in g_scratch_addr.
+constexpr size_t kGemmScratchBytes = size_t(32) << 20;
+__device__ __align__(256) unsigned char scratch[kGemmScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+unsigned char *g_scratch_addr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per device.
+poe2d_status scratch_address(unsigned char **out)
+{
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(POE2D_E_INVALID, "gemm_bf16_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_addr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(POE2D_E_HIP, std::string("gemm_bf16_noalloc: hipGetSymbolAddress(scratch): ") +
+                                         hipGetErrorString(e));
+        g_scratch_addr[dev] = (unsigned char *)p;
+    }
+    *out = g_scratch_addr[dev];
+    return POE2D_OK;
+}
 
 // Forward pass of member e on the first n rows of m->xin (already packed into h16[0]).
 poe2d_status forward(poe2d_mlp *m, Member &e, int n)
@@ -594,6 +635,45 @@ poe2d_status poe2d_gemm_bf16(int M, int N, int K, const float *A, const float *B
     if (e == hipSuccess) e = hipMemcpy(C, dc, (size_t)M * N * sizeof(float), hipMemcpyDeviceToHost);
     dfree(da); dfree(db); dfree(dc); dfree(work);
     if (e != hipSuccess) return fail(POE2D_E_HIP, std::string("gemm_bf16: ") + hipGetErrorString(e));
+    return POE2D_OK;
+}
+
+poe2d_status poe2d_gemm_bf16_noalloc(int M, int N, int K, const float *A, const float *B, float *C, int ksplit)
+{
+    if (M < 1 || N < 1 || K < 1 || !A || !B || !C || ksplit < 1)
+        return fail(POE2D_E_INVALID, "gemm_bf16_noalloc: bad arguments");
+    const size_t Kp = ((size_t)K + 31) / 32 * 32;
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    // scratch regions: A and B as FP32, A and B as bf16 (K padded to Kp), C, split-K partials
+    const size_t sa32 = up((size_t)M * K * 4), sb32 = up((size_t)N * K * 4);
+    const size_t sa16 = up((size_t)M * Kp * 2), sb16 = up((size_t)N * Kp * 2), sc = up((size_t)M * N * 4);
+    const size_t wf = (size_t)ksplit * M * N, sw = ksplit > 1 ? up(wf * 4) : 0;
+    const size_t need = sa32 + sb32 + sa16 + sb16 + sc + sw;
+    if (need > kGemmScratchBytes)
+        return fail(POE2D_E_INVALID, "gemm_bf16_noalloc: these operands need " + std::to_string(need) +
+                                         " bytes of device scratch, more than its " +
+                                         std::to_string(kGemmScratchBytes) + " (use poe2d_gemm_bf16)");
+    unsigned char *base = nullptr;
+    poe2d_status st = scratch_address(&base);
+    if (st) return st;
+    float *a32 = (float *)base, *b32 = (float *)(base + sa32);
+    uint16_t *a16 = (uint16_t *)(base + sa32 + sb32), *b16 = (uint16_t *)(base + sa32 + sb32 + sa16);
+    float *c = (float *)(base + sa32 + sb32 + sa16 + sb16);
+    float *work = sw ? (float *)(base + sa32 + sb32 + sa16 + sb16 + sc) : nullptr;
+    // Nothing here has to start at zero: the packing kernel writes every bf16 element (zeros in the
+    // padding), and the product writes every element of C and of each split's partial sums.
+    HIP_TRY(hipMemcpy(a32, A, (size_t)M * K * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(b32, B, (size_t)N * K * 4, hipMemcpyHostToDevice));
+    pack_bf16_rows<<<blocks((size_t)M * Kp), 256, 0, nullptr>>>(a32, M, K, (int)Kp, a16);
+    pack_bf16_rows<<<blocks((size_t)N * Kp), 256, 0, nullptr>>>(b32, N, K, (int)Kp, b16);
+    HIP_TRY(hipGetLastError());
+    GemmOut o;
+    o.f32 = c;
+    o.ldf = N;
+    HIP_TRY(gemm_bf16(a16, (int)Kp, b16, (int)Kp, M, N, (int)Kp, EPI_LINEAR, o, work, sw ? wf : 0, sw ? ksplit : 1,
+                      nullptr));
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(C, c, (size_t)M * N * sizeof(float), hipMemcpyDeviceToHost));
     return POE2D_OK;
 }
 

@@ -131,3 +131,15 @@ def gemm_bf16(a, b, ksplit=1):
     c = np.zeros((a.shape[0], b.shape[0]), np.float32)
     _abi.check(lib, lib.poe2d_gemm_bf16(a.shape[0], b.shape[0], a.shape[1], _f(a), _f(b), _f(c), ksplit), "gemm_bf16")
     return c
+
+
+def gemm_bf16_noalloc(a, b, ksplit=1):
+    """As gemm_bf16, through the entry point that allocates no device memory (its operands, product
+    and partial sums must fit the library's fixed 32 MiB device scratch)."""
+    lib = _abi.load_library()
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    c = np.zeros((a.shape[0], b.shape[0]), np.float32)
+    _abi.check(lib, lib.poe2d_gemm_bf16_noalloc(a.shape[0], b.shape[0], a.shape[1], _f(a), _f(b), _f(c), ksplit),
+               "gemm_bf16_noalloc")
+    return c
