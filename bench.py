@@ -106,15 +106,15 @@ def compulsory_bytes_per_step(npts, strip_points, schedule, palette):
     return per_point * npts / per_launch_steps + 48.0 * strip_points
 
 
-def cpu_leg(prob, n, steps, gpu_traces):
+def cpu_leg(prob, make, steps, gpu_traces):
     """The oracle on the host: (1) the CPU baseline, its reference-structured form (five sweeps plus
     three full-grid receiver scans per step, one thread) timed over `steps` steps of the same model;
     (2) as the checker, the first steps of the GPU's source run (gpu_traces, [3, nrec, k]) against the
     FP32 oracle run for k steps on all host threads.  Returns (baseline, parity or None)."""
-    from oracle.pyoracle import OracleSolver, material_biot
-    from poe2d_amd.configs import config1
+    from oracle import pyoracle
+    from oracle.pyoracle import OracleSolver
 
-    cprob = config1(nt=steps + 1, n=n, material_fn=material_biot)
+    cprob = make(steps + 1, pyoracle)
     s = OracleSolver(cprob, threads=1)
     s.set_reference_scans(122, 680, cprob.nz // 2 + 100, cprob.nz // 2 + 200)
     s.run(1)  # untimed first step (page-in)
@@ -132,7 +132,7 @@ def cpu_leg(prob, n, steps, gpu_traces):
     except OSError:
         pass
     baseline = {"value": npts * steps / dt / 1e9, "unit": "Gcell-updates/s", "cores": 1, "kind": "port",
-                "sample": f"{steps} time steps of the {cprob.nx + 1}x{cprob.nz + 1} configuration-1 model "
+                "sample": f"{steps} time steps of the same {cprob.nx + 1}x{cprob.nz + 1} model "
                           f"({dt:.1f} s), reference-structured oracle (5 sweeps + 3 full-grid receiver scans), "
                           f"1 thread; host {cpu_model or platform.processor()}, nproc {os.cpu_count()}"}
     if gpu_traces is None:
@@ -240,6 +240,42 @@ def time_batch(members, nt, warmup, reps, schedules=(0, 3), label="batch", fixed
 
 
 SCHEDULE_NAMES = {0: "one step per launch", 3: "two steps per launch"}
+
+
+def secondary_continuous(n, nt):
+    """Configuration 1's geometry over a medium whose porosity and frame moduli vary continuously
+    (every point its own coefficients, so no material palette: both kernels read the 32-byte
+    coefficient groups), each schedule autotuned and timed over one nt-step solve after a warm-up
+    solve."""
+    from poe2d_amd.configs import config1_continuous
+    from poe2d_amd.solver import Solver
+
+    prob = config1_continuous(nt=nt, n=n)
+    npts = prob.npoints
+    cols, rows = strip_counts(prob)
+    strips = cols * (prob.nz + 1) + rows * (prob.nx + 1)
+    out = {"workload": prob.meta["workload"] + f", {nt} steps", "nt": nt}
+    with Solver(prob) as s:
+        out["material_palette"] = s.material_palette()
+        for sched in (0, 3):
+            s.reset()
+            s.set_schedule(sched, 256, 12 if sched == 3 else 8)
+            t = s.autotune(10)
+            s.set_schedule(sched, *t)
+            for _ in range(2):
+                s.reset()
+                s.run(nt)
+            ms, k = s.last_run_ms()
+            step_s = ms / 1e3 / k
+            out[f"schedule_{sched}"] = {"kernel": SCHEDULE_NAMES[sched], "tile": f"{t[0]}x{t[1]} (autotuned)",
+                                        "us_per_step": round(step_s * 1e6, 1),
+                                        "gcell_updates_per_s": round(npts / step_s / 1e9, 2),
+                                        "compulsory_frac": round(compulsory_bytes_per_step(npts, strips, sched, False)
+                                                                 / step_s / 1e9 / HBM_PEAK_GBS, 4)}
+            print(f"[bench] continuous model schedule {sched}: {t[0]}x{t[1]}, {step_s * 1e6:.1f} us per step",
+                  file=sys.stderr, flush=True)
+    out["two_step_speedup"] = round(out["schedule_0"]["us_per_step"] / out["schedule_3"]["us_per_step"], 3)
+    return out
 
 
 def batch_bench(args):
@@ -423,8 +459,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1, help="untimed full solves")
     ap.add_argument("--n", type=int, default=4000, help="NX = NZ (grid has n+1 points per side)")
     ap.add_argument("--nt", type=int, default=None, help="time steps per solve (6000; 3000 for the batches)")
-    ap.add_argument("--workload", default="config1", choices=("config1", "batchA", "batchB", "surrogate"),
-                    help="config1: the 4001^2 single shot (the headline line); batchA / batchB: "
+    ap.add_argument("--workload", default="config1",
+                    choices=("config1", "continuous", "batchA", "batchB", "surrogate"),
+                    help="config1: the 4001^2 single shot (the headline line); continuous: the same geometry "
+                         "over a continuously varying medium (no material palette); batchA / batchB: "
                          "configuration 2's batches; surrogate: configurations 3 and 4")
     ap.add_argument("--sur-train", type=int, default=4096,
                     help="surrogate: training models (0: time the network alone)")
@@ -447,6 +485,8 @@ def main():
     ap.add_argument("--secondary", type=int, default=1,
                     help="1: add configuration 2's batches (briefly) and the network timing to the line")
     ap.add_argument("--batch-nt", type=int, default=300, help="steps per run of the secondary batch timings")
+    ap.add_argument("--continuous-nt", type=int, default=1000,
+                    help="steps per run of the secondary continuous-model timing")
     ap.add_argument("--tile", default="auto", metavar="BWxTH",
                     help="tile shape of the step kernel (block width x tile rows); 'auto' runs the "
                          "library's autotuner first")
@@ -470,15 +510,21 @@ def main():
     if args.workload == "surrogate":
         surrogate_bench(args)
         return
-    if args.workload != "config1":
+    if args.workload not in ("config1", "continuous"):
         batch_bench(args)
         return
 
     from poe2d_amd import F32, F64
-    from poe2d_amd.configs import config1
+    from poe2d_amd.configs import config1, config1_continuous
     from poe2d_amd.solver import Solver
 
-    prob = config1(nt=args.nt, n=args.n)
+    if args.workload == "config1":
+        def make(nt, lib=None):
+            return config1(nt=nt, n=args.n, material_fn=None if lib is None else lib.material_biot)
+    else:
+        def make(nt, lib=None):
+            return config1_continuous(nt=nt, n=args.n, material_fn=None if lib is None else lib.material_biot)
+    prob = make(args.nt)
     npts = (prob.nx + 1) * (prob.nz + 1)
     cols, rows = strip_counts(prob)
     bytes_step = algorithmic_bytes_per_step(prob.nx, prob.nz, cols, rows)
@@ -550,7 +596,9 @@ def main():
     traffic, traffic_src = recorded_traffic(tile, args.n, schedule)
     compulsory = compulsory_bytes_per_step(npts, cols * (prob.nz + 1) + rows * (prob.nx + 1), schedule, palette > 0)
     out = {
-        "metric": "Gcell-updates/s (FP32 Biot C-PML time loop, 4001x4001, 6000 steps)",
+        "metric": ("Gcell-updates/s (FP32 Biot C-PML time loop, 4001x4001, 6000 steps)" if args.workload == "config1"
+                   else f"Gcell-updates/s (FP32 Biot C-PML time loop, {prob.nx + 1}x{prob.nz + 1} continuous model, "
+                        f"{args.nt} steps)"),
         "value": round(gcell * 1.0, 3),
         "unit": "Gcell-updates/s",
         "n_gpus": 1,
@@ -596,11 +644,12 @@ def main():
                               "avoid per step (state read and written once per launch, the one-byte material "
                               "index or the coefficients, C-PML records) over the same time")},
     }
-    if args.secondary:
-        out["secondary"] = {"configuration_2": secondary_batches(args.batch_nt),
+    if args.secondary and args.workload == "config1":
+        out["secondary"] = {"continuous_model": secondary_continuous(args.n, args.continuous_nt),
+                            "configuration_2": secondary_batches(args.batch_nt),
                             "surrogate_network": network_timing(True)[0]}
     if args.cpu_steps > 0:
-        baseline, parity = cpu_leg(prob, args.n, args.cpu_steps, parity_traces)
+        baseline, parity = cpu_leg(prob, make, args.cpu_steps, parity_traces)
         out["cpu_baseline"] = baseline
         if parity is not None:
             out["parity_rel_l2_max"] = float(f"{parity['max']:.4g}")

@@ -118,3 +118,31 @@ def config2b(nt=3000, material_fn=None, nmodels=32, nmin=300, nmax=1500, seed=1)
                              receivers=line_receivers(x0, step, nrec, min(cz + 40, nz - 25)), npml=20, f0=80.0,
                              meta={"config": "2B", "gas_fraction": frac, "seed": seed + k}))
     return probs
+
+
+def continuous_model(nx, nz):
+    """Biot parameters of a medium that varies continuously in depth and offset (every point its own
+    coefficients, so no material palette): the brine rock with porosity falling from 0.30 to 0.18
+    with depth, frame moduli rising with it (K_fr 9.6 -> 19.2 GPa, mu_fr 8 -> 14.4 GPa), and a
+    lateral undulation of +-0.03 in porosity and +-10 % in the moduli."""
+    z = np.linspace(0.0, 1.0, nz + 1)[:, None]
+    x = np.linspace(0.0, 1.0, nx + 1)[None, :]
+    wave = np.sin(2 * np.pi * (3.0 * x + 1.5 * z))
+    out = biot_fields(nx, nz)
+    out["phi"] = 0.30 - 0.12 * z + 0.03 * wave
+    out["K_fr"] = BRINE["K_fr"] * (1.0 + z) * (1.0 + 0.1 * wave)
+    out["mu_fr"] = BRINE["mu_fr"] * (1.0 + 0.8 * z) * (1.0 + 0.1 * wave)
+    return {k: np.broadcast_to(v, (nz + 1, nx + 1)).copy() for k, v in out.items()}
+
+
+def config1_continuous(nt=6000, n=4000, material_fn=None):
+    """Configuration 1's geometry (4001 x 4001 points, Ricker force at the centre, 801 receivers 40
+    points below it) over continuous_model's medium."""
+    coeffs = coefficients(continuous_model(n, n), material_fn)
+    c = n // 2
+    nrec = 801 if n >= 4000 else max(5, (n - 200) // 4)
+    x0 = c - 4 * (nrec // 2)
+    return Problem(nx=n, nz=n, dx=0.5, dz=0.5, dt=5.0e-5, nt=nt, coeffs=coeffs,
+                   source=SourceSpec(kind=_abi.SRC_FORCE_Y_RICKER, ix=c, iz=c, amp=1.0e10),
+                   receivers=line_receivers(x0, 4, nrec, c + 40), npml=20, f0=80.0,
+                   meta={"config": "1c", "workload": f"biot-cpml {n + 1}x{n + 1} continuous porosity and frame moduli"})

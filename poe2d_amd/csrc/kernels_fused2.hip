@@ -1,4 +1,4 @@
-// Two time steps per launch (temporal blocking of the single-pass step, palette grids).
+// Two time steps per launch (temporal blocking of the single-pass step).
 //
 // A block owns BW - 12 output columns (6 halo columns on each side) and th rows of one model and
 // marches down its rows as the single-pass kernel does, running two copies of that kernel's row
@@ -40,8 +40,11 @@ namespace poe2d {
 // + 2 of each row: lv 3 step-n velocity rows (vx, wx, vy), ls 3 stage-1 stress rows (sxx, p, sxy),
 // lv2 6 step-(n+1) velocity rows (the first R2 in use), ls2 3 stage-2 stress rows; lpal the MS,
 // the MV and the velocity-constant entries (palette sections PAL_MS, PAL_MV, PAL_CN, kMaxPalette2
-// entries each).
-template <class T, int BW, bool AVG, bool EDGE>
+// entries each).  PAL = false: grids without a palette the LDS copy can hold read each point's
+// coefficients from the MS and MV groups where they are used (32 B per point from HBM once per
+// launch; the re-reads of the same rows by the other stage and the neighbouring row steps hit the
+// caches) and form the velocity-update constants per point, which rounds as the palette's do.
+template <class T, int BW, bool AVG, bool EDGE, bool PAL>
 __device__ __forceinline__ void step2_tile(StepArgs<T> A, const int tile, const long long n, T (*lv)[3][BW + 4],
                                            T (*ls)[3][BW + 4], T (*lv2)[3][BW + 4], T (*ls2)[3][BW + 4],
                                            Vec4<T> *lpal)
@@ -105,7 +108,19 @@ __device__ __forceinline__ void step2_tile(StepArgs<T> A, const int tile, const 
     const uint32_t colm = (uint32_t)(cc + kPadCols);
     gcuchar *const mid = (gcuchar *)A.mid;
     auto ldm = [&](int row) -> int { return mid[colm + (uint32_t)(min(row, last_row) + kPadRows) * (uint32_t)P]; };
-    {
+    // coefficients of row `row` at this lane: from the palette entry `id` in LDS, or from the groups
+    const gchar *const MSg = (const gchar *)A.mat + G_MS * gsb;
+    const gchar *const MVg = (const gchar *)A.mat + G_MV * gsb;
+    // (groups: the row's coefficients arrive through the register rings msr / mvr below)
+    auto ms_at = [&](int id, const Vec4<T> &ring) -> Vec4<T> {
+        if constexpr (PAL) return lpal[PAL_MS * kMaxPalette2 + id];
+        else return ring;
+    };
+    auto mv_at = [&](int id, const Vec4<T> &ring) -> Vec4<T> {
+        if constexpr (PAL) return lpal[PAL_MV * kMaxPalette2 + id];
+        else return ring;
+    };
+    if constexpr (PAL) {
         const GN *const pal = (const GN *)A.pal;
         for (int i = lane; i < PAL_SECTIONS * kMaxPalette2; i += BW) {
             const int e = i % kMaxPalette2;
@@ -158,8 +173,8 @@ __device__ __forceinline__ void step2_tile(StepArgs<T> A, const int tile, const 
     // sets sxx, p, syy.
     auto stress = [&](int r, const Vec4<T> &vm2, const Vec4<T> &vm1, const Vec4<T> &v0, const Vec4<T> &vp1,
                       const Vec4<T> &vp2, const Vec4<T> &so, const T (*xv)[BW + 4], int id_cur, int id_nxt,
-                      auto pxo, auto pxn_, auto pzo, auto pzn_, bool keep, T srcv, T &sxx,
-                      T &p, T &syy) __attribute__((always_inline)) -> Vec4<T> {
+                      const Vec4<T> &m_cur, const Vec4<T> &m_nxt, auto pxo, auto pxn_, auto pzo, auto pzn_,
+                      bool keep, T srcv, T &sxx, T &p, T &syy) __attribute__((always_inline)) -> Vec4<T> {
         const bool zr1 = r >= 2 && r < nz - 1;
         const bool zr2 = r >= 1 && r < nz - 1;
         const bool u1 = xr1 && zr1, u2 = xr2 && zr2;
@@ -198,7 +213,7 @@ __device__ __forceinline__ void step2_tile(StepArgs<T> A, const int tile, const 
             exx_n = so.x + dt * dvx;
             eyy_n = so.y + dt * dvz;
             zeta_n = (so.z - dwx * dt) - dwz * dt;
-            const Vec4<T> ms_cur = lpal[PAL_MS * kMaxPalette2 + id_cur];
+            const Vec4<T> ms_cur = ms_at(id_cur, m_cur);
             p = pore_pressure(exx_n, eyy_n, zeta_n, ms_cur.x, ms_cur.y);
             if (src_p && r == A.src_z) p += srcv;
             if (xr3) {
@@ -214,8 +229,8 @@ __device__ __forceinline__ void step2_tile(StepArgs<T> A, const int tile, const 
             if (sx >= 0) ddx = cpml_from(ddx, o_dvy, pxn_(qx) + PX_DVY, keep, ax, bx, rkx);
             // half-grid memory coefficients, full-grid K_z divisor (reference :224-226)
             if (szs >= 0) ddz = cpml_from(ddz, o_dvxz, pzn_(qz) + PZ_DVX, keep, azh, bzh, rkz);
-            const T mu_nxt = lpal[PAL_MS * kMaxPalette2 + id_nxt].w;
-            const T mu_xy = AVG ? harmonic_mu(lpal[PAL_MS * kMaxPalette2 + id_cur].w, mu_nxt) : mu_nxt;
+            const T mu_nxt = ms_at(id_nxt, m_nxt).w;
+            const T mu_xy = AVG ? harmonic_mu(ms_at(id_cur, m_cur).w, mu_nxt) : mu_nxt;
             sxy_n = so.w + mu_xy * (ddx + ddz) * dt;
         }
         return Vec4<T>{exx_n, eyy_n, zeta_n, sxy_n};
@@ -224,8 +239,9 @@ __device__ __forceinline__ void step2_tile(StepArgs<T> A, const int tile, const 
     // One velocity update of row j at this lane: v the step's old velocities of row j, xs the
     // new stress row j (x neighbours), sxy of rows j-2 .. j+1 and syy, p of rows j-1 .. j+2.
     auto velocity = [&](int j, Vec4<T> v, const T (*xs)[BW + 4], T sxy_m2, T sxy_m1, T sxy_0, T sxy_p1, T syy_m1,
-                        T syy_0, T syy_p1, T syy_p2, T p_m1, T p_0, T p_p1, T p_p2, int id_j, int id_j1, auto pxo,
-                        auto pxn_, auto pzo, auto pzn_, bool keep, T srcv) __attribute__((always_inline)) -> Vec4<T> {
+                        T syy_0, T syy_p1, T syy_p2, T p_m1, T p_0, T p_p1, T p_p2, int id_j, int id_j1,
+                        const Vec4<T> &m_j, const Vec4<T> &m_j1, auto pxo, auto pxn_, auto pzo, auto pzn_, bool keep,
+                        T srcv) __attribute__((always_inline)) -> Vec4<T> {
         const int szj = EDGE ? zstrip(A, j) : -1;
         const bool u4 = xr4 && j >= 2 && j < nz, u5 = xr5 && j >= 1 && j < nz - 1;
         // C-PML: the update's old records and row coefficients, all read before any record is written
@@ -251,8 +267,11 @@ __device__ __forceinline__ void step2_tile(StepArgs<T> A, const int tile, const 
                 e2 = cpml_from(e2, o_dpx, pxn_(qx) + PX_DPX, keep, ax, bx, rkx);
             }
             if (szj >= 0) e3 = cpml_from(e3, o_dsxyz, pzn_(qz) + PZ_DSXY, keep, az, bz, rkz);
-            const Vec4<T> mv_j = lpal[PAL_MV * kMaxPalette2 + id_j];
-            biot_velocity_cn(v.x, v.z, e1 + e3, e2, mv_j.x, mv_j.y, mv_j.z, lpal[PAL_CN * kMaxPalette2 + id_j]);
+            const Vec4<T> mv_j = mv_at(id_j, m_j);
+            if constexpr (PAL)
+                biot_velocity_cn(v.x, v.z, e1 + e3, e2, mv_j.x, mv_j.y, mv_j.z, lpal[PAL_CN * kMaxPalette2 + id_j]);
+            else
+                biot_velocity(v.x, v.z, e1 + e3, e2, mv_j.x, mv_j.y, mv_j.z, mv_j.w, A.inv_dt);
         }
         if (u5) {
             T e1 = d4(xs[2][lane + 3], xs[2][lane + 2], xs[2][lane + 4], xs[2][lane + 1], rdx);
@@ -266,11 +285,13 @@ __device__ __forceinline__ void step2_tile(StepArgs<T> A, const int tile, const 
             T div = e1 + e2;
             if (src_f && j == A.src_z) div += srcv;
             if constexpr (AVG) {
-                const Vec4<T> mvy = mean_rows(lpal[PAL_MV * kMaxPalette2 + id_j], lpal[PAL_MV * kMaxPalette2 + id_j1]);
+                const Vec4<T> mvy = mean_rows(mv_at(id_j, m_j), mv_at(id_j1, m_j1));
                 biot_velocity(v.y, v.w, div, e3, mvy.x, mvy.y, mvy.z, mvy.w, A.inv_dt);
-            } else {
+            } else if constexpr (PAL) {
                 const Vec4<T> mv_j1 = lpal[PAL_MV * kMaxPalette2 + id_j1];
                 biot_velocity_cn(v.y, v.w, div, e3, mv_j1.x, mv_j1.y, mv_j1.z, lpal[PAL_CN * kMaxPalette2 + id_j1]);
+            } else {
+                biot_velocity(v.y, v.w, div, e3, m_j1.x, m_j1.y, m_j1.z, m_j1.w, A.inv_dt);
             }
         }
         return v;
@@ -296,8 +317,13 @@ __device__ __forceinline__ void step2_tile(StepArgs<T> A, const int tile, const 
     Vec4<T> v1r[6];  // step-(n+1) velocities, rows s-7 .. s-2
     Vec4<T> s1r[6];  // step-(n+1) strains, rows s-4 .. s
     T sxyr2[6], syyr2[6], pr2[6];  // stage-2 stress rows s-8 .. s-4
+    // groups: coefficients of rows s-4 .. s+1 (MS) and s-6 .. s-1 (MV), each row's load issued one
+    // row step before its first use, into the slot of the row six back, whose last use (stage 2)
+    // has just passed
+    Vec4<T> msr[6], mvr[6];
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
+        msr[q] = mvr[q] = zero4;
         sxyr[q] = syyr[q] = pr[q] = T(0);
         sxyr2[q] = syyr2[q] = pr2[q] = T(0);
         idr[q] = idr2[q] = 0;
@@ -314,10 +340,17 @@ __device__ __forceinline__ void step2_tile(StepArgs<T> A, const int tile, const 
     sr[0] = ld(So, s0);
     sr[1] = ld(So, s0 + 1);
     sr[2] = zero4;
-    idr[5] = ldm(s0 - 1);
-    idr[0] = ldm(s0);
-    idr[1] = ldm(s0 + 1);
-    idr[2] = ldm(s0 + 2);
+    if constexpr (PAL) {
+        idr[5] = ldm(s0 - 1);
+        idr[0] = ldm(s0);
+        idr[1] = ldm(s0 + 1);
+        idr[2] = ldm(s0 + 2);
+    } else {
+        msr[0] = ld(MSg, s0);
+        msr[1] = ld(MSg, s0 + 1);
+        mvr[4] = ld(MVg, s0 - 2);
+        mvr[5] = ld(MVg, s0 - 1);
+    }
     __syncthreads();  // palette in LDS
 
     int s = s0;
@@ -334,8 +367,10 @@ __device__ __forceinline__ void step2_tile(StepArgs<T> A, const int tile, const 
         sr[(PH + 2) % 3] = ld(So, s + 2);
         // material through the palette in LDS, looked up where used: stage 1 reads MS of rows s,
         // s+1 and MV of rows s-2, s-1; stage 2 MS of rows s-4, s-3 and MV of rows s-6, s-5
-        idr2[(PH + 5) % 6] = idr[(PH + 5) % 6];  // row s-1 moves to the older ring
-        idr[(PH + 3) % 6] = ldm(s + 3);           // into the slot of row s-3, moved on at row step s-2
+        if constexpr (PAL) {
+            idr2[(PH + 5) % 6] = idr[(PH + 5) % 6];  // row s-1 moves to the older ring
+            idr[(PH + 3) % 6] = ldm(s + 3);           // into the slot of row s-3, moved on at row step s-2
+        }
 
         // ---- x neighbours through LDS: step-n velocities of row s, step-(n+1) velocities of row s-3
         {
@@ -356,8 +391,8 @@ __device__ __forceinline__ void step2_tile(StepArgs<T> A, const int tile, const 
 
         // ---- stage 1: stresses of row s (step n), kept for stage 2 in the register ring
         T sxx, p, syy;
-        const Vec4<T> s1 = stress(s, V(-2), V(-1), V(0), V(1), V(2), so, lv[PH % 3], idr[PH % 6], idr[(PH + 1) % 6], px_in, pxm,
-                                  pz_in, pzm, ps1, src1, sxx, p, syy);
+        const Vec4<T> s1 = stress(s, V(-2), V(-1), V(0), V(1), V(2), so, lv[PH % 3], idr[PH % 6], idr[(PH + 1) % 6],
+                                  msr[PH % 6], msr[(PH + 1) % 6], px_in, pxm, pz_in, pzm, ps1, src1, sxx, p, syy);
         s1r[PH % 6] = s1;
         {
             T(*xs)[BW + 4] = ls[PH % 3];
@@ -375,8 +410,8 @@ __device__ __forceinline__ void step2_tile(StepArgs<T> A, const int tile, const 
         // ---- stage 1: velocities of row s-2 (step n -> n+1); stage 2 reads rows [zA-4, zB+3]
         if (s - 2 >= zA - 4 && s - 2 <= zB + 3)
             V1(-2) = velocity(s - 2, V(-2), ls[(PH + 1) % 3], SXY(-4), SXY(-3), SXY(-2), SXY(-1), SYY(-3), SYY(-2),
-                              SYY(-1), SYY(0), PP(-3), PP(-2), PP(-1), PP(0), idr2[(PH + 4) % 6], idr[(PH + 5) % 6], px_in, pxm,
-                              pz_in, pzm, pv1, src1);
+                              SYY(-1), SYY(0), PP(-3), PP(-2), PP(-1), PP(0), idr2[(PH + 4) % 6], idr[(PH + 5) % 6],
+                              mvr[(PH + 4) % 6], mvr[(PH + 5) % 6], px_in, pxm, pz_in, pzm, pv1, src1);
         else
             V1(-2) = V(-2);
         if constexpr (EDGE) V(4) = ld(Vo, s + 4);  // into the slot of row s-2, now dead
@@ -389,8 +424,8 @@ __device__ __forceinline__ void step2_tile(StepArgs<T> A, const int tile, const 
         Vec4<T> s2 = zero4;
         if (r2 >= zA - 2 && r2 <= zB + 1) {
             s2 = stress(r2, V1(-6), V1(-5), V1(-4), V1(-3), V1(-2), s1r[(PH + 2) % 6], lv2[(PH + 2) % R2],
-                        idr2[(PH + 2) % 6], idr2[(PH + 3) % 6], pxm, px_out, pzm, pz_out, srow, src2, sxx2,
-                        p2, syy2);
+                        idr2[(PH + 2) % 6], idr2[(PH + 3) % 6], msr[(PH + 2) % 6], msr[(PH + 3) % 6], pxm, px_out,
+                        pzm, pz_out, srow, src2, sxx2, p2, syy2);
             if (srow) *(GN *)(Sn + off(r2)) = nvec4<T>{s2.x, s2.y, s2.z, s2.w};
         }
         {
@@ -412,8 +447,13 @@ __device__ __forceinline__ void step2_tile(StepArgs<T> A, const int tile, const 
             const bool vrow = own;
             const Vec4<T> v2 = velocity(j2, V1(-6), ls2[PH % 3], SXY2(-8), SXY2(-7), SXY2(-6), SXY2(-5), SYY2(-7),
                                         SYY2(-6), SYY2(-5), SYY2(-4), PP2(-7), PP2(-6), PP2(-5), PP2(-4),
-                                        idr2[PH % 6], idr2[(PH + 1) % 6], pxm, px_out, pzm, pz_out, vrow, src2);
+                                        idr2[PH % 6], idr2[(PH + 1) % 6], mvr[PH % 6], mvr[(PH + 1) % 6], pxm, px_out,
+                                        pzm, pz_out, vrow, src2);
             if (vrow) *(GN *)(Vn + off(j2)) = nvec4<T>{v2.x, v2.y, v2.z, v2.w};
+        }
+        if constexpr (!PAL) {
+            msr[(PH + 2) % 6] = ld(MSg, s + 2);  // the slot of row s-4, used by stage 2 above for the last time
+            mvr[PH % 6] = ld(MVg, s);            // the slot of row s-6, likewise
         }
         ++s;
     };
@@ -436,8 +476,8 @@ __device__ __forceinline__ void step2_tile(StepArgs<T> A, const int tile, const 
 // latency-bound and about twice as long as interior tiles: started first, they overlap with the
 // interior tiles that follow instead of finishing the launch on their own.  Items with a negative
 // model index pad the shares to equal length.
-template <class T, int BW, bool AVG>
-__global__ void __launch_bounds__(BW, 3)
+template <class T, int BW, bool AVG, bool PAL>
+__global__ void __launch_bounds__(BW, PAL ? 3 : 2)
     fused_step2(const StepArgs<T> *__restrict__ models, const WorkItem *__restrict__ work, int nwork, int per_xcd,
                 const long long *__restrict__ base, int k)
 {
@@ -452,35 +492,38 @@ __global__ void __launch_bounds__(BW, 3)
     if (item.model < 0) return;
     const long long n = *(const kcll *)base + k;  // the launch advances steps n and n + 1
     if (item.tile < 0)
-        step2_tile<T, BW, AVG, true>(models[item.model], ~item.tile, n, lv, ls, lv2, ls2, lpal);
+        step2_tile<T, BW, AVG, true, PAL>(models[item.model], ~item.tile, n, lv, ls, lv2, ls2, lpal);
     else
-        step2_tile<T, BW, AVG, false>(models[item.model], item.tile, n, lv, ls, lv2, ls2, lpal);
+        step2_tile<T, BW, AVG, false, PAL>(models[item.model], item.tile, n, lv, ls, lv2, ls2, lpal);
 }
 
-template <class T, bool AVG>
+template <class T, bool AVG, bool PAL>
 hipError_t launch_fused2_bw(const StepArgs<T> *models, const WorkItem *work, int nwork, int per_xcd, int bw,
                             const long long *base, int k, hipStream_t stream)
 {
     const dim3 grid(8 * per_xcd);
     switch (bw) {
-    case 64: fused_step2<T, 64, AVG><<<grid, 64, 0, stream>>>(models, work, nwork, per_xcd, base, k); break;
-    case 128: fused_step2<T, 128, AVG><<<grid, 128, 0, stream>>>(models, work, nwork, per_xcd, base, k); break;
-    case 256: fused_step2<T, 256, AVG><<<grid, 256, 0, stream>>>(models, work, nwork, per_xcd, base, k); break;
+    case 64: fused_step2<T, 64, AVG, PAL><<<grid, 64, 0, stream>>>(models, work, nwork, per_xcd, base, k); break;
+    case 128: fused_step2<T, 128, AVG, PAL><<<grid, 128, 0, stream>>>(models, work, nwork, per_xcd, base, k); break;
+    case 256: fused_step2<T, 256, AVG, PAL><<<grid, 256, 0, stream>>>(models, work, nwork, per_xcd, base, k); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
 template <class T>
-hipError_t launch_fused2(const StepArgs<T> *models, const WorkItem *work, int nwork, int per_xcd, int bw, bool avg,
-                         const long long *base, int k, hipStream_t stream)
+hipError_t launch_fused2(const StepArgs<T> *models, const WorkItem *work, int nwork, int per_xcd, int bw, bool palette,
+                         bool avg, const long long *base, int k, hipStream_t stream)
 {
     if (nwork < 1 || per_xcd < 1 || 8LL * per_xcd < nwork) return hipErrorInvalidValue;
-    return avg ? launch_fused2_bw<T, true>(models, work, nwork, per_xcd, bw, base, k, stream)
-               : launch_fused2_bw<T, false>(models, work, nwork, per_xcd, bw, base, k, stream);
+    if (palette)
+        return avg ? launch_fused2_bw<T, true, true>(models, work, nwork, per_xcd, bw, base, k, stream)
+                   : launch_fused2_bw<T, false, true>(models, work, nwork, per_xcd, bw, base, k, stream);
+    return avg ? launch_fused2_bw<T, true, false>(models, work, nwork, per_xcd, bw, base, k, stream)
+               : launch_fused2_bw<T, false, false>(models, work, nwork, per_xcd, bw, base, k, stream);
 }
 
-template hipError_t launch_fused2<float>(const StepArgs<float> *, const WorkItem *, int, int, int, bool,
+template hipError_t launch_fused2<float>(const StepArgs<float> *, const WorkItem *, int, int, int, bool, bool,
                                          const long long *, int, hipStream_t);
 
 }  // namespace poe2d

@@ -505,6 +505,11 @@ poe2d_status build_list(StepList &L, const std::vector<poe2d_solver *> &members,
 {
     const int nm = (int)members.size();
     const int material = members[0]->grid.staggered_material;
+    // a palette launch reads each member's material index and palette: refuse members without one
+    // (or, for the two-step kernel, with more entries than its LDS copy holds)
+    for (const poe2d_solver *m : members)
+        if (palette && (m->m->npal < 1 || !m->m->mid || !m->m->pal || (two && m->m->npal > kMaxPalette2)))
+            return fail(POE2D_E_INVALID, "run: a palette launch over a grid without a usable palette");
     for (const poe2d_solver *m : members)
         if (m->grid.staggered_material != material)
             return fail(POE2D_E_INVALID, "run: batch members differ in staggered material");
@@ -637,7 +642,7 @@ hipError_t launch_list_chunk(const StepList &L, int count, hipStream_t stream)
         if constexpr (sizeof(T) == 4) {
             for (int k = 0; k < count; ++k) {
                 const hipError_t e = launch_fused2<T>((const StepArgs<T> *)L.models, L.work, L.nwork, L.per_xcd, L.bw,
-                                                      L.avg, L.base, 2 * k, stream);
+                                                      L.palette, L.avg, L.base, 2 * k, stream);
                 if (e != hipSuccess) return e;
             }
             return launch_base_advance(L.base, 2 * count, stream);
@@ -704,13 +709,15 @@ poe2d_status stability_check(poe2d_solver *s)
     return POE2D_OK;
 }
 
-// The kernel a handle's steps run on: schedule 3 takes the two-step kernel on FP32 grids whose
-// palette fits that kernel's LDS copy, and the single-step kernel (as schedule 0, 64-thread blocks
-// widened to 128) otherwise.
+// The kernel a handle's steps run on: schedule 3 takes the two-step kernel on FP32 grids (through
+// the palette where the grid's fits that kernel's LDS copy, else through the coefficient groups),
+// and the single-step kernel (as schedule 0, 64-thread blocks widened to 128) on FP64 grids.
 bool two_step_kernel(const poe2d_solver *s)
 {
-    return s->schedule == SCHED_FUSED2 && s->precision == POE2D_F32 && s->m->npal > 0 && s->m->npal <= kMaxPalette2;
+    return s->schedule == SCHED_FUSED2 && s->precision == POE2D_F32;
 }
+// Whether the two-step kernel reads handle s through its palette.
+bool two_step_palette(const poe2d_solver *s) { return s->m->npal > 0 && s->m->npal <= kMaxPalette2; }
 int effective_schedule(const poe2d_solver *s)
 {
     return s->schedule == SCHED_FUSED2 && !two_step_kernel(s) ? SCHED_FUSED : s->schedule;
@@ -742,10 +749,12 @@ poe2d_status run_steps(poe2d_solver *s, int nsteps)
         if (!s->psi_mid && (st = dev_alloc(&s->psi_mid, std::max<size_t>(s->m->psi_half_elems(), 1) * s->rs)))
             return st;
         if ((st = upload_source<T>(s, s->step, nsteps))) return st;
-        if (pairs && (st = build_list<T>(s->list2, {s}, s->block_w, s->tile_rows, true, s->stream, true))) return st;
-        // (the single-step kernel has no 64-thread blocks)
-        if (rem && (st = build_list<T>(s->list, {s}, std::max(s->block_w, 128), s->tile_rows, true, s->stream, false,
-                                       pairs)))
+        if (pairs && (st = build_list<T>(s->list2, {s}, s->block_w, s->tile_rows, two_step_palette(s), s->stream, true)))
+            return st;
+        // (the single-step kernel has no 64-thread blocks; it reads the palette whenever the grid
+        // has one, which may be larger than the two-step kernel's)
+        if (rem && (st = build_list<T>(s->list, {s}, std::max(s->block_w, 128), s->tile_rows, s->m->npal > 0,
+                                       s->stream, false, pairs)))
             return st;
         HIP_TRY(hipEventRecord(s->ev0, s->stream));
         if (pairs && (st = run_list<T>(s->list2, s->step, pairs, s->graphs, s->stream))) return st;
@@ -870,12 +879,13 @@ template <class T>
 poe2d_status batch_run_steps(poe2d_batch *b, int nsteps)
 {
     const long long n0 = b->members[0]->step;
-    bool palette = true, two = sizeof(T) == 4;
+    bool palette = true, two = sizeof(T) == 4, palette2 = true;
     for (poe2d_solver *s : b->members) {
         poe2d_status st = upload_source<T>(s, n0, nsteps);
         if (st) return st;
         palette = palette && (s->schedule == SCHED_FUSED || s->schedule == SCHED_FUSED2) && s->m->npal > 0;
         two = two && two_step_kernel(s);
+        palette2 = palette2 && two_step_palette(s);
     }
     // (the single-step kernel widens 64-thread blocks to 128, below)
     if (two && b->block_w == 512)
@@ -886,7 +896,7 @@ poe2d_status batch_run_steps(poe2d_batch *b, int nsteps)
         for (poe2d_solver *s : b->members)
             if (!s->psi_mid && (st = dev_alloc(&s->psi_mid, std::max<size_t>(s->m->psi_half_elems(), 1) * s->rs)))
                 return st;
-        if ((st = build_list<T>(b->list2, b->members, b->block_w, b->tile_rows, true, b->stream, true))) return st;
+        if ((st = build_list<T>(b->list2, b->members, b->block_w, b->tile_rows, palette2, b->stream, true))) return st;
     }
     if (rem && (st = build_list<T>(b->list, b->members, std::max(b->block_w, 128), b->tile_rows, palette, b->stream,
                                    false, pairs)))
@@ -1305,7 +1315,8 @@ std::string tune_key(const std::vector<poe2d_solver *> &members, int kernel)
                     std::to_string(kernel);
     std::vector<std::string> shapes;
     for (const poe2d_solver *s : members)
-        shapes.push_back(std::to_string(s->grid.nx) + "x" + std::to_string(s->grid.nz) + (s->m->npal > 0 ? "p" : ""));
+        shapes.push_back(std::to_string(s->grid.nx) + "x" + std::to_string(s->grid.nz) +
+                         (s->m->npal == 0 ? "" : s->m->npal <= kMaxPalette2 ? "p" : "q"));
     std::sort(shapes.begin(), shapes.end());
     for (const std::string &q : shapes) k += "/" + q;
     return k;

@@ -314,15 +314,17 @@ hipError_t launch_base_advance(long long *base, int delta, hipStream_t stream);
 // Output columns of one tile for block width bw (halo of 4 columns on each side).
 inline int fused_tile_width(int bw) { return bw - 8; }
 
-// The two-step kernel (FP32, palette grids): output columns per tile (halo of 6 columns on each
-// side), and the launch; n = *base + k is the first of the two steps.
+// The two-step kernel (FP32): output columns per tile (halo of 6 columns on each side), and the
+// launch; n = *base + k is the first of the two steps.  With `palette` every model is read through
+// its one-byte material index and a palette of at most kMaxPalette2 entries, else through the
+// coefficient groups.
 __host__ __device__ constexpr int fused2_tile_width(int bw) { return bw - 12; }
 constexpr int kMaxPalette2 = 28;  // palette entries the two-step kernel keeps in LDS (blocks of 64 threads: 12 per CU)
 // Items whose tile field is negative (~tile) are edge tiles: tiles touching a C-PML strip or
 // holding receivers (the others run a variant without that code).
 template <class T>
-hipError_t launch_fused2(const StepArgs<T> *models, const WorkItem *work, int nwork, int per_xcd, int bw, bool avg,
-                         const long long *base, int k, hipStream_t stream);
+hipError_t launch_fused2(const StepArgs<T> *models, const WorkItem *work, int nwork, int per_xcd, int bw, bool palette,
+                         bool avg, const long long *base, int k, hipStream_t stream);
 // Rows and columns around a two-step tile whose points the kernel computes (halos of both steps and
 // the over-run of its row loop): a tile with a C-PML strip point in this range is an edge tile.
 constexpr int kFused2RowsBefore = 10, kFused2RowsAfter = 12, kFused2ColsBefore = 6;

@@ -135,3 +135,31 @@ def test_config1_full_run_fp32_vs_fp64(c1):
     c, r = np.unravel_index(e.argmax(), e.shape)
     say(f"  FP32 vs FP64 max rel L2 = {e.max():.3e} (component {c}, receiver {r}); median {np.median(e):.2e}")
     assert e.max() <= 1e-4
+
+
+def test_continuous_model_whole_grid_two_step_vs_single_step():
+    """The continuously varying medium at configuration 1's size (no palette: the two-step kernel
+    reads the coefficient groups), from a random state: 21 steps on the two-step kernel equal the
+    single-step kernel bitwise over the whole grid."""
+    from poe2d_amd.configs import config1_continuous
+    prob = config1_continuous(nt=21)
+    init = random_state(prob.nx, seed=11)
+
+    def gpu(schedule, bw, th):
+        with Solver(prob) as s:
+            assert s.material_palette() == 0
+            s.set_schedule(schedule, bw, th)
+            for f, a in init.items():
+                s.set_field(f, a)
+            s.run(21)
+            return s.seismograms(), {f: s.snapshot(f) for f in ("vx", "vy", "wy", "sxy", "p")}, s.checkpoint()
+
+    say("continuous 4001^2 random state: two-step kernel on the coefficient groups, 21 steps")
+    two = gpu(3, 256, 24)
+    say("continuous 4001^2 random state: single-step kernel, 21 steps")
+    one = gpu(0, 256, 8)
+    assert np.array_equal(two[0], one[0])
+    for f in two[1]:
+        assert np.abs(two[1][f]).max() > 0, f
+        assert np.array_equal(two[1][f], one[1][f]), f
+    assert np.array_equal(two[2], one[2])

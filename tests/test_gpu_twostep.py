@@ -116,7 +116,7 @@ def test_two_step_parity_config0():
     assert_parity(g, prob, "two-step config0 1000 steps")
 
 
-def test_fp64_and_no_palette_fall_back():
+def test_fp64_falls_back_to_the_single_step_kernel():
     prob = _problem(nt=41)
     with Solver(prob, precision=_abi.F64) as s:
         s.set_schedule(3, 256, 12)
@@ -146,8 +146,8 @@ def _layered(nlayers, n=160, nt=61):
 
 @pytest.mark.parametrize("extra", [0, 1])
 def test_palette_size_limit(extra):
-    # at the two-step kernel's palette limit it runs; one entry more and the run takes the
-    # single-step kernel: either way the results equal schedule 0's bit for bit
+    # at the two-step kernel's palette limit it reads the palette; one entry more and it reads the
+    # coefficient groups: either way the results equal schedule 0's bit for bit
     prob = _layered(_abi.MAX_PALETTE_TWO_STEP + extra)
     with Solver(prob) as s:
         assert s.material_palette() == _abi.MAX_PALETTE_TWO_STEP + extra
@@ -198,3 +198,57 @@ def test_batch_two_step_equals_members_alone():
                 _same(_state(c), _state(ref), "batch clone")
         finally:
             c.close()
+
+
+def _graded(n=300, nt=151, kind=_abi.SRC_PRESSURE_DGAUSS, **opts):
+    # porosity grading with depth and offset: every row and column its own coefficients (no palette)
+    fields = biot_fields(n, n, random_patchy_mask(n, n, 0.2, seed=31), BRINE, dict(GAS, mu_fr=6.0e9))
+    ramp = np.linspace(-1, 1, n + 1)
+    fields["phi"] = fields["phi"] * (1.0 + 0.15 * ramp[:, None]) * (1.0 + 0.05 * ramp[None, :])
+    coeffs = material_biot(**fields)
+    ix = np.unique(np.r_[np.arange(25, n - 24, 7), [115, 116, 117, 243, 244, 245]]).astype(np.int32)
+    ix = ix[ix < n - 20]
+    rec = ReceiverSpec(ix=ix, iz=np.full(len(ix), 2 * n // 3, np.int32))
+    return Problem(nx=n, nz=n, dx=0.5, dz=0.5, dt=5e-5, nt=nt, coeffs=coeffs,
+                   source=SourceSpec(kind=kind, ix=150, iz=140, amp=1e10), receivers=rec, npml=20, f0=80.0, **opts)
+
+
+@pytest.mark.parametrize("material", [_abi.MATERIAL_AS_WRITTEN, _abi.MATERIAL_AVERAGED])
+def test_two_step_without_palette_equals_single_step(material):
+    """A grid with more coefficient tuples than the palette holds runs the two-step kernel through
+    the coefficient groups (material read where used, velocity constants per point): bitwise equal
+    to the single-step kernel, over tile shapes, an odd step count and both staggered-material
+    settings."""
+    prob = _graded(staggered_material=material)
+    with Solver(prob) as s:
+        assert s.material_palette() == 0
+    ref = _run(prob, 0, 256, 14)
+    for bw, th in ((256, 12), (128, 18), (64, 24)):
+        say(f"two-step on the coefficient groups, material {material}: bw {bw} th {th}")
+        _same(_run(prob, 3, bw, th), ref, f"groups bw {bw} th {th}")
+
+
+def test_two_step_continuous_model():
+    """A medium continuous in depth and offset (porosity and frame moduli graded, every point its
+    own coefficients): the two-step kernel equals the single-step kernel."""
+    from poe2d_amd.configs import config1_continuous
+    prob = config1_continuous(nt=121, n=300)
+    ref = _run(prob, 0, 256, 14)
+    assert np.abs(ref[0]).max() > 0
+    _same(_run(prob, 3, 256, 24), ref, "continuous model")
+
+
+def test_batch_mixing_palette_and_groups():
+    """A batch of a palette grid and a grid without one on the two-step kernel (the batch reads
+    every member through the groups) equals each member alone on the single-step kernel."""
+    from poe2d_amd.solver import Batch
+    probs = [_problem(n=200, nt=91, seed=3, src=(100, 90)), _graded(n=240, nt=91)]
+    alone = [_run(p, 0, 256, 14) for p in probs]
+    with Solver(probs[0]) as a, Solver(probs[1]) as b:
+        for s in (a, b):
+            s.set_schedule(3, 256, 12)
+        with Batch([a, b]) as batch:
+            batch.set_schedule(128, 18)
+            batch.run(91)
+        _same(_state(a), alone[0], "palette member")
+        _same(_state(b), alone[1], "groups member")
