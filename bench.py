@@ -406,6 +406,7 @@ def surrogate_bench(args):
     quantiles = [float(q) for q in args.sur_quantiles.split(",")]
     _, verr, vspread = sur.calibrate(params[ntr:], gathers[ntr:], bound=0.05, quantile=quantiles[0])
     del gathers
+    dump = {"val_params": params[ntr:], "val_err": verr, "val_spread": vspread}
     cand = S.draw_params(ncand, seed=4)
     screens = []
     for q in quantiles:
@@ -414,6 +415,9 @@ def surrogate_bench(args):
             f"[surrogate] re-solved {d} of {n}", file=sys.stderr, flush=True))
         je = res.judge_errors
         ffb = len(res.fallback) / res.n
+        tag = f"p{round(q * 100)}"
+        dump.update({f"{tag}_threshold": thr, f"{tag}_fallback": res.fallback, f"{tag}_judged": res.judged,
+                     f"{tag}_judge_err": je})
         value = res.models_per_second
         print(f"[surrogate] bound p{round(q * 100)} <= 5 %: {value:.1f} models/s, fallback {ffb:.4f}",
               file=sys.stderr, flush=True)
@@ -430,6 +434,9 @@ def surrogate_bench(args):
             "after_fallback_rel_l2_p50": round(S.mixture_quantile(je, ffb, 0.5), 4),
             "after_fallback_rel_l2_p99": round(S.mixture_quantile(je, ffb, 0.99), 4)})
     sur.close()
+    if args.sur_dump:
+        # the held-out errors and spreads and the screening outcome, for study off the GPU
+        np.savez_compressed(args.sur_dump, cand_params=cand, **dump)
     first = screens[0]
     print(json.dumps({
         "metric": "surrogate-screened models/s (configuration 4 screening, configuration 3 geometry)",
@@ -473,6 +480,9 @@ def main():
     ap.add_argument("--sur-receivers", type=int, default=1,
                     help="surrogate: 1 = per-receiver gas inputs besides the four parameters, 0 = the "
                          "four parameters alone")
+    ap.add_argument("--sur-dump", default="", metavar="NPZ",
+                    help="surrogate: write the held-out errors and spreads and each screen's fallback, judged "
+                         "models and their errors to this file")
     ap.add_argument("--sur-quantiles", default="0.5,0.99",
                     help="surrogate: error quantiles after fallback held within 5 %% by the threshold, one "
                          "screening pass each (the first gives the line's value)")

@@ -246,6 +246,15 @@ poe2d_status poe2d_set_graphs(poe2d_solver *s, int enable);
  * (.cu:1615-1736 inside the shot loop), re-reading and re-uploading the model. */
 poe2d_status poe2d_clone(poe2d_solver *s, poe2d_solver **out);
 
+/* Replace the coefficients of a handle's grid (same grid, source and receivers) and reset it, so
+ * a handle serves one model after another without device allocations: the new P velocity's
+ * Courant check and C-PML profiles, the coefficient groups and the material palette, then the
+ * zero state of poe2d_reset.  The reference re-reads its model files and re-derives the
+ * coefficients per run (seismic_CPML_2D_poroelastic_withCUDA.cu:604-713) and keeps its device
+ * arrays across shots (zero_kernel, :1736).  POE2D_E_STATE when the model is shared with clones;
+ * a refused set (POE2D_E_CFL, POE2D_E_INVALID) leaves the handle as it was. */
+poe2d_status poe2d_set_coeffs(poe2d_solver *s, const poe2d_coeffs *coeffs);
+
 /* ---- Batches: several handles advanced in lock-step by one launch per time step ----
  * A batch is a view over existing handles (distinct handles, same device and precision,
  * single-pass schedules, ragged grids allowed); they keep their own results, read with the

@@ -123,6 +123,7 @@ def _declare(lib):
         "poe2d_autotune": ([vp, i, c_int_p, c_int_p], i),
         "poe2d_set_graphs": ([vp, i], i),
         "poe2d_clone": ([vp, ctypes.POINTER(vp)], i),
+        "poe2d_set_coeffs": ([vp, ctypes.POINTER(Coeffs)], i),
         "poe2d_batch_create": ([i, ctypes.POINTER(vp), ctypes.POINTER(vp)], i),
         "poe2d_batch_set_schedule": ([vp, i, i], i),
         "poe2d_batch_set_graphs": ([vp, i], i),

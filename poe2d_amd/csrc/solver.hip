@@ -306,8 +306,12 @@ poe2d_status upload_coeffs(Model *md, const poe2d_coeffs *c)
     auto at = [&](const void *a, size_t i) {
         return a == c->lambdac ? effective_lambda(g, raw(c->lambdac, i), raw(c->alpha, i), raw(c->M, i)) : raw(a, i);
     };
-    std::vector<Vec4<T>> host(md->group_points(), Vec4<T>{T(0), T(0), T(0), T(0)});
+    // both groups are filled and checked before either is uploaded, so a refused set leaves the
+    // model's coefficients as they were
+    std::vector<Vec4<T>> hgrp[2];
     for (int grp = 0; grp < 2; ++grp) {
+        std::vector<Vec4<T>> &host = hgrp[grp];
+        host.assign(md->group_points(), Vec4<T>{T(0), T(0), T(0), T(0)});
         const void **src = grp == 0 ? ms_src : mv_src;
         for (int z = 0; z <= g.nz; ++z)
             for (int x = 0; x <= g.nx; ++x) {
@@ -326,9 +330,10 @@ poe2d_status upload_coeffs(Model *md, const poe2d_coeffs *c)
                 }
                 host[md->origin() + (size_t)z * md->pitch + x] = Vec4<T>{(T)v[0], (T)v[1], (T)v[2], (T)v[3]};
             }
-        HIP_TRY(hipMemcpy((unsigned char *)md->mat + (size_t)(grp == 0 ? G_MS : G_MV) * md->group_bytes(), host.data(),
-                          md->group_bytes(), hipMemcpyHostToDevice));
     }
+    for (int grp = 0; grp < 2; ++grp)
+        HIP_TRY(hipMemcpy((unsigned char *)md->mat + (size_t)(grp == 0 ? G_MS : G_MV) * md->group_bytes(),
+                          hgrp[grp].data(), md->group_bytes(), hipMemcpyHostToDevice));
 
     // palette: distinct 8-tuples in order of first appearance
     using Key = std::array<T, 8>;
@@ -358,17 +363,21 @@ poe2d_status upload_coeffs(Model *md, const poe2d_coeffs *c)
             }
             idx[md->origin() + (size_t)z * md->pitch + x] = (unsigned char)last_id;
         }
+    md->npal = 0;
     if (pal.size() > (size_t)kMaxPalette) return POE2D_OK;  // no palette: the groups serve every schedule
     std::vector<Vec4<T>> hp(PAL_SECTIONS * kMaxPalette, Vec4<T>{T(0), T(0), T(0), T(0)});
     for (size_t e = 0; e < pal.size(); ++e) {
         hp[PAL_MS * kMaxPalette + e] = Vec4<T>{pal[e][0], pal[e][1], pal[e][2], pal[e][3]};
         hp[PAL_MV * kMaxPalette + e] = Vec4<T>{pal[e][4], pal[e][5], pal[e][6], pal[e][7]};
     }
-    void *q = nullptr;
-    poe2d_status st = dev_alloc(&q, md->group_points());
-    if (st) return st;
-    md->mid = (unsigned char *)q;
-    if ((st = dev_alloc(&md->pal, hp.size() * sizeof(Vec4<T>)))) return st;
+    // (a model whose coefficients are replaced keeps the index and palette arrays it has)
+    poe2d_status st = POE2D_OK;
+    if (!md->mid) {
+        void *q = nullptr;
+        if ((st = dev_alloc(&q, md->group_points()))) return st;
+        md->mid = (unsigned char *)q;
+    }
+    if (!md->pal && (st = dev_alloc(&md->pal, hp.size() * sizeof(Vec4<T>)))) return st;
     HIP_TRY(hipMemcpy(md->mid, idx.data(), idx.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(md->pal, hp.data(), hp.size() * sizeof(Vec4<T>), hipMemcpyHostToDevice));
     md->npal = (int)pal.size();
@@ -377,7 +386,8 @@ poe2d_status upload_coeffs(Model *md, const poe2d_coeffs *c)
         // would compute them per point (the same expressions and reciprocal)
         launch_cn_palette((Vec4<float> *)md->pal, md->npal, (float)(1.0 / g.dt), nullptr);
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipDeviceSynchronize());
+        // (the null stream only: other handles' streams are non-blocking and may be busy)
+        HIP_TRY(hipStreamSynchronize(nullptr));
     }
     return POE2D_OK;
 }
@@ -418,6 +428,39 @@ std::vector<int> strip_map(const std::vector<double> *t, int n, int *count, int 
     *lo = l;
     *hi = h;
     return map;
+}
+
+// The P velocity of the model's coefficients (unless the grid fixes it), the Courant check and the
+// C-PML profiles that follow from them, into md->cp_max and md->htab; the model is left unchanged
+// when the check fails.
+poe2d_status model_profiles(Model *md, const poe2d_coeffs *c, const char *what)
+{
+    const poe2d_grid *g = &md->grid;
+    const double cp = g->cp_max > 0 ? g->cp_max : cp_max_from_coeffs(*g, *c);
+    if (!(cp > 0) || !std::isfinite(cp))
+        return fail(POE2D_E_INVALID, std::string(what) + ": coefficients give no positive P velocity");
+    const double courant = cp * g->dt / std::min(g->dx, g->dz);
+    if (g->check_cfl && courant > 6.0 / 7.0 / std::sqrt(2.0))
+        return fail(POE2D_E_CFL, std::string(what) + ": time step is too large, simulation will be unstable (Courant " +
+                                     std::to_string(courant) + " > 6/7/sqrt(2))");
+    double amax = g->alpha_max;
+    if (amax <= 0)
+        amax = g->precision == POE2D_F32 ? (double)(float)(2.e0 * kRefPiF * ((float)g->f0 / 2.e0))
+                                         : 3.14159265358979323846 * g->f0;
+    std::vector<double> tab[NTABLES];
+    for (int axis = 0; axis < 2; ++axis) {
+        const int n = (axis == 0 ? g->nx : g->nz) + 1;
+        std::vector<double> *t = tab + (axis == 0 ? TB_AX : TB_AZ);
+        for (int k = 0; k < 6; ++k) t[k].assign(n, 0.0);
+        poe2d_status st = poe2d_cpml_profile(axis, n, axis == 0 ? g->dx : g->dz, g->dt, g->npml,
+                                             g->npower, g->rcoef, g->kmax, amax, cp,
+                                             g->precision, t[0].data(), t[1].data(), t[2].data(),
+                                             t[3].data(), t[4].data(), t[5].data());
+        if (st != POE2D_OK) return st;
+    }
+    md->cp_max = cp;
+    for (int k = 0; k < NTABLES; ++k) md->htab[k].swap(tab[k]);
+    return POE2D_OK;
 }
 
 // Arguments of the step kernels for handle s, reading state buffer `in` and writing `out`, with
@@ -978,27 +1021,8 @@ poe2d_status poe2d_create(const poe2d_grid *g, const poe2d_coeffs *c, poe2d_solv
     if (e != hipSuccess) return bail(fail(POE2D_E_HIP, std::string("create: no HIP device: ") + hipGetErrorString(e)));
 
     // stability and C-PML set-up
-    md->cp_max = g->cp_max > 0 ? g->cp_max : cp_max_from_coeffs(*g, *c);
-    if (!(md->cp_max > 0) || !std::isfinite(md->cp_max))
-        return bail(fail(POE2D_E_INVALID, "create: coefficients give no positive P velocity"));
-    const double courant = md->cp_max * g->dt / std::min(g->dx, g->dz);
-    if (g->check_cfl && courant > 6.0 / 7.0 / std::sqrt(2.0))
-        return bail(fail(POE2D_E_CFL, "create: time step is too large, simulation will be unstable (Courant " +
-                                          std::to_string(courant) + " > 6/7/sqrt(2))"));
-    double amax = g->alpha_max;
-    if (amax <= 0)
-        amax = g->precision == POE2D_F32 ? (double)(float)(2.e0 * kRefPiF * ((float)g->f0 / 2.e0))
-                                         : 3.14159265358979323846 * g->f0;
-    for (int axis = 0; axis < 2; ++axis) {
-        const int n = (axis == 0 ? g->nx : g->nz) + 1;
-        std::vector<double> *t = md->htab + (axis == 0 ? TB_AX : TB_AZ);
-        for (int k = 0; k < 6; ++k) t[k].assign(n, 0.0);
-        poe2d_status st = poe2d_cpml_profile(axis, n, axis == 0 ? g->dx : g->dz, g->dt, g->npml,
-                                             g->npower, g->rcoef, g->kmax, amax, md->cp_max,
-                                             g->precision, t[0].data(), t[1].data(), t[2].data(),
-                                             t[3].data(), t[4].data(), t[5].data());
-        if (st != POE2D_OK) return bail(st);
-    }
+    poe2d_status st = model_profiles(md, c, "create");
+    if (st) return bail(st);
     std::vector<int> xm = strip_map(md->htab + TB_AX, g->nx + 1, &md->wxs, &md->xlo, &md->xhi);
     std::vector<int> zm = strip_map(md->htab + TB_AZ, g->nz + 1, &md->wzs, &md->ztop, &md->zbot);
     md->tn = std::max(g->nx, g->nz) + 1;
@@ -1008,7 +1032,6 @@ poe2d_status poe2d_create(const poe2d_grid *g, const poe2d_coeffs *c, poe2d_solv
     // the single-pass kernel addresses a group with 32-bit byte offsets
     if (md->group_bytes() >= (1ull << 32))
         return bail(fail(POE2D_E_INVALID, "create: grid too large (one field group exceeds 4 GiB)"));
-    poe2d_status st = POE2D_OK;
     if ((st = dev_alloc_zero(&md->mat, NMAT_GROUPS * md->group_bytes(), nullptr))) return bail(st);
     if ((st = dev_alloc_zero(&md->tab, (size_t)NTABLES * md->tn * md->rs, nullptr))) return bail(st);
     void *q = nullptr;
@@ -1045,6 +1068,45 @@ poe2d_status poe2d_clone(poe2d_solver *s, poe2d_solver **out)
     c->graphs = s->graphs;
     *out = c;
     return POE2D_OK;
+}
+
+poe2d_status poe2d_set_coeffs(poe2d_solver *s, const poe2d_coeffs *c)
+{
+    poe2d_status st = check_handle(s);
+    if (st) return st;
+    Model *md = s->m;
+    const poe2d_grid &g = md->grid;
+    if (md->refs > 1) return fail(POE2D_E_STATE, "set_coeffs: the model is shared with clones");
+    if (!c) return fail(POE2D_E_INVALID, "set_coeffs: null coefficients");
+    if (c->dtype != POE2D_F32 && c->dtype != POE2D_F64)
+        return fail(POE2D_E_INVALID, "set_coeffs: coefficient dtype must be POE2D_F32 or POE2D_F64");
+    if (!c->rho || !c->rhof || !c->m || !c->mu || !c->lambdac || !c->M || !c->alpha || !c->b)
+        return fail(POE2D_E_INVALID, "set_coeffs: null coefficient array");
+    if (c->pitch && c->pitch < (size_t)g.nx + 1) return fail(POE2D_E_INVALID, "set_coeffs: coefficient pitch < nx+1");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    // The C-PML profiles scale with the P velocity; their strips (where a profile is active) do not,
+    // so the memory variables keep their layout.
+    // A refused set (Courant check, invalid coefficients) leaves the handle as it was.
+    const int wxs = md->wxs, wzs = md->wzs, xlo = md->xlo, xhi = md->xhi, ztop = md->ztop, zbot = md->zbot;
+    const double old_cp = md->cp_max;
+    std::vector<double> old_tab[NTABLES];
+    for (int k = 0; k < NTABLES; ++k) old_tab[k] = md->htab[k];
+    auto restore = [&](poe2d_status e) {
+        md->cp_max = old_cp;
+        for (int k = 0; k < NTABLES; ++k) md->htab[k].swap(old_tab[k]);
+        return e;
+    };
+    if ((st = model_profiles(md, c, "set_coeffs"))) return st;
+    int nwx, nwz, lo, hi, top, bot;
+    strip_map(md->htab + TB_AX, g.nx + 1, &nwx, &lo, &hi);
+    strip_map(md->htab + TB_AZ, g.nz + 1, &nwz, &top, &bot);
+    if (nwx != wxs || nwz != wzs || lo != xlo || hi != xhi || top != ztop || bot != zbot)
+        return restore(fail(POE2D_E_INVALID, "set_coeffs: the C-PML strips would change (create a new handle)"));
+    st = md->precision == POE2D_F32 ? upload_coeffs<float>(md, c) : upload_coeffs<double>(md, c);
+    if (st) return restore(st);
+    st = md->precision == POE2D_F32 ? upload_tables<float>(md) : upload_tables<double>(md);
+    if (st) return st;
+    return poe2d_reset(s);
 }
 
 poe2d_status poe2d_set_schedule(poe2d_solver *s, int schedule, int block_w, int tile_rows)

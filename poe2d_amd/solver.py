@@ -241,6 +241,14 @@ class Solver(HandleSolver):
             c.set_receivers(receivers, c.problem.nt)
         return c
 
+    def set_coeffs(self, coeffs, coeff_dtype=None):
+        """Replace the grid's coefficients (dict of (nz+1, nx+1) arrays) and reset the handle: the
+        same handle then solves another model of the same grid without device allocations."""
+        cdt = coeff_dtype or self.real
+        bufs = CoeffBuffers(coeffs, self.problem.nx, self.problem.nz, cdt)
+        self._check(self._lib.poe2d_set_coeffs(self._h, ctypes.byref(bufs.struct)), "set_coeffs")
+        self.problem = dataclasses.replace(self.problem, coeffs=coeffs)
+
     def autotune(self, nsteps=10):
         """Time the single-pass tile shapes on this grid, keep the fastest, reset the handle;
         returns (block_w, tile_rows)."""

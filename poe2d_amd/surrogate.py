@@ -135,7 +135,9 @@ def features(params, geom=CONFIG3, receivers=False):
     return (INPUT_SCALE * f).astype(np.float32)
 
 
-def model_problem(p, seed=LAYOUT_SEED, geom=CONFIG3):
+def model_problem(p, seed=LAYOUT_SEED, geom=CONFIG3, dtype=np.float64):
+    """The problem of parameter set p: its two materials' coefficients placed by the gas mask, as
+    `dtype` arrays (float32 is what an FP32 handle rounds them to itself)."""
     gas, radius, phi, kappa = (float(v) for v in p)
     n, dx = geom["n"], geom["dx"]
     mask = patchy_mask(n, n, dx, dx, gas, seed, radius, radius) if gas > 0 else None
@@ -145,8 +147,12 @@ def model_problem(p, seed=LAYOUT_SEED, geom=CONFIG3):
     # whole grid), then placed by the mask
     pair = material_biot(**biot_fields(1, 0, np.array([[False, True]]), rock, fluid))
     shape = (n + 1, n + 1)
-    coeffs = {k: (np.where(mask, v[0, 1], v[0, 0]) if mask is not None else np.full(shape, v[0, 0]))
-              for k, v in pair.items()}
+    coeffs = {}
+    for k, v in pair.items():
+        a = np.full(shape, v[0, 0], dtype=dtype)
+        if mask is not None:
+            a[mask.astype(bool, copy=False)] = v[0, 1]
+        coeffs[k] = a
     c = n // 2
     ix, iz = receiver_points(geom)
     rec = ReceiverSpec(ix=ix, iz=iz, decimate=geom["dec"])
@@ -184,40 +190,154 @@ class SolveTiming:
         return self.n / self.gpu_seconds
 
 
-def simulate(params, seeds=None, geom=CONFIG3, batch=64, workers=8, progress=None):
+class SolverPool:
+    """Solver handles of one geometry and the batches that advance them, reused from model to model:
+    a model enters a handle through poe2d_set_coeffs (its coefficients replaced, the state reset), so
+    after the first batch no device memory is allocated per model."""
+
+    def __init__(self, problems):
+        self.solvers = []
+        self.batches = {}
+        try:
+            for p in problems:
+                s = Solver(p)
+                s.set_schedule(3, *TILE)  # two steps per launch (bitwise equal to one)
+                self.solvers.append(s)
+        except Exception:
+            self.close()
+            raise
+
+    @property
+    def size(self):
+        return len(self.solvers)
+
+    def load(self, problems, executor=None):
+        """The first len(problems) members take these models (same geometry), on the executor's
+        threads when one is given (each handle's set-up is independent of the others')."""
+        pairs = list(zip(self.solvers, problems))
+        if executor is None:
+            for s, p in pairs:
+                s.set_coeffs(p.coeffs)
+        else:
+            list(executor.map(lambda sp: sp[0].set_coeffs(sp[1].coeffs), pairs))
+
+    def has_batch(self, k):
+        return k in self.batches
+
+    def batch(self, k):
+        """The batch of the first k members.  On first use it is made, its tile shape autotuned and
+        its step graph captured by a warm-up run (the members are reset afterwards): nothing else
+        may use the GPU from another thread meanwhile, as HIP refuses other threads' calls while a
+        stream captures.  Later runs replay the graph."""
+        if k not in self.batches:
+            b = Batch(self.solvers[:k])
+            self.batches[k] = b
+            b.set_schedule(*TILE)
+            b.autotune(4)  # timed once per batch class in the process, then cached
+            b.run(32)      # one graph of 16 step pairs: captured here
+            for s in self.solvers[:k]:
+                s.reset()
+        return self.batches[k]
+
+    def close(self):
+        for b in self.batches.values():
+            b.close()
+        self.batches = {}
+        for s in self.solvers:
+            s.close()
+        self.solvers = []
+
+
+class SimulatePools:
+    """The two pools simulate() alternates between (one solving on the GPU while the host loads the
+    other), kept across calls of one geometry."""
+
+    def __init__(self, batch=64):
+        self.batch_size = batch
+        self.pools = [None, None]
+
+    def get(self, i, problems, executor=None):
+        """Pool i holding `problems` in its first members: made for them, or loaded with them."""
+        p = self.pools[i]
+        if p is None or p.size < len(problems):
+            if p is not None:
+                p.close()
+            p = self.pools[i] = SolverPool(problems)
+        else:
+            p.load(problems, executor)
+        return p
+
+    def close(self):
+        for p in self.pools:
+            if p is not None:
+                p.close()
+        self.pools = [None, None]
+
+
+def simulate(params, seeds=None, geom=CONFIG3, batch=64, workers=8, progress=None, pools=None):
     """Gathers [n, 3, nrec, nsamp] (v_x, v_y, p) of the models, solved in batches of `batch`
-    handles advanced together while host threads build the next models; returns (gathers,
-    SolveTiming).  seeds: patch-layout seed per model (LAYOUT_SEED for all when None); progress:
-    called with (models done, n) after each batch."""
+    handles advanced together; returns (gathers, SolveTiming).  Host threads build the models'
+    coefficients ahead, and two pools of handles alternate: while one batch runs on the GPU the host
+    loads the next models into the other pool (poe2d_set_coeffs) and reads back the batch before.
+    seeds: patch-layout seed per model (LAYOUT_SEED for all when None); progress: called with
+    (models done, n) after each batch; pools: a SimulatePools of this geometry to reuse (made and
+    closed here when None)."""
     t0 = time.perf_counter()
     n = len(params)
     seeds = np.full(n, LAYOUT_SEED) if seeds is None else np.asarray(seeds)
     out = np.zeros((n,) + gather_shape(geom), np.float32)
+    own = pools is None
+    if own:
+        pools = SimulatePools(batch)
+    batch = pools.batch_size
     gpu_ms = 0.0
-    build = lambda k: model_problem(params[k], seeds[k], geom)
-    with ThreadPoolExecutor(workers) as pool:
-        pending = [pool.submit(build, k) for k in range(min(batch, n))]
-        for i in range(0, n, batch):
-            problems = [f.result() for f in pending]
-            pending = [pool.submit(build, k) for k in range(i + batch, min(i + 2 * batch, n))]
-            solvers = [Solver(p) for p in problems]
-            del problems
-            try:
-                for s in solvers:
-                    s.set_schedule(3, *TILE)  # two steps per launch (bitwise equal to one)
-                with Batch(solvers) as b:
-                    b.set_schedule(*TILE)
-                    b.autotune(4)  # timed once per batch class in the process, then cached
-                    b.run(geom["nt"])
-                    gpu_ms += b.last_run_ms()[0]
-                for k, s in enumerate(solvers):
-                    out[i + k] = s.seismograms()
-            finally:
-                for s in solvers:
-                    s.close()
-            if progress:
-                progress(min(i + batch, n), n)
+    build = lambda k: model_problem(params[k], seeds[k], geom, np.float32)
+    chunks = [range(i, min(i + batch, n)) for i in range(0, n, batch)]
+
+    def run(b):
+        b.run(geom["nt"])
+        return b.last_run_ms()[0]
+
+    try:
+        with ThreadPoolExecutor(workers) as builders, ThreadPoolExecutor(1) as runner:
+            pending = [builders.submit(build, k) for k in chunks[0]] if chunks else []
+            running = None  # (future of the batch run, pool, chunk)
+            done = 0
+            for ci, chunk in enumerate(chunks):
+                problems = [f.result() for f in pending]
+                pool = pools.get(ci % 2, problems, builders)
+                del problems
+                pending = [builders.submit(build, k) for k in chunks[ci + 1]] if ci + 1 < len(chunks) else []
+                if not pool.has_batch(len(chunk)) and running is not None:
+                    # a new batch captures its step graph: the GPU work before it finishes first
+                    gpu_ms, done = _collect(running, out, gpu_ms, done, n, progress, builders)
+                    running = None
+                b = pool.batch(len(chunk))
+                fut = runner.submit(run, b)  # queued behind the batch before, if it still runs
+                if running is not None:
+                    gpu_ms, done = _collect(running, out, gpu_ms, done, n, progress, builders)
+                running = (fut, pool, chunk)
+            if running is not None:
+                gpu_ms, done = _collect(running, out, gpu_ms, done, n, progress, builders)
+    finally:
+        if own:
+            pools.close()
     return out, SolveTiming(n, time.perf_counter() - t0, gpu_ms * 1e-3)
+
+
+def _collect(running, out, gpu_ms, done, n, progress, executor):
+    """Wait for a batch run and read its gathers back (per handle, on the executor's threads)."""
+    fut, pool, chunk = running
+    gpu_ms += fut.result()
+
+    def read(k):
+        out[chunk[k]] = pool.solvers[k].seismograms()
+
+    list(executor.map(read, range(len(chunk))))
+    done += len(chunk)
+    if progress:
+        progress(done, n)
+    return gpu_ms, done
 
 
 def after_fallback(err, spread, threshold):
@@ -278,6 +398,7 @@ class Surrogate:
         self.nout = int(np.prod(self.shape))
         self.net_sizes = [4 + (2 * geom["nrec"] + 1 if self.receivers else 0), hidden, hidden, hidden, self.nout]
         self.net = Network(self.net_sizes, members=members, max_batch=max_batch, seed=seed)
+        self.pools = SimulatePools()  # solver handles for the re-solves, kept across screens
         self.scale = np.ones(self.shape[0], np.float32)    # RMS of each component of the gathers
 
     def _scaled(self, g):
@@ -341,7 +462,7 @@ class Surrogate:
         gathers = pred  # re-solved rows replaced in place; the judged rows are accepted ones
         if len(fallback):
             gathers[fallback], fb = simulate(params[fallback], None if seeds is None else seeds[fallback],
-                                             self.geom, progress=progress)
+                                             self.geom, progress=progress, pools=self.pools)
         t_screen = time.perf_counter() - t0
         accepted = np.setdiff1d(np.arange(len(params)), fallback)
         rng = np.random.default_rng(seed)
@@ -351,9 +472,10 @@ class Surrogate:
         jt = SolveTiming(0, 0.0, 0.0)
         if nj:
             true, jt = simulate(params[judged], None if seeds is None else seeds[judged], self.geom,
-                                progress=progress)
+                                progress=progress, pools=self.pools)
             err = self.errors(pred[judged], true)
         return ScreenResult(len(params), fallback, judged, err, t_screen, t_pred, fb, jt, gathers)
 
     def close(self):
+        self.pools.close()
         self.net.close()

@@ -192,3 +192,53 @@ def test_config2b_batch_at_shape():
         for x, y in zip(results[i], alone):
             assert np.array_equal(x, y)
     _parity_vs_oracle(probs[5], results[5][0], f"config2b model 5 ({sides[5][0]}x{sides[5][1]})")
+
+
+def test_set_coeffs_reuses_a_handle():
+    """A handle given another model's coefficients (poe2d_set_coeffs) equals a handle created for
+    that model, bitwise: palette to palette, palette to none and back, after a run and a
+    checkpoint; a refused set (Courant check, bad coefficients, a model shared with a clone) leaves
+    the handle as it was."""
+    import dataclasses
+    from poe2d_amd.configs import config1_continuous
+    a = make_problem(200, 180, seed=41, kind=_abi.SRC_PRESSURE_DGAUSS, nt=121)
+    b = make_problem(200, 180, seed=42, nt=121)
+    b = dataclasses.replace(b, source=a.source, receivers=a.receivers)
+    c = dataclasses.replace(a, coeffs=config1_continuous(n=200, nt=121).coeffs)  # no palette
+    c = dataclasses.replace(c, coeffs={k: v[:181, :] for k, v in c.coeffs.items()})
+    refs = {}
+    for name, p in (("a", a), ("b", b), ("c", c)):
+        with Solver(p) as s:
+            s.run()
+            refs[name] = state(s)
+    with Solver(a) as s:
+        s.run(50)
+        for name, p in (("b", b), ("c", c), ("a", a), ("c", c)):
+            s.set_coeffs(p.coeffs)
+            assert s.step_count() == 0 and not s.seismograms().any()
+            s.run()
+            got = state(s)
+            say(f"set_coeffs -> model {name}: bitwise equal {all(np.array_equal(x, y) for x, y in zip(got, refs[name]))}")
+            for x, y in zip(got, refs[name]):
+                assert np.array_equal(x, y), name
+        # refusals leave the handle as it was: a stiffer medium past the Courant bound, a bad mass
+        stiff = {k: v.copy() for k, v in a.coeffs.items()}
+        stiff["mu"] = stiff["mu"] * 4.0
+        stiff["lambdac"] = stiff["lambdac"] * 4.0
+        with pytest.raises(_abi.Poe2dError) as ei:
+            s.set_coeffs(stiff)
+        assert ei.value.status == _abi.POE2D_E_CFL
+        bad = {k: v.copy() for k, v in a.coeffs.items()}
+        bad["m"][50, 50] = 0.0
+        with pytest.raises(_abi.Poe2dError) as ei:
+            s.set_coeffs(bad)
+        assert ei.value.status == _abi.POE2D_E_INVALID
+        s.reset()
+        s.run()
+        for x, y in zip(state(s), refs["c"]):
+            assert np.array_equal(x, y)
+        clone = s.clone(source=a.source, receivers=a.receivers)
+        with pytest.raises(_abi.Poe2dError) as ei:
+            s.set_coeffs(a.coeffs)
+        assert ei.value.status == _abi.POE2D_E_STATE
+        clone.close()
