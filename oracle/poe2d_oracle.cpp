@@ -47,6 +47,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <type_traits>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
@@ -142,6 +143,7 @@ struct Base {
 
 template <class T>
 struct Solver : Base {
+    typedef T Real;
     int NX = 0, NZ = 0;
     size_t P = 0;
     T DX = 0, DZ = 0, DT = 0;
@@ -592,6 +594,23 @@ int poe2d_oracle_set_reference_scans(poe2d_oracle *o, int panj, int panj2, int a
         s->vsp_vz.assign(panj * nslot, 0); s->vsp_vx.assign(panj * nslot, 0);
         s->hor_vz.assign(panj2 * nslot, 0); s->hor_vx.assign(panj2 * nslot, 0);
         s->pt0.assign(nslot, 0); s->pt2.assign(nslot, 0); s->pt3.assign(nslot, 0); s->pt4.assign(nslot, 0);
+    });
+    return 0;
+}
+
+int poe2d_oracle_reference_scan(poe2d_oracle *o, int which, int index, double *out, int n)
+{
+    if (!o || !out || which < 0 || which > 7 || index < 0 || n < 0) return fail(POE2D_E_INVALID, "oracle: bad scan query");
+    DISPATCH(o, {
+        if (!s->scans) return fail(POE2D_E_STATE, "oracle: reference scans are off");
+        typedef typename std::remove_pointer<decltype(s)>::type::Real R;
+        const std::vector<R> *src[8];
+        src[0] = &s->vsp_vz; src[1] = &s->vsp_vx; src[2] = &s->hor_vz; src[3] = &s->hor_vx;
+        src[4] = &s->pt0; src[5] = &s->pt2; src[6] = &s->pt3; src[7] = &s->pt4;
+        const int width = which < 2 ? s->panj : which < 4 ? s->panj2 : 1;
+        if (index >= width || (size_t)n * width > src[which]->size())
+            return fail(POE2D_E_INVALID, "oracle: scan index or sample count out of range");
+        for (int k = 0; k < n; ++k) out[k] = (double)(*src[which])[(size_t)k * width + index];
     });
     return 0;
 }

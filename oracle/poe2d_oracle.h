@@ -26,6 +26,10 @@ int poe2d_oracle_set_threads(poe2d_oracle *o, int nthreads);
  * seismic_CPML_2d_poroelastic.cpp:657-739 run every step (panj VSP traces at x = is, panj2
  * horizontal traces at z = 10, four point traces at a0/a1/50/js). */
 int poe2d_oracle_set_reference_scans(poe2d_oracle *o, int panj, int panj2, int a0_pos, int a1_pos);
+/* Samples 0..n-1 recorded by those scans: which = 0 VSP v_z, 1 VSP v_x (index = line position
+ * 0..panj-1), 2 horizontal v_z, 3 horizontal v_x (0..panj2-1), 4..7 the point traces at rows 50,
+ * a0_pos, a1_pos, js (index 0). */
+int poe2d_oracle_reference_scan(poe2d_oracle *o, int which, int index, double *out, int n);
 int poe2d_oracle_run(poe2d_oracle *o, int nsteps);
 int poe2d_oracle_step_count(poe2d_oracle *o, int *nsteps);
 int poe2d_oracle_seismograms(poe2d_oracle *o, void *out, size_t out_bytes);

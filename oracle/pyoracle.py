@@ -32,6 +32,7 @@ def load():
         "set_receivers": ([vp, ctypes.POINTER(_abi.Receivers)], i),
         "set_threads": ([vp, i], i),
         "set_reference_scans": ([vp, i, i, i, i], i),
+        "reference_scan": ([vp, i, i, dp, i], i),
         "run": ([vp, i], i),
         "step_count": ([vp, _abi.c_int_p], i),
         "seismograms": ([vp, vp, sz], i),
@@ -69,6 +70,13 @@ class OracleSolver(HandleSolver):
     def set_reference_scans(self, panj, panj2, a0, a1):
         self._check(self._lib.poe2d_oracle_set_reference_scans(self._h, panj, panj2, a0, a1),
                     "set_reference_scans")
+
+    def reference_scan(self, which, index, n):
+        out = np.zeros(n)
+        self._check(self._lib.poe2d_oracle_reference_scan(self._h, which, index,
+                                                          out.ctypes.data_as(_abi.c_double_p), n),
+                    "reference_scan")
+        return out
 
     def cpml_table(self, axis, which):
         n = (self.problem.nx if axis == 0 else self.problem.nz) + 1

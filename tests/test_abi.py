@@ -95,6 +95,30 @@ def test_patchy_mask_fraction_and_seed():
     assert abs(m.sum() - np.pi * 64) < 2 * np.pi * 8
 
 
+def _lattice_restated(kind, rows=240, cols=5000, r=8):
+    """patchymodel.m's layer of type `kind`, restated from its loops: disk centres i = start:step:cols
+    and j = start:step:rows (1-based), every point within distance r of a centre set to 1 (the
+    MATLAB sums the disks and clamps values above 1)."""
+    dia = 2 * r
+    lattices = {1: [(r + 1, dia, r + 1, dia)],
+                2: [(r + 1, 2 * dia, r + 1, 2 * dia)],
+                3: [(r + 1, 4 * dia, r + 1, 2 * dia)],
+                4: [(r + 1, 2 * dia, r + 1, 2 * dia), (3 * r + 1, 2 * dia, 3 * r + 1, 2 * dia)],
+                5: [(r + 1, 2 * dia, 3 * r + 1, 2 * dia), (r + 1, dia, r + 1, 2 * dia)],
+                6: [(r + 1, 4 * dia, r + 1, 2 * dia), (5 * r + 1, 4 * dia, 3 * r + 1, 2 * dia)]}[kind]
+    out = np.zeros((rows, cols), np.int32)
+    dj, di = np.mgrid[-r:r + 1, -r:r + 1]
+    disk = (np.sqrt(di.astype(float) ** 2 + dj ** 2) <= r).astype(np.int32)
+    pad = np.zeros((rows + 2 * r, cols + 2 * r), np.int32)
+    for x0, sx, z0, sz in lattices:
+        for i in range(x0, cols + 1, sx):
+            for j in range(z0, rows + 1, sz):
+                # 1-based centre (i, j) -> padded 0-based window rows j-1 .. j-1+2r
+                pad[j - 1:j + 2 * r, i - 1:i + 2 * r] += disk
+    out = pad[r:r + rows, r:r + cols]
+    return (out > 0).astype(np.uint8)
+
+
 def test_patchymodel_lattice_counts():
     # type 1: radius-8 disks centred every 16 points; a lone disk has 197 points
     one = patchy_lattice(1, rows=17, cols=17, r=8)
@@ -106,6 +130,8 @@ def test_patchymodel_lattice_counts():
     for k in range(2, 7):
         lay = patchy_lattice(k)
         assert 0 < lay.sum() < t1.sum()
+        assert np.array_equal(lay, _lattice_restated(k)), k
+    assert np.array_equal(t1, _lattice_restated(1))
     med = patchymodel_medium(1)
     assert med.shape == (2800, 5000)
     assert set(np.unique(med)) == {1.0, 2.0, 3.0}

@@ -544,3 +544,43 @@ def test_float_oracle_tracks_double_oracle():
         r[prec] = s.seismograms()
     err = trace_rel_l2(r[F32], r[F64])
     assert err.max() < 1e-4, err.max()
+
+
+def test_reference_scans_record_what_the_gather_records():
+    """The reference-structured scans (three full-grid passes per step, CPP:657-739; the CPU
+    baseline runs them for their cost) record the same samples as the receiver gather at the
+    reference's receiver positions: the VSP line at column is, the horizontal line at row 10 and
+    the point traces at column (NX+1)/2 - 1."""
+    from oracle.pyoracle import OracleSolver, material_biot
+    from poe2d_amd.configs import biot_fields
+    from poe2d_amd.solver import Problem, ReceiverSpec, SourceSpec
+    nx, nz, nt, panj, panj2, a0, a1 = 130, 119, 60, 10, 13, 70, 90
+    src = SourceSpec(kind=_abi.SRC_PRESSURE_DGAUSS, ix=60, iz=40, amp=1e10)
+    len2, length = (nz + 1) // panj, (nx + 1) // panj2
+    xc = (nx + 1) // 2 - 1
+    vsp = [(src.ix, (m + 1) * len2 - 1) for m in range(panj)]
+    hor = [((m + 1) * length - 1, 10) for m in range(panj2)]
+    pts = [(xc, 50), (xc, a0), (xc, a1), (xc, src.iz)]
+    pos = vsp + hor + pts
+    rec = ReceiverSpec(ix=np.array([p[0] for p in pos], np.int32), iz=np.array([p[1] for p in pos], np.int32),
+                       comp_mask=_abi.VX | _abi.VY)
+    coeffs = material_biot(**biot_fields(nx, nz))
+    prob = Problem(nx=nx, nz=nz, dx=0.5, dz=0.5, dt=5e-5, nt=nt, coeffs=coeffs, source=src, receivers=rec,
+                   npml=10, f0=200.0)
+    o = OracleSolver(prob, _abi.F32, threads=2)
+    o.set_reference_scans(panj, panj2, a0, a1)
+    o.run(nt)
+    g = o.seismograms()  # [vx, vy][receiver][slot]
+    assert np.abs(g[1]).max() > 0
+    for m in range(panj):
+        assert np.array_equal(o.reference_scan(0, m, nt), g[1, m].astype(np.float64)), m
+        assert np.array_equal(o.reference_scan(1, m, nt), g[0, m].astype(np.float64)), m
+    for m in range(panj2):
+        assert np.array_equal(o.reference_scan(2, m, nt), g[1, panj + m].astype(np.float64)), m
+        assert np.array_equal(o.reference_scan(3, m, nt), g[0, panj + m].astype(np.float64)), m
+    for q in range(4):
+        assert np.array_equal(o.reference_scan(4 + q, 0, nt), g[1, panj + panj2 + q].astype(np.float64)), q
+    # the compared samples carry signal on every line
+    assert np.abs(o.reference_scan(0, 3, nt)).max() > 0 and np.abs(o.reference_scan(3, 4, nt)).max() > 0
+    assert np.abs(o.reference_scan(7, 0, nt)).max() > 0
+    o.close()
