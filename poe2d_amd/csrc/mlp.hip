@@ -319,13 +319,103 @@ poe2d_status backward(poe2d_mlp *m, Member &e, int n, float *loss)
     return POE2D_OK;
 }
 
-poe2d_status adam_all(poe2d_mlp *m, Member &e, float lr)
+// One training step of member e on the batch in h16[0] / tgt (n rows, a multiple of 32), loss
+// (mean) to *loss when non-null.  The output layer's forward product ends in the loss gradient
+// (EPI_MSE: no output array is written), and its weight-gradient product ends in the Adam step
+// (EPI_ADAM: the gradient never reaches memory); each layer's input gradient is formed from its
+// weights before they are updated.  Every value is computed as forward() and backward() compute
+// it, followed by the Adam step of each layer, except for the summation order of the output bias
+// gradient and of the loss.
+poe2d_status train_step(poe2d_mlp *m, Member &e, int n, float lr, float *loss)
 {
+    const int L = m->L;
+    for (int li = 0; li < L - 1; ++li) {
+        Layer &l = e.layers[li];
+        GemmOut o;
+        o.bias = l.b;
+        o.pre = m->pre[li];
+        o.ldp = l.outp;
+        o.b16 = m->h16[li + 1];
+        o.ldb = l.outp;
+        HIP_TRY(gemm_bf16(m->h16[li], l.inp, l.w16, l.inp, n, l.out, l.inp, EPI_GELU, o, m->work, m->work_floats, 1,
+                          m->stream));
+    }
+    Layer &top = e.layers[L - 1];
+    {
+        GemmOut o;
+        o.bias = top.b;
+        o.tgt = m->tgt;
+        o.ldt = top.outp;
+        o.scale = 2.0f / ((float)n * (float)top.out);
+        o.g = m->g16[L - 1];
+        o.ldg = top.outp;
+        o.gt = m->g16t[L - 1];
+        o.ldgt = m->max_batch;
+        o.partial = m->partial;
+        o.dbp = m->work;  // (the split-K partials come later)
+        o.lddb = top.outp;
+        HIP_TRY(gemm_bf16(m->h16[L - 1], top.inp, top.w16, top.inp, n, top.out, top.inp, EPI_MSE, o, m->work,
+                          m->work_floats, 1, m->stream));
+    }
+    HIP_TRY(sum_rows(m->work, (n + 127) / 128, top.outp, top.out, top.gb, m->stream));
+    if (loss) {
+        std::vector<float> part(gemm_tiles(n, top.out));
+        HIP_TRY(hipMemcpyAsync(part.data(), m->partial, part.size() * sizeof(float), hipMemcpyDeviceToHost, m->stream));
+        HIP_TRY(hipStreamSynchronize(m->stream));
+        double sum = 0;
+        for (float v : part) sum += v;
+        *loss = (float)(sum / ((double)n * top.out));
+    }
     e.t += 1;
-    for (Layer &l : e.layers) {
-        HIP_TRY(adam_matrix_step(l.w, l.mw, l.vw, l.gw, l.outp, l.inp, lr, 0.9f, 0.999f, 1e-8f, e.t, l.w16, l.w16t,
-                                 m->stream));
-        HIP_TRY(adam_step(l.b, l.mb, l.vb, l.gb, (size_t)l.outp, lr, 0.9f, 0.999f, 1e-8f, e.t, nullptr, m->stream));
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float c1 = 1.f - powf(b1, (float)e.t), c2 = 1.f - powf(b2, (float)e.t);
+    for (int li = L - 1; li >= 0; --li) {
+        Layer &l = e.layers[li];
+        if (li < L - 1) HIP_TRY(transpose_bf16(m->g16[li], l.outp, m->g16t[li], m->max_batch, n, l.outp, m->stream));
+        HIP_TRY(transpose_bf16(m->h16[li], l.inp, m->h16t[li], m->max_batch, n, l.inp, m->stream));
+        if (li > 0) {
+            // G_{l-1} = (G_l W_l) * gelu'(Z_{l-1}) with the weights of this step
+            const int K = l.outp;
+            int ksplit = 1;
+            const int tiles = ((n + 127) / 128) * ((l.in + 127) / 128);
+            while (tiles * ksplit < 512 && ksplit * 2 * 128 <= K / 4 && (size_t)ksplit * 2 * n * l.in <= m->work_floats)
+                ksplit *= 2;
+            GemmOut od;
+            od.f32 = m->dh;
+            od.ldf = l.inp;
+            HIP_TRY(gemm_bf16(m->g16[li], l.outp, l.w16t, l.outp, n, l.in, K, EPI_LINEAR, od, m->work, m->work_floats,
+                              ksplit, m->stream));
+            HIP_TRY(gelu_bwd(m->dh, m->pre[li - 1], n, l.in, l.inp, m->g16[li - 1], l.inp, m->stream));
+        }
+        if (li == L - 1) {
+            // dW = G^T H and the Adam step in one product: A = G^T [outp][n], B = H^T [inp][n]
+            GemmOut ow;
+            ow.w = l.w;
+            ow.mom = l.mw;
+            ow.vel = l.vw;
+            ow.ldw = l.inp;
+            ow.lr = lr;
+            ow.b1 = b1;
+            ow.b2 = b2;
+            ow.eps = eps;
+            ow.c1 = c1;
+            ow.c2 = c2;
+            ow.w16 = l.w16;
+            ow.w16t = l.w16t;
+            ow.ld16t = l.outp;
+            HIP_TRY(gemm_bf16(m->g16t[li], m->max_batch, m->h16t[li], m->max_batch, l.outp, l.inp, n, EPI_ADAM, ow,
+                              m->work, m->work_floats, 1, m->stream));
+        } else {
+            GemmOut o;
+            o.f32 = l.gw;
+            o.ldf = l.inp;
+            HIP_TRY(gemm_bf16(m->g16t[li], m->max_batch, m->h16t[li], m->max_batch, l.outp, l.inp, n, EPI_LINEAR, o,
+                              m->work, m->work_floats, 1, m->stream));
+            HIP_TRY(bias_gradient(m->g16t[li], m->max_batch, l.outp, n, l.gb, m->stream));
+            HIP_TRY(adam_matrix_step(l.w, l.mw, l.vw, l.gw, l.outp, l.inp, lr, b1, b2, eps, e.t, l.w16, l.w16t,
+                                     m->stream));
+        }
+        HIP_TRY(adam_step(l.b, l.mb, l.vb, l.gb, (size_t)l.outp, lr, b1, b2, eps, e.t, nullptr, m->stream));
     }
     return POE2D_OK;
 }
@@ -416,8 +506,9 @@ poe2d_status poe2d_mlp_create(int nlayers, const int *sizes, int members, int ma
         if (li > 0 && li < nlayers) hidden_max = std::max(hidden_max, w);
     }
     const int outp = pad128(sizes[nlayers]);
-    m->work_floats = (size_t)64 * B * hidden_max;
-    m->nred = std::max({m->nred, B, mse_blocks(sizes[nlayers])});
+    // split-K partials of the input-gradient products, and the output bias gradient's per-tile sums
+    m->work_floats = std::max((size_t)64 * B * hidden_max, (size_t)((B + 127) / 128) * pad128(sizes[nlayers]));
+    m->nred = std::max({m->nred, B, mse_blocks(sizes[nlayers]), gemm_tiles(B, sizes[nlayers])});
     if ((st = dmalloc(&m->dh, (size_t)B * std::max(hidden_max, pad128(sizes[0])))) ||
         (st = dmalloc(&m->y, (size_t)B * outp)) || (st = dmalloc(&m->tgt, (size_t)B * outp)) ||
         (st = dmalloc(&m->xin, (size_t)B * sizes[0])) || (st = dmalloc(&m->work, m->work_floats)) ||
@@ -540,9 +631,7 @@ poe2d_status poe2d_mlp_train(poe2d_mlp *m, int member, int nsteps, int batch, co
         gather_rows<<<blocks((size_t)batch * out), 256, 0, m->stream>>>(m->data_t, out, m->idx, batch, out, m->tgt, outp);
         pack_input<<<blocks((size_t)batch * in), 256, 0, m->stream>>>(m->xin, batch, in, m->h16[0], pad128(in));
         HIP_TRY(hipGetLastError());
-        if ((st = forward(m, e, batch)) || (st = backward(m, e, batch, losses ? losses + s : nullptr)) ||
-            (st = adam_all(m, e, lr)))
-            return st;
+        if ((st = train_step(m, e, batch, lr, losses ? losses + s : nullptr))) return st;
     }
     HIP_TRY(hipStreamSynchronize(m->stream));
     return POE2D_OK;
@@ -596,8 +685,7 @@ poe2d_status poe2d_mlp_time(poe2d_mlp *m, int batch, int iters, double *forward_
     HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
     *forward_ms = ms / iters;
     HIP_TRY(hipEventRecord(e0, m->stream));
-    for (int i = 0; i < iters && !st; ++i)
-        if (!(st = forward(m, e, batch)) && !(st = backward(m, e, batch, nullptr))) st = adam_all(m, e, 0.f + 1e-12f);
+    for (int i = 0; i < iters && !st; ++i) st = train_step(m, e, batch, 1e-12f, nullptr);
     HIP_TRY(hipEventRecord(e1, m->stream));
     HIP_TRY(hipEventSynchronize(e1));
     HIP_TRY(hipEventElapsedTime(&ms, e0, e1));

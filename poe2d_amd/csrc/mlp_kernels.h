@@ -29,7 +29,11 @@ __host__ __device__ inline float bf16_to_f32(uint16_t h)
     return f;
 }
 
-enum { EPI_LINEAR = 0, EPI_GELU = 1, EPI_PARTIAL = 2 };
+// EPI_MSE: instead of the output, the mean-squared-error gradient of output (product + bias)
+// against a target, bf16, row-major and transposed, with per-block sums of squared errors.
+// EPI_ADAM: the product is a weight gradient; the epilogue takes the Adam step of those master
+// weights and writes their bf16 copy and its transpose.
+enum { EPI_LINEAR = 0, EPI_GELU = 1, EPI_PARTIAL = 2, EPI_MSE = 3, EPI_ADAM = 4 };
 
 // Where a GEMM's results go: bias added first (when given); `pre` receives the value before the
 // activation; f32 / b16 the value after it.  Any pointer may be null.
@@ -41,11 +45,30 @@ struct GemmOut {
     float *pre = nullptr;
     int ldp = 0;
     const float *bias = nullptr;
+    // EPI_MSE: g[m][n] = bf16(scale (out - tgt)), gt[n][m] its transpose, partial[block] = the
+    // block's sum of (out - tgt)^2, dbp[tile row][n] = the sum of g over the tile's rows (lddb
+    // apart; sum_rows adds them up into the bias gradient)
+    const float *tgt = nullptr;
+    int ldt = 0;
+    float scale = 0.f;
+    uint16_t *g = nullptr, *gt = nullptr;
+    int ldg = 0, ldgt = 0;
+    float *partial = nullptr;
+    float *dbp = nullptr;
+    int lddb = 0;
+    // EPI_ADAM: master weights w (leading dimension ldw; the product's [m][n] is weight [m][n]), Adam
+    // moments, the step's constants, the bf16 copy w16 (ldw) and its transpose w16t (ld16t)
+    float *w = nullptr, *mom = nullptr, *vel = nullptr;
+    int ldw = 0;
+    float lr = 0.f, b1 = 0.f, b2 = 0.f, eps = 0.f, c1 = 1.f, c2 = 1.f;
+    uint16_t *w16 = nullptr, *w16t = nullptr;
+    int ld16t = 0;
 };
 
 // C = A B^T (M x N, reduction over K), A [M][lda] and B [N][ldb] bf16 with K contiguous; K and the
-// leading dimensions multiples of 32 / 8.  epi: EPI_LINEAR or EPI_GELU.  ksplit > 1 splits K over
-// blocks into `work` (ksplit * M * N floats) and reduces.
+// leading dimensions multiples of 32 / 8.  epi: EPI_LINEAR, EPI_GELU, EPI_MSE (partial holds one
+// entry per 128 x 128 tile) or EPI_ADAM.  ksplit > 1 splits K over blocks into `work` (ksplit * M *
+// N floats) and reduces (EPI_LINEAR and EPI_GELU only).
 hipError_t gemm_bf16(const uint16_t *A, int lda, const uint16_t *B, int ldb, int M, int N, int K, int epi,
                      const GemmOut &out, float *work, size_t work_floats, int ksplit, hipStream_t stream);
 
@@ -59,8 +82,13 @@ hipError_t gelu_bwd(const float *dh, const float *pre, int M, int N, int ld, uin
 // ([N][ldgt]), its column sums (the bias gradient) into db[N], and per-block sums of squares into
 // partial[mse_blocks(N)].
 int mse_blocks(int N);
+// Entries of `partial` the EPI_MSE product writes for an M x N output.
+inline int gemm_tiles(int M, int N) { return ((M + 127) / 128) * ((N + 127) / 128); }
 hipError_t mse(const float *y, int ldy, const float *t, int ldt, int M, int N, uint16_t *g, int ldg, uint16_t *gt,
                int ldgt, float *db, float *partial, hipStream_t stream);
+
+// db[n] = sum over r < rows of part[r * ld + n], n < N (the bias gradient from EPI_MSE's dbp).
+hipError_t sum_rows(const float *part, int rows, int ld, int N, float *db, hipStream_t stream);
 
 // db[r] = sum_k gt[r][k] over the K = batch entries of the transposed gradient gt [R][ldgt].
 hipError_t bias_gradient(const uint16_t *gt, int ldgt, int R, int K, float *db, hipStream_t stream);

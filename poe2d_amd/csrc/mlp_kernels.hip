@@ -36,14 +36,38 @@ __device__ __forceinline__ float gelu_grad(float x)
     return cdf + x * pdf;
 }
 
+// One Adam update of weight i (bias-corrected: c1 = 1 - b1^t, c2 = 1 - b2^t) from its gradient g.
+__device__ __forceinline__ float adam_value(float w, float &m, float &v, float g, float lr, float b1, float b2,
+                                            float eps, float c1, float c2)
+{
+    m = b1 * m + (1.f - b1) * g;
+    v = b2 * v + (1.f - b2) * g * g;
+    return w - lr * (m / c1) / (sqrtf(v / c2) + eps);
+}
+
+__device__ __forceinline__ float adam_update(float *w, float *mom, float *vel, float g, size_t i, float lr, float b1,
+                                             float b2, float eps, float c1, float c2)
+{
+    float m = mom[i], v = vel[i];
+    const float x = adam_value(w[i], m, v, g, lr, b1, b2, eps, c1, c2);
+    mom[i] = m;
+    vel[i] = v;
+    w[i] = x;
+    return x;
+}
+
 // C (+)= A B^T over k in [k0, k1) for the 128 x 128 tile (tm, tn); k0, k1 multiples of BK.
 template <int EPI>
 __global__ void __launch_bounds__(256) gemm_nt(const uint16_t *__restrict__ A, int lda, const uint16_t *__restrict__ B,
                                                int ldb, int M, int N, int K, int ksplit, GemmOut out)
 {
-    __shared__ uint16_t la[2][BM * LDS_ROW];
-    __shared__ uint16_t lb[2][BN * LDS_ROW];
+    // operand tiles (la, lb: two k tiles each), reused by the EPI_MSE / EPI_ADAM epilogues for the
+    // transposed bf16 tile
+    __shared__ uint16_t lds[2 * BM * LDS_ROW + 2 * BN * LDS_ROW];
+    uint16_t(*la)[BM * LDS_ROW] = (uint16_t(*)[BM * LDS_ROW])lds;
+    uint16_t(*lb)[BN * LDS_ROW] = (uint16_t(*)[BN * LDS_ROW])(lds + 2 * BM * LDS_ROW);
     __shared__ float lbias[BN];
+    __shared__ float red[256];
     const int tiles_n = (N + BN - 1) / BN;
     const int tile = blockIdx.x;
     const int tm = tile / tiles_n, tn = tile % tiles_n;
@@ -133,6 +157,103 @@ __global__ void __launch_bounds__(256) gemm_nt(const uint16_t *__restrict__ A, i
     }
     if (it < nk) compute(la[0], lb[0]);  // odd count: the last tile is in buffer 0
 
+    if constexpr (EPI == EPI_MSE || EPI == EPI_ADAM) {
+        // bf16 results of the tile go to memory row-major (g, or the updated weights' copy w16) and,
+        // through a transposed tile in LDS, column-major (gt, or w16t) in 256-byte rows
+        constexpr int TP = BM + 8;  // bf16 per transposed row
+        uint16_t *tt = lds;
+        float lsum = 0.f;
+        __syncthreads();  // every wave is done with the operand tiles
+        // Per 32 x 32 sub-tile: every operand load is issued before any store (the compiler cannot
+        // tell the arrays apart and would otherwise hold each load behind the store before it).
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int nl = wn * 64 + j * 32 + (lane & 31), n = tn * BN + nl;
+                auto row = [&](int e) { return wm * 64 + i * 32 + (e >> 2) * 8 + (lane >> 5) * 4 + (e & 3); };
+                float va[16], vb[16], vc[16];
+#pragma unroll
+                for (int e = 0; e < 16; ++e) {
+                    const int m = tm * BM + row(e);
+                    const bool ok = m < M && n < N;
+                    if constexpr (EPI == EPI_MSE) {
+                        va[e] = ok ? out.tgt[(size_t)m * out.ldt + n] : 0.f;
+                    } else {
+                        const size_t idx = (size_t)m * out.ldw + n;
+                        va[e] = ok ? out.w[idx] : 0.f;
+                        vb[e] = ok ? out.mom[idx] : 0.f;
+                        vc[e] = ok ? out.vel[idx] : 0.f;
+                    }
+                }
+#pragma unroll
+                for (int e = 0; e < 16; ++e) {
+                    const int ml = row(e), m = tm * BM + ml;
+                    uint16_t h = 0;
+                    if (m < M && n < N) {
+                        if constexpr (EPI == EPI_MSE) {
+                            // the output (product + bias) minus its target: loss and bf16 gradient
+                            const float d = (acc[i][j][e] + lbias[nl]) - va[e];
+                            lsum += d * d;
+                            h = bf16_rne(out.scale * d);
+                            out.g[(size_t)m * out.ldg + n] = h;
+                        } else {
+                            // the product is this weight's gradient: Adam on the master weight
+                            const size_t idx = (size_t)m * out.ldw + n;
+                            float mo = vb[e], ve = vc[e];
+                            const float x = adam_value(va[e], mo, ve, acc[i][j][e], out.lr, out.b1, out.b2, out.eps,
+                                                       out.c1, out.c2);
+                            out.mom[idx] = mo;
+                            out.vel[idx] = ve;
+                            out.w[idx] = x;
+                            h = bf16_rne(x);
+                            out.w16[idx] = h;
+                        }
+                    }
+                    tt[nl * TP + ml] = h;
+                }
+            }
+        __syncthreads();
+        {
+            const int nl = t >> 1, mh = (t & 1) * 64, n = tn * BN + nl;
+            uint16_t *dst = EPI == EPI_MSE ? out.gt : out.w16t;
+            const size_t ld = EPI == EPI_MSE ? (size_t)out.ldgt : (size_t)out.ld16t;
+            if (n < N)
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const int m0 = tm * BM + mh + q * 8;
+                    const uint16_t *src = tt + nl * TP + mh + q * 8;
+                    if (m0 + 8 <= M) {
+                        *(u16x8 *)(dst + (size_t)n * ld + m0) = *(const u16x8 *)src;
+                    } else {
+                        for (int r = 0; r < 8 && m0 + r < M; ++r) dst[(size_t)n * ld + m0 + r] = src[r];
+                    }
+                }
+        }
+        if constexpr (EPI == EPI_MSE) {
+            // the bias gradient's partial sum over this tile's rows, per column, from the bf16
+            // gradients in the transposed tile (fixed order: two halves of 64 rows, then the pair)
+            {
+                const int nl = t >> 1, mh = (t & 1) * 64;
+                float cs = 0.f;
+                for (int q = 0; q < 64; ++q) cs += bf16_to_f32(tt[nl * TP + mh + q]);
+                red[t] = cs;
+                __syncthreads();
+                const int n = tn * BN + nl;
+                if ((t & 1) == 0 && n < N) out.dbp[(size_t)tm * out.lddb + n] = red[t] + red[t + 1];
+                __syncthreads();
+            }
+            red[t] = lsum;
+            __syncthreads();
+            for (int h = 128; h > 0; h >>= 1) {
+                if (t < h) red[t] += red[t + h];
+                __syncthreads();
+            }
+            if (t == 0) out.partial[blockIdx.x] = red[0];
+        }
+        return;
+    }
+
     // epilogue: full tiles store without per-element bounds checks; the bias comes from LDS (staged
     // before the k loop), so no global load is pending here and the stores need no waits
     const bool full = (tm + 1) * BM <= M && (tn + 1) * BN <= N;
@@ -170,6 +291,16 @@ __global__ void __launch_bounds__(256) gemm_nt(const uint16_t *__restrict__ A, i
                 }
             }
     }
+}
+
+// db[n] = sum over r < rows of part[r][n] (n < N), in order.
+__global__ void column_sums(const float *__restrict__ part, int rows, int ld, int N, float *__restrict__ db)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    float v = 0.f;
+    for (int r = 0; r < rows; ++r) v += part[(size_t)r * ld + n];
+    db[n] = v;
 }
 
 // out[m][n] = sum over splits of part[s][m][n] (+ bias), then as the GEMM epilogue.
@@ -277,18 +408,6 @@ __global__ void __launch_bounds__(256) bias_grad(const uint16_t *__restrict__ gt
     if (lane == 0) db[r] = s;
 }
 
-__device__ __forceinline__ float adam_update(float *w, float *mom, float *vel, float g, size_t i, float lr, float b1,
-                                             float b2, float eps, float c1, float c2)
-{
-    const float m = b1 * mom[i] + (1.f - b1) * g;
-    const float v = b2 * vel[i] + (1.f - b2) * g * g;
-    mom[i] = m;
-    vel[i] = v;
-    const float x = w[i] - lr * (m / c1) / (sqrtf(v / c2) + eps);
-    w[i] = x;
-    return x;
-}
-
 // One Adam step on n master weights (FP32), refreshing their bf16 copy (when given).
 __global__ void adam(float *__restrict__ w, float *__restrict__ mom, float *__restrict__ vel, const float *__restrict__ grad,
                      size_t n, float lr, float b1, float b2, float eps, float c1, float c2, uint16_t *__restrict__ w16)
@@ -341,6 +460,19 @@ hipError_t gemm_bf16(const uint16_t *A, int lda, const uint16_t *B, int ldb, int
     const int tiles = ((M + BM - 1) / BM) * ((N + BN - 1) / BN);
     if (ksplit < 1) ksplit = 1;
     ksplit = std::min(ksplit, K / BK);
+    if (epi == EPI_MSE || epi == EPI_ADAM) {
+        // (no split: the epilogue needs the whole sum)
+        if (out.ldg % 8 && epi == EPI_MSE) return hipErrorInvalidValue;
+        const dim3 grid(tiles, 1);
+        if (epi == EPI_MSE) {
+            if (!out.tgt || !out.g || !out.gt || !out.partial || !out.dbp || out.ldgt % 8) return hipErrorInvalidValue;
+            gemm_nt<EPI_MSE><<<grid, 256, 0, stream>>>(A, lda, B, ldb, M, N, K, 1, out);
+        } else {
+            if (!out.w || !out.mom || !out.vel || !out.w16 || !out.w16t || out.ld16t % 8) return hipErrorInvalidValue;
+            gemm_nt<EPI_ADAM><<<grid, 256, 0, stream>>>(A, lda, B, ldb, M, N, K, 1, out);
+        }
+        return hipGetLastError();
+    }
     if (ksplit == 1) {
         const dim3 grid(tiles, 1);
         if (epi == EPI_GELU) gemm_nt<EPI_GELU><<<grid, 256, 0, stream>>>(A, lda, B, ldb, M, N, K, 1, out);
@@ -376,6 +508,12 @@ hipError_t mse(const float *y, int ldy, const float *t, int ldt, int M, int N, u
                int ldgt, float *db, float *partial, hipStream_t stream)
 {
     mse_grad<<<mse_blocks(N), 256, 0, stream>>>(y, ldy, t, ldt, M, N, g, ldg, gt, ldgt, db, partial);
+    return hipGetLastError();
+}
+
+hipError_t sum_rows(const float *part, int rows, int ld, int N, float *db, hipStream_t stream)
+{
+    column_sums<<<(N + 255) / 256, 256, 0, stream>>>(part, rows, ld, N, db);
     return hipGetLastError();
 }
 

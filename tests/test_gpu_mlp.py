@@ -119,11 +119,11 @@ def test_screening_network_shape_forward_and_gradients():
 
 
 def test_adam_matches_the_bias_corrected_update():
-    """Five Adam steps of one member against the bias-corrected update (beta 0.9 / 0.999, eps 1e-8)
-    restated in FP64 from the GPU's own gradients of each step's batch (the gradients themselves are
-    checked against the reference above); after every step the bf16 copy of each layer equals the
-    round-to-nearest-even of its master weights and the transposed copy equals its transpose, bit for
-    bit."""
+    """Five training steps of one member against the bias-corrected Adam update (beta 0.9 / 0.999,
+    eps 1e-8) restated in FP64 from the GPU's own gradients of each step's batch (the gradients
+    themselves are checked against the reference above), and the step's loss against theirs; after
+    every step the bf16 copy of each layer equals the round-to-nearest-even of its master weights
+    and the transposed copy equals its transpose, bit for bit."""
     rng = np.random.default_rng(21)
     sizes = [4, 96, 64, 160, 300]
     params = random_params(sizes, rng)
@@ -139,8 +139,11 @@ def test_adam_matches_the_bias_corrected_update():
         v = [(np.zeros_like(a), np.zeros_like(b)) for a, b in w]
         for step in range(1, 6):
             idx = rng.permutation(96)[:64].astype(np.int32)
-            _, grads = net.gradients(x[idx], t[idx], member=1)
-            net.train(idx, 64, lr, member=1)
+            loss_ref, grads = net.gradients(x[idx], t[idx], member=1)
+            # the training step forms the loss gradient in the output product's epilogue and the
+            # output layer's Adam step in its weight-gradient product's epilogue
+            loss = net.train(idx, 64, lr, member=1)[0]
+            assert abs(loss - loss_ref) <= 1e-5 * loss_ref, (loss, loss_ref)
             got = net.get_params(1)
             copies = net.get_bf16(1)
             c1, c2 = 1.0 - b1 ** step, 1.0 - b2 ** step
