@@ -86,6 +86,11 @@ def recorded_traffic(tile, n, schedule=0):
         return None, None
 
 
+def components(mask):
+    from poe2d_amd import _abi
+    return ", ".join(n for bit, n in ((_abi.VX, "vx"), (_abi.VY, "vy"), (_abi.P, "p")) if mask & bit)
+
+
 def trace_rel_l2(g, o, floor=1e-3):
     """Per-trace relative L2 distance ||g - o|| / max(||o||, floor x the largest trace norm of the
     component) over the last axis; g, o shaped [ncomp, ntrace, nsamp]."""
@@ -150,7 +155,7 @@ def cpu_leg(prob, make, steps, gpu_traces):
           f"rel L2 {err.max():.3e} ({reached} traces above 1e-3 of their component's largest)",
           file=sys.stderr, flush=True)
     return baseline, {"max": float(err.max()),
-                      "what": f"max over the 801 x 3 traces of the per-trace relative L2 distance between the GPU "
+                      "what": f"max over the {gpu_traces.shape[1]} x {gpu_traces.shape[0]} traces of the per-trace relative L2 distance between the GPU "
                               f"and the FP32 oracle (the reference's CPU loop restated) over the first {k} steps "
                               f"of the source run ({reached} traces carry signal by then); oracle on {threads} "
                               f"host threads"}
@@ -243,14 +248,23 @@ SCHEDULE_NAMES = {0: "one step per launch", 3: "two steps per launch"}
 
 
 def secondary_continuous(n, nt):
-    """Configuration 1's geometry over a medium whose porosity and frame moduli vary continuously
-    (every point its own coefficients, so no material palette: both kernels read the 32-byte
-    coefficient groups), each schedule autotuned and timed over one nt-step solve after a warm-up
-    solve."""
+    """Configuration 1's geometry over a medium whose porosity and frame moduli vary continuously."""
     from poe2d_amd.configs import config1_continuous
+    return secondary_no_palette(config1_continuous(nt=nt, n=n), nt, "continuous model")
+
+
+def secondary_reference_grid(nt):
+    """The reference program's own grid (13600 x 2440 points, 1.25 m, 10 C-PML points) over a
+    synthetic layered model whose c_p varies at every point."""
+    from poe2d_amd.configs import reference_grid
+    return secondary_no_palette(reference_grid(nt=nt), nt, "reference grid")
+
+
+def secondary_no_palette(prob, nt, label):
+    """A grid with no material palette (both kernels read the 32-byte coefficient groups), each
+    schedule autotuned and timed over one nt-step solve after a warm-up solve."""
     from poe2d_amd.solver import Solver
 
-    prob = config1_continuous(nt=nt, n=n)
     npts = prob.npoints
     cols, rows = strip_counts(prob)
     strips = cols * (prob.nz + 1) + rows * (prob.nx + 1)
@@ -272,7 +286,7 @@ def secondary_continuous(n, nt):
                                         "gcell_updates_per_s": round(npts / step_s / 1e9, 2),
                                         "compulsory_frac": round(compulsory_bytes_per_step(npts, strips, sched, False)
                                                                  / step_s / 1e9 / HBM_PEAK_GBS, 4)}
-            print(f"[bench] continuous model schedule {sched}: {t[0]}x{t[1]}, {step_s * 1e6:.1f} us per step",
+            print(f"[bench] {label} schedule {sched}: {t[0]}x{t[1]}, {step_s * 1e6:.1f} us per step",
                   file=sys.stderr, flush=True)
     out["two_step_speedup"] = round(out["schedule_0"]["us_per_step"] / out["schedule_3"]["us_per_step"], 3)
     return out
@@ -467,9 +481,10 @@ def main():
     ap.add_argument("--n", type=int, default=4000, help="NX = NZ (grid has n+1 points per side)")
     ap.add_argument("--nt", type=int, default=None, help="time steps per solve (6000; 3000 for the batches)")
     ap.add_argument("--workload", default="config1",
-                    choices=("config1", "continuous", "batchA", "batchB", "surrogate"),
+                    choices=("config1", "continuous", "reference", "batchA", "batchB", "surrogate"),
                     help="config1: the 4001^2 single shot (the headline line); continuous: the same geometry "
-                         "over a continuously varying medium (no material palette); batchA / batchB: "
+                         "over a continuously varying medium (no material palette); reference: the reference "
+                         "program's 13600x2440 grid over a synthetic layered model; batchA / batchB: "
                          "configuration 2's batches; surrogate: configurations 3 and 4")
     ap.add_argument("--sur-train", type=int, default=4096,
                     help="surrogate: training models (0: time the network alone)")
@@ -497,6 +512,8 @@ def main():
     ap.add_argument("--batch-nt", type=int, default=300, help="steps per run of the secondary batch timings")
     ap.add_argument("--continuous-nt", type=int, default=1000,
                     help="steps per run of the secondary continuous-model timing")
+    ap.add_argument("--reference-nt", type=int, default=1000,
+                    help="steps per run of the secondary timing on the reference program's grid")
     ap.add_argument("--tile", default="auto", metavar="BWxTH",
                     help="tile shape of the step kernel (block width x tile rows); 'auto' runs the "
                          "library's autotuner first")
@@ -520,20 +537,23 @@ def main():
     if args.workload == "surrogate":
         surrogate_bench(args)
         return
-    if args.workload not in ("config1", "continuous"):
+    if args.workload not in ("config1", "continuous", "reference"):
         batch_bench(args)
         return
 
     from poe2d_amd import F32, F64
-    from poe2d_amd.configs import config1, config1_continuous
+    from poe2d_amd.configs import config1, config1_continuous, reference_grid
     from poe2d_amd.solver import Solver
 
     if args.workload == "config1":
         def make(nt, lib=None):
             return config1(nt=nt, n=args.n, material_fn=None if lib is None else lib.material_biot)
-    else:
+    elif args.workload == "continuous":
         def make(nt, lib=None):
             return config1_continuous(nt=nt, n=args.n, material_fn=None if lib is None else lib.material_biot)
+    else:
+        def make(nt, lib=None):
+            return reference_grid(nt=nt, material_reference_fn=None if lib is None else lib.material_reference)
     prob = make(args.nt)
     npts = (prob.nx + 1) * (prob.nz + 1)
     cols, rows = strip_counts(prob)
@@ -603,12 +623,12 @@ def main():
     gcell = npts * args.nt / (ms_solve / 1e3) / 1e9
     step_ms = ms_solve / args.nt
     achieved = bytes_step / (step_ms / 1e3) / 1e9
-    traffic, traffic_src = recorded_traffic(tile, args.n, schedule)
+    # the committed counter readings are of configuration 1's patchy model
+    traffic, traffic_src = recorded_traffic(tile, args.n, schedule) if args.workload == "config1" else (None, None)
     compulsory = compulsory_bytes_per_step(npts, cols * (prob.nz + 1) + rows * (prob.nx + 1), schedule, palette > 0)
     out = {
         "metric": ("Gcell-updates/s (FP32 Biot C-PML time loop, 4001x4001, 6000 steps)" if args.workload == "config1"
-                   else f"Gcell-updates/s (FP32 Biot C-PML time loop, {prob.nx + 1}x{prob.nz + 1} continuous model, "
-                        f"{args.nt} steps)"),
+                   else f"Gcell-updates/s (FP32 Biot C-PML time loop, {prob.meta['workload']}, {args.nt} steps)"),
         "value": round(gcell * 1.0, 3),
         "unit": "Gcell-updates/s",
         "n_gpus": 1,
@@ -623,14 +643,15 @@ def main():
         "fp64_rel_l2_max": None if fp64_err is None else float(f"{fp64_err:.4g}"),
         "parity_rel_l2_max": None,
         "config": {"workload": prob.meta["workload"] + f", {args.nt} steps, "
-                               f"{prob.receivers.n} receivers x (vx, vy, p)",
+                               f"{prob.receivers.n} receivers x ({components(prob.receivers.comp_mask)})",
                    "grid": [prob.nx + 1, prob.nz + 1], "nt": args.nt, "npml": prob.npml,
                    "gas_fraction": prob.meta.get("gas_fraction_realised"),
                    "step": "one full solve", "wall_ms_median": round(statistics.median(wall_ms), 3),
                    "traces_finite": finite, "tile": f"{tile[0]}x{tile[1]} ({tile_note})",
                    "schedule": SCHEDULE_NAMES[schedule], "material_palette": palette,
-                   "fp64_rel_l2_max": "max over the 801 x 3 traces of the per-trace relative L2 distance between "
-                                      "this FP32 solve and the FP64 build's solve of the same model, all 6000 steps"},
+                   "fp64_rel_l2_max": f"max over the {seis.shape[1]} x {seis.shape[0]} traces of the per-trace "
+                                      "relative L2 distance between this FP32 solve and the FP64 build's solve of the "
+                                      f"same model, all {args.nt} steps"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None if traffic is None else round(traffic),
@@ -656,6 +677,7 @@ def main():
     }
     if args.secondary and args.workload == "config1":
         out["secondary"] = {"continuous_model": secondary_continuous(args.n, args.continuous_nt),
+                            "reference_grid": secondary_reference_grid(args.reference_nt),
                             "configuration_2": secondary_batches(args.batch_nt),
                             "surrogate_network": network_timing(True)[0]}
     if args.cpu_steps > 0:

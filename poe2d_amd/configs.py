@@ -146,3 +146,50 @@ def config1_continuous(nt=6000, n=4000, material_fn=None):
                    source=SourceSpec(kind=_abi.SRC_FORCE_Y_RICKER, ix=c, iz=c, amp=1.0e10),
                    receivers=line_receivers(x0, 4, nrec, c + 40), npml=20, f0=80.0,
                    meta={"config": "1c", "workload": f"biot-cpml {n + 1}x{n + 1} continuous porosity and frame moduli"})
+
+
+# Per-medium fluid tables for runs of the reference program's grid over synthetic models (K_f, eta,
+# kappa, phi, tortuosity, rho_f of media 0..3).  The reference's own tables (porosity 1e-6) put the
+# fluid-wave speed sqrt(M / m) of media 0 and 1 near 11 km/s, above that grid's Courant limit.
+SYNTHETIC_MEDIA = dict(kf=[2.4e9, 2.25e9, 0.05e9, 0.02e9], eta=[0.003, 0.001, 1.5e-5, 1.0e-5],
+                       kappa=[1.5e-13, 1.0e-12, 1.0e-12, 5.0e-13], phi=[0.2, 0.25, 0.25, 0.01],
+                       a=[2.0, 2.0, 2.0, 2.0], rhof=[1000.0, 1000.0, 100.0, 160.0])
+
+# The reference program's grid (seismic_CPML_2D_poroelastic_withCUDA.cu:486-510): NX = 13599,
+# NZ = 2439, dx = dz = 1.25 m, dt = 95 us, 10 C-PML points.
+REFERENCE_GRID = dict(nx=13599, nz=2439, dx=1.25, dt=95e-6, npml=10)
+
+
+def marmousi_like(nx=REFERENCE_GRID["nx"], nz=REFERENCE_GRID["nz"]):
+    """Synthetic float32 model grids in the form of the reference program's inputs, each (nz+1,
+    nx+1): medium index 0..3 in undulating layers, c_p rising from 1800 to 4500 m/s with depth plus
+    a lateral undulation of +-200 m/s, c_s = c_p / 1.9, saturated density 1900 + 0.12 c_p kg/m^3.
+    Every point has its own c_p, so the grid has no material palette."""
+    z = np.linspace(0.0, 1.0, nz + 1, dtype=np.float64)[:, None]
+    x = np.linspace(0.0, 1.0, nx + 1, dtype=np.float64)[None, :]
+    wave = np.sin(2.0 * np.pi * (4.0 * x + 2.0 * z))
+    depth = z + 0.03 * wave
+    medium = np.select([depth < 0.25, depth < 0.5, depth < 0.75], [1.0, 2.0, 0.0], 3.0)
+    cp = 1800.0 + 2700.0 * z + 200.0 * wave
+    cs = cp / 1.9
+    rho = 1900.0 + 0.12 * cp
+    return tuple(np.ascontiguousarray(a, dtype=np.float32) for a in (medium, cp, cs, rho))
+
+
+def reference_grid(nt=300, f0=30.0, isource=6800, jsource=40, a0_pos=50, a1_pos=70, material_reference_fn=None):
+    """The reference program's run on its own grid over marmousi_like's model with SYNTHETIC_MEDIA:
+    the pressure source of the reference (derivative of a Gaussian, gated at 0.4 s) at (isource,
+    jsource), its 122 VSP, 680 horizontal and 3 point receivers recording v_x and v_z.
+    material_reference_fn(cp, cs, rho_sat, medium, tables) replaces the library's front end."""
+    from . import refio
+    from .material import material_reference
+    g = REFERENCE_GRID
+    medium, cp, cs, rho = marmousi_like(g["nx"], g["nz"])
+    coeffs = (material_reference_fn or material_reference)(cp, cs, rho, medium, SYNTHETIC_MEDIA)
+    ix, iz, nv, nh, npt = refio.reference_receivers(g["nx"], g["nz"], isource, jsource, a0_pos, a1_pos)
+    return Problem(nx=g["nx"], nz=g["nz"], dx=g["dx"], dz=g["dx"], dt=g["dt"], nt=nt, coeffs=coeffs,
+                   source=SourceSpec(kind=_abi.SRC_PRESSURE_DGAUSS, ix=isource, iz=jsource, amp=1e10, t_cut=0.4),
+                   receivers=ReceiverSpec(ix=ix, iz=iz, comp_mask=_abi.VX | _abi.VY), npml=g["npml"], f0=f0,
+                   cp_max=float(cp.max()), check_cfl=0,
+                   meta={"config": "reference grid", "workload": f"biot-cpml {g['nx'] + 1}x{g['nz'] + 1} "
+                         "reference grid, synthetic layered model (no palette)"})

@@ -73,3 +73,4 @@ def trace_rel_l2(g, o, floor=1e-3):
     den = np.linalg.norm(o, axis=-1)
     fl = floor * den.max(axis=-1, keepdims=True)
     return num / np.maximum(np.maximum(den, fl), 1e-300)
+

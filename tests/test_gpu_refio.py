@@ -4,6 +4,7 @@ snapshot files must match the oracle run of the same problem (material from the 
 restatement of the reference front end, receivers from a brute-force restatement of the
 reference's gather conditions).  Bound: per-trace relative L2 <= 1e-4 (the ASCII files carry 6
 significant digits, ~1e-6 relative)."""
+import os
 import subprocess
 
 import numpy as np
@@ -12,6 +13,7 @@ import pytest
 from oracle import pyoracle
 from oracle.pyoracle import OracleSolver
 from poe2d_amd import _abi, refio
+from poe2d_amd.configs import SYNTHETIC_MEDIA, marmousi_like
 from poe2d_amd.solver import Problem, ReceiverSpec, SourceSpec
 
 from .helpers import random_patchy_mask, trace_rel_l2
@@ -19,9 +21,7 @@ from .test_refio import brute_force_layout
 
 pytestmark = pytest.mark.gpu
 
-MEDIA = dict(kf=[2.4e9, 2.25e9, 0.05e9, 0.02e9], eta=[0.003, 0.001, 1.5e-5, 1.0e-5],
-             kappa=[1.5e-13, 1.0e-12, 1.0e-12, 5.0e-13], phi=[0.2, 0.25, 0.25, 0.01],
-             a=[2.0, 2.0, 2.0, 2.0], rhof=[1000.0, 1000.0, 100.0, 160.0])
+MEDIA = SYNTHETIC_MEDIA
 
 
 def say(msg):
@@ -103,3 +103,62 @@ def test_reference_program_matches_oracle(tmp_path, law, material):
     rel = np.linalg.norm(snap - vz_last) / np.linalg.norm(vz_last)
     say(f"vz00400.vzo vs oracle: rel L2 {rel:.2e}")
     assert rel <= 1e-4
+
+
+def test_reference_grid_shape(tmp_path):
+    """poe2d-run on the reference program's own grid (its defaults: 13600 x 2440 points, 1.25 m,
+    95 us, 10 C-PML points, 122 VSP and 680 horizontal receivers) over synthetic float32 model
+    files in its format (x fastest, mar_*_2440x13600.bin).  The model varies at every point, so the
+    run takes the two-step kernel on the coefficient groups.  The media come from a table file:
+    the reference's default tables (porosity 1e-6) give media 0 and 1 a fluid-wave speed of about
+    11 km/s, above this grid's Courant limit of 7.98 km/s (test_refio pins this), and the oracle
+    diverges on them as well.  300 steps: the default schedule (two steps per launch, autotuned tile) equals
+    the single-step kernel bitwise in every output file, and matches the FP32 oracle within 1e-4
+    per trace and in the final snapshots."""
+    nx, nz, nstep, is_, js, a0, a1, f0 = 13599, 2439, 300, 6800, 40, 50, 70, 30
+    medium, cp, cs, rho = marmousi_like(nx, nz)
+    for name, a in (("poro", medium), ("cp", cp), ("cs", cs), ("rho", rho)):
+        a.astype("<f4").tofile(tmp_path / f"mar_{name}_2440x13600.bin")  # x fastest
+    with open(tmp_path / "media.txt", "w") as f:
+        for k in range(4):
+            f.write(" ".join(repr(float(MEDIA[c][k])) for c in ("kf", "eta", "kappa", "phi", "a", "rhof")) + "\n")
+    (tmp_path / "input1.txt").write_text(f"{nstep}\n{is_} {js}\n{a0} {a1}\n")
+    (tmp_path / "input2.txt").write_text(f"{f0}\n")
+    runs = {"default": [], "single-step": ["--schedule", "0", "--tile", "256x8"]}
+    for tag, extra in runs.items():
+        (tmp_path / tag).mkdir()
+        say(f"poe2d-run 13600 x 2440, NSTEP={nstep}, {tag} schedule")
+        r = subprocess.run([refio.RUNNER, "--media", "media.txt", "--outdir", tag, *extra], cwd=tmp_path,
+                           capture_output=True, text=True, timeout=300)
+        say(r.stdout.strip().replace("\n", " | "))
+        assert r.returncode == 0, r.stderr
+    names = sorted(p.name for p in (tmp_path / "default").iterdir())
+    assert names == sorted(p.name for p in (tmp_path / "single-step").iterdir())
+    assert f"vz{nstep:05d}.vzo" in names and f"vx{nstep:05d}.vxo" in names
+    for name in names:
+        assert (tmp_path / "default" / name).read_bytes() == (tmp_path / "single-step" / name).read_bytes(), name
+    say(f"{len(names)} output files bitwise equal between the schedules")
+
+    say("oracle FP32, same problem, all host threads")
+    ix, iz, nv, nh, npt = refio.reference_receivers(nx, nz, is_, js, a0, a1)
+    coeffs = pyoracle.material_reference(cp, cs, rho, medium, MEDIA)
+    prob = Problem(nx=nx, nz=nz, dx=1.25, dz=1.25, dt=95e-6, nt=nstep, coeffs=coeffs,
+                   source=SourceSpec(kind=_abi.SRC_PRESSURE_DGAUSS, ix=is_, iz=js, amp=1e10, t_cut=0.4),
+                   receivers=ReceiverSpec(ix=ix, iz=iz, comp_mask=_abi.VX | _abi.VY), npml=10, f0=float(f0),
+                   cp_max=float(cp.max()), check_cfl=0)
+    o = OracleSolver(prob, precision=_abi.F32, threads=min(os.cpu_count() or 1, 16))
+    o.run()
+    seis = o.seismograms()
+    out = tmp_path / "default"
+    files = {k: refio.read_traces(out / v) for k, v in refio.output_names(f0).items()}
+    got = np.concatenate([files["vsp_vz"][:, :nv].T, files["hor_vz"][:, :nh].T,
+                          np.stack([files["ats"], files["bwh"], files["src"]])])
+    err = trace_rel_l2(got[None], seis[1][None])
+    say(f"v_z traces: max rel L2 vs oracle {err.max():.2e} over {got.shape[0]} traces")
+    assert err.max() <= 1e-4
+    for fname, field in ((f"vz{nstep:05d}.vzo", "vy"), (f"vx{nstep:05d}.vxo", "vx")):
+        snap = refio.read_grid(out / fname, nx + 1, nz + 1)
+        want = o.snapshot(field)
+        rel = np.linalg.norm(snap - want) / np.linalg.norm(want)
+        say(f"{fname} vs oracle: rel L2 {rel:.2e}")
+        assert rel <= 1e-4

@@ -139,3 +139,20 @@ def test_program_courant_exit(tmp_path):
     assert r.returncode == 0, r.stderr
     assert "time step is too large, simulation will be unstable" in r.stdout
     assert not list(tmp_path.glob("*.vsp")) and not list(tmp_path.glob("*.vzo"))
+
+
+def test_reference_default_media_exceed_the_reference_grid_courant_limit():
+    """The reference's default fluid tables (porosity 1e-6) give media 0 and 1 a fluid modulus
+    M = K_f / phi of 2.4e15 Pa against m = a rho_f / phi = 2e7 kg/m^3: a fluid-wave speed
+    sqrt(M / m) of about 11 km/s, above the Courant limit 6/7/sqrt(2) dx/dt = 7.98 km/s of the
+    reference grid (1.25 m, 95 us), which the reference's check against the input c_p does not
+    see.  Runs of its default grid over synthetic models therefore take a media table."""
+    from poe2d_amd.material import REFERENCE_TABLES, material_reference
+    n = 4
+    one = lambda v: np.full(n, v, dtype=np.float32)
+    c = material_reference(one(3000.0), one(1600.0), one(2260.0), np.arange(n, dtype=np.float32),
+                           REFERENCE_TABLES)
+    speed = np.sqrt(c["M"].astype(np.float64) / c["m"])
+    limit = 6.0 / 7.0 / np.sqrt(2.0) * 1.25 / 95e-6
+    assert 7974.0 < limit < 7975.0
+    assert (speed[:2] > 1.35 * limit).all() and (speed[2:] < limit).all()
