@@ -273,6 +273,17 @@ poe2d_status forward(poe2d_mlp *m, Member &e, int n)
     return POE2D_OK;
 }
 
+// k splits of an input-gradient product (n x in over k = K): enough blocks for about four per CU
+// (1024 with the splits), each split at least 4 k tiles of 32, within the partials buffer.
+int input_grad_splits(const poe2d_mlp *m, int n, int in, int K)
+{
+    int ksplit = 1;
+    const int tiles = ((n + 127) / 128) * ((in + 127) / 128);
+    while (tiles * ksplit < 1024 && ksplit * 2 * 128 <= K / 4 && (size_t)ksplit * 2 * n * in <= m->work_floats)
+        ksplit *= 2;
+    return ksplit;
+}
+
 // Gradients of the mean squared error for member e on the batch in h16[0] / tgt (n rows, n a
 // multiple of 32); loss (mean) to *loss when non-null.
 poe2d_status backward(poe2d_mlp *m, Member &e, int n, float *loss)
@@ -304,11 +315,7 @@ poe2d_status backward(poe2d_mlp *m, Member &e, int n, float *loss)
         if (li == 0) break;
         // G_{l-1} = (G_l W_l) * gelu'(Z_{l-1}): A = G_l [n][outp], B = W_l^T [inp][outp]
         const int K = l.outp;
-        int ksplit = 1;
-        const int tiles = ((n + 127) / 128) * ((l.in + 127) / 128);
-        while (tiles * ksplit < 512 && ksplit * 2 * 128 <= K / 4 &&
-               (size_t)ksplit * 2 * n * l.in <= m->work_floats)
-            ksplit *= 2;
+        const int ksplit = input_grad_splits(m, n, l.in, K);
         GemmOut od;
         od.f32 = m->dh;
         od.ldf = l.inp;
@@ -376,10 +383,7 @@ poe2d_status train_step(poe2d_mlp *m, Member &e, int n, float lr, float *loss)
         if (li > 0) {
             // G_{l-1} = (G_l W_l) * gelu'(Z_{l-1}) with the weights of this step
             const int K = l.outp;
-            int ksplit = 1;
-            const int tiles = ((n + 127) / 128) * ((l.in + 127) / 128);
-            while (tiles * ksplit < 512 && ksplit * 2 * 128 <= K / 4 && (size_t)ksplit * 2 * n * l.in <= m->work_floats)
-                ksplit *= 2;
+            const int ksplit = input_grad_splits(m, n, l.in, K);
             GemmOut od;
             od.f32 = m->dh;
             od.ldf = l.inp;
@@ -406,6 +410,8 @@ poe2d_status train_step(poe2d_mlp *m, Member &e, int n, float lr, float *loss)
             HIP_TRY(gemm_bf16(m->g16t[li], m->max_batch, m->h16t[li], m->max_batch, l.outp, l.inp, n, EPI_ADAM, ow,
                               m->work, m->work_floats, 1, m->stream));
         } else {
+            // (the hidden layers' 512 x 512 products make 16 tiles: too few blocks to stream the
+            // optimiser state from the product's epilogue; a separate Adam pass over 64 tiles is faster)
             GemmOut o;
             o.f32 = l.gw;
             o.ldf = l.inp;
@@ -507,7 +513,7 @@ poe2d_status poe2d_mlp_create(int nlayers, const int *sizes, int members, int ma
     }
     const int outp = pad128(sizes[nlayers]);
     // split-K partials of the input-gradient products, and the output bias gradient's per-tile sums
-    m->work_floats = std::max((size_t)64 * B * hidden_max, (size_t)((B + 127) / 128) * pad128(sizes[nlayers]));
+    m->work_floats = std::max((size_t)128 * B * hidden_max, (size_t)((B + 127) / 128) * pad128(sizes[nlayers]));
     m->nred = std::max({m->nred, B, mse_blocks(sizes[nlayers]), gemm_tiles(B, sizes[nlayers])});
     if ((st = dmalloc(&m->dh, (size_t)B * std::max(hidden_max, pad128(sizes[0])))) ||
         (st = dmalloc(&m->y, (size_t)B * outp)) || (st = dmalloc(&m->tgt, (size_t)B * outp)) ||

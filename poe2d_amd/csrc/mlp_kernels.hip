@@ -5,7 +5,9 @@
 // GEMM layout ("NT"): C[m][n] = sum_k A[m][k] B[n][k], both operands K-contiguous bf16 rows.  A
 // block of 256 threads (4 waves, 2 x 2) owns a 128 x 128 tile of C; each wave computes 64 x 64
 // with 2 x 2 v_mfma_f32_32x32x16_bf16.  Operand tiles of 128 rows x 32 k go through LDS, double
-// buffered, one barrier per k tile; loads run two k tiles ahead of the MFMAs.  MFMA lane layout (gfx950, 32x32x16 bf16): lane l supplies
+// buffered, one barrier per k tile; loads run two to four k tiles ahead of the MFMAs, through
+// register sets sized so that every variant runs three waves per SIMD without spilling.  MFMA lane
+// layout (gfx950, 32x32x16 bf16): lane l supplies
 // row l % 32 of A and column l % 32 of B with 8 k values (the same 8 for both; here k = (l / 32) * 8
 // + j of the 16-wide k step), and element i of its 16 results is C[(i / 4) * 8 + (l / 32) * 4 + i % 4]
 // [l % 32].
@@ -58,25 +60,48 @@ __device__ __forceinline__ float adam_update(float *w, float *mom, float *vel, f
 
 // C (+)= A B^T over k in [k0, k1) for the 128 x 128 tile (tm, tn); k0, k1 multiples of BK.
 template <int EPI>
-__global__ void __launch_bounds__(256) gemm_nt(const uint16_t *__restrict__ A, int lda, const uint16_t *__restrict__ B,
+__global__ void __launch_bounds__(256, 3) gemm_nt(const uint16_t *__restrict__ A, int lda, const uint16_t *__restrict__ B,
                                                int ldb, int M, int N, int K, int ksplit, GemmOut out)
 {
     // operand tiles (la, lb: two k tiles each), reused by the EPI_MSE / EPI_ADAM epilogues for the
-    // transposed bf16 tile
+    // transposed bf16 tile and the EPI_MSE sums (40 KiB)
     __shared__ uint16_t lds[2 * BM * LDS_ROW + 2 * BN * LDS_ROW];
     uint16_t(*la)[BM * LDS_ROW] = (uint16_t(*)[BM * LDS_ROW])lds;
     uint16_t(*lb)[BN * LDS_ROW] = (uint16_t(*)[BN * LDS_ROW])(lds + 2 * BM * LDS_ROW);
-    __shared__ float lbias[BN];
-    __shared__ float red[256];
+    // Tile order.  Row-major outputs: block b takes tile b, n fastest, so the blocks in flight write
+    // long runs of the same rows.  EPI_MSE and split-K: blocks go to the 8 XCDs round-robin (block
+    // b, counted x fastest, to XCD b % 8), and XCD x takes the x-th run of consecutive items
+    // instead.  EPI_MSE walks m fastest, so the m tiles of one n tile follow each other on one XCD:
+    // they share its operand tile in that XCD's L2 and finish the rows of the transposed gradient
+    // together (225 -> 200 us on the 256 x 151552 x 512 output layer; the same order slowed the
+    // plain epilogue from 89 to 104 us).  Split-K walks the tiles of one k range first, so every
+    // tile of a k range runs on one XCD and each operand slice is read from HBM once.
     const int tiles_n = (N + BN - 1) / BN;
-    const int tile = blockIdx.x;
-    const int tm = tile / tiles_n, tn = tile % tiles_n;
-    const int split = blockIdx.y;
+    int tm, tn, split = 0;
+    if constexpr (EPI == EPI_MSE || EPI == EPI_PARTIAL) {
+        const int nb = (int)(gridDim.x * gridDim.y), q8 = nb >> 3, r8 = nb & 7;
+        const int b = (int)(blockIdx.y * gridDim.x + blockIdx.x);
+        const int xcd = b & 7, slot = b >> 3;
+        const int item = xcd < r8 ? xcd * (q8 + 1) + slot : r8 * (q8 + 1) + (xcd - r8) * q8 + slot;
+        const int tile = item % (int)gridDim.x;
+        if (EPI == EPI_PARTIAL) split = item / (int)gridDim.x;
+        const int tiles_m = (M + BM - 1) / BM;
+        tm = EPI == EPI_MSE ? tile % tiles_m : tile / tiles_n;
+        tn = EPI == EPI_MSE ? tile / tiles_m : tile % tiles_n;
+    } else {
+        tm = (int)blockIdx.x / tiles_n;
+        tn = (int)blockIdx.x % tiles_n;
+    }
     const int kper = ((K / BK + ksplit - 1) / ksplit) * BK;
     const int k0 = split * kper, k1 = min(K, k0 + kper);
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wm = wave >> 1, wn = wave & 1;
-    if (t < BN) lbias[t] = (EPI != EPI_PARTIAL && out.bias && tn * BN + t < N) ? out.bias[tn * BN + t] : 0.f;
+    // the bias of this lane's two result columns (wn * 64 + j * 32 + lane % 32)
+    float bj[2];
+    for (int j = 0; j < 2; ++j) {
+        const int n = tn * BN + wn * 64 + j * 32 + (lane & 31);
+        bj[j] = (EPI != EPI_PARTIAL && EPI != EPI_ADAM && out.bias && n < N) ? out.bias[n] : 0.f;
+    }
 
     // each thread loads 2 x 16 bytes of each operand tile: rows r0 and r0 + 64, k chunk kc * 8
     const int r0 = t >> 2, kc = t & 3;
@@ -104,9 +129,12 @@ __global__ void __launch_bounds__(256) gemm_nt(const uint16_t *__restrict__ A, i
         for (int j = 0; j < 2; ++j)
             for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.f;
 
-    // Two register sets carry operand tiles from memory: the tile loaded during k step i is stored
-    // to LDS at the end of step i + 1, so each load has two steps of MFMA work to land.
-    u16x8 va0[2], vb0[2], va1[2], vb1[2];
+    // PF register sets carry operand tiles from memory: tile i lives in set i % PF, loaded at the
+    // start of k step i - PF and stored to LDS at the end of step i - 1, so each load has about PF - 1
+    // steps of MFMA work to land.  As many sets as fit beside each epilogue at three waves per SIMD
+    // (168 registers) without spilling.
+    constexpr int PF = (EPI == EPI_ADAM || EPI == EPI_MSE) ? 2 : EPI == EPI_PARTIAL ? 4 : 3;
+    u16x8 ra[PF][2], rb[PF][2];
     const int nk = k1 > k0 ? (k1 - k0) / BK : 0;
     auto compute = [&](const uint16_t *LA, const uint16_t *LB) {
 #pragma unroll
@@ -127,35 +155,34 @@ __global__ void __launch_bounds__(256) gemm_nt(const uint16_t *__restrict__ A, i
         }
     };
     const int last = k0 + (nk - 1) * BK;  // tiles past the end re-read the last one (never used)
+    auto kof = [&](int i) { return min(k0 + i * BK, last); };
     if (nk > 0) {
-        load_tile(A, lda, M, tm * BM, k0, va0);
-        load_tile(B, ldb, N, tn * BN, k0, vb0);
-        load_tile(A, lda, M, tm * BM, min(k0 + BK, last), va1);
-        load_tile(B, ldb, N, tn * BN, min(k0 + BK, last), vb1);
-        store_tile(la[0], va0, a_ok);
-        store_tile(lb[0], vb0, b_ok);
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+            load_tile(A, lda, M, tm * BM, kof(u), ra[u]);
+            load_tile(B, ldb, N, tn * BN, kof(u), rb[u]);
+        }
+        store_tile(la[0], ra[0], a_ok);
+        store_tile(lb[0], rb[0], b_ok);
     }
     __syncthreads();
-    int it = 0;
-    for (; it + 2 <= nk; it += 2) {
-        // LDS buffer 0 holds tile it; set 1 carries tile it + 1
-        load_tile(A, lda, M, tm * BM, min(k0 + (it + 2) * BK, last), va0);
-        load_tile(B, ldb, N, tn * BN, min(k0 + (it + 2) * BK, last), vb0);
-        compute(la[0], lb[0]);
-        store_tile(la[1], va1, a_ok);
-        store_tile(lb[1], vb1, b_ok);
-        __syncthreads();
-        // LDS buffer 1 holds tile it + 1; set 0 carries tile it + 2
-        load_tile(A, lda, M, tm * BM, min(k0 + (it + 3) * BK, last), va1);
-        load_tile(B, ldb, N, tn * BN, min(k0 + (it + 3) * BK, last), vb1);
-        compute(la[1], lb[1]);
-        if (it + 2 < nk) {
-            store_tile(la[0], va0, a_ok);
-            store_tile(lb[0], vb0, b_ok);
+    for (int it0 = 0; it0 < nk; it0 += PF) {
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+            const int it = it0 + u;
+            if (it < nk) {
+                // LDS buffer it & 1 holds tile it, so set u is free: refill it with tile it + PF
+                load_tile(A, lda, M, tm * BM, kof(it + PF), ra[u]);
+                load_tile(B, ldb, N, tn * BN, kof(it + PF), rb[u]);
+                compute(la[it & 1], lb[it & 1]);
+                if (it + 1 < nk) {
+                    store_tile(la[(it + 1) & 1], ra[(u + 1) % PF], a_ok);
+                    store_tile(lb[(it + 1) & 1], rb[(u + 1) % PF], b_ok);
+                }
+                __syncthreads();
+            }
         }
-        __syncthreads();
     }
-    if (it < nk) compute(la[0], lb[0]);  // odd count: the last tile is in buffer 0
 
     if constexpr (EPI == EPI_MSE || EPI == EPI_ADAM) {
         // bf16 results of the tile go to memory row-major (g, or the updated weights' copy w16) and,
@@ -180,10 +207,11 @@ __global__ void __launch_bounds__(256) gemm_nt(const uint16_t *__restrict__ A, i
                     if constexpr (EPI == EPI_MSE) {
                         va[e] = ok ? out.tgt[(size_t)m * out.ldt + n] : 0.f;
                     } else {
+                        // the optimiser state streams through once per step: non-temporal
                         const size_t idx = (size_t)m * out.ldw + n;
-                        va[e] = ok ? out.w[idx] : 0.f;
-                        vb[e] = ok ? out.mom[idx] : 0.f;
-                        vc[e] = ok ? out.vel[idx] : 0.f;
+                        va[e] = ok ? __builtin_nontemporal_load(out.w + idx) : 0.f;
+                        vb[e] = ok ? __builtin_nontemporal_load(out.mom + idx) : 0.f;
+                        vc[e] = ok ? __builtin_nontemporal_load(out.vel + idx) : 0.f;
                     }
                 }
 #pragma unroll
@@ -193,21 +221,19 @@ __global__ void __launch_bounds__(256) gemm_nt(const uint16_t *__restrict__ A, i
                     if (m < M && n < N) {
                         if constexpr (EPI == EPI_MSE) {
                             // the output (product + bias) minus its target: loss and bf16 gradient
-                            const float d = (acc[i][j][e] + lbias[nl]) - va[e];
+                            const float d = (acc[i][j][e] + bj[j]) - va[e];
                             lsum += d * d;
                             h = bf16_rne(out.scale * d);
-                            out.g[(size_t)m * out.ldg + n] = h;
                         } else {
                             // the product is this weight's gradient: Adam on the master weight
                             const size_t idx = (size_t)m * out.ldw + n;
                             float mo = vb[e], ve = vc[e];
                             const float x = adam_value(va[e], mo, ve, acc[i][j][e], out.lr, out.b1, out.b2, out.eps,
                                                        out.c1, out.c2);
-                            out.mom[idx] = mo;
-                            out.vel[idx] = ve;
-                            out.w[idx] = x;
+                            __builtin_nontemporal_store(mo, out.mom + idx);
+                            __builtin_nontemporal_store(ve, out.vel + idx);
+                            __builtin_nontemporal_store(x, out.w + idx);
                             h = bf16_rne(x);
-                            out.w16[idx] = h;
                         }
                     }
                     tt[nl * TP + ml] = h;
@@ -230,39 +256,54 @@ __global__ void __launch_bounds__(256) gemm_nt(const uint16_t *__restrict__ A, i
                     }
                 }
         }
+        {
+            // row-major from the same tile (g, or w16): thread t writes row t / 2, columns
+            // (t & 1) * 64 .. + 63, in 16-byte pieces
+            const int ml = t >> 1, nh = (t & 1) * 64, m = tm * BM + ml;
+            uint16_t *dst = EPI == EPI_MSE ? out.g : out.w16;
+            const size_t ld = EPI == EPI_MSE ? (size_t)out.ldg : (size_t)out.ldw;
+            if (m < M)
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const int nl0 = nh + q * 8, n0 = tn * BN + nl0;
+                    u16x8 v;
+#pragma unroll
+                    for (int r = 0; r < 8; ++r) v[r] = tt[(nl0 + r) * TP + ml];
+                    if (n0 + 8 <= N) {
+                        *(u16x8 *)(dst + (size_t)m * ld + n0) = v;
+                    } else {
+                        for (int r = 0; r < 8 && n0 + r < N; ++r) dst[(size_t)m * ld + n0 + r] = v[r];
+                    }
+                }
+        }
         if constexpr (EPI == EPI_MSE) {
             // the bias gradient's partial sum over this tile's rows, per column, from the bf16
-            // gradients in the transposed tile (fixed order: two halves of 64 rows, then the pair)
-            {
-                const int nl = t >> 1, mh = (t & 1) * 64;
-                float cs = 0.f;
-                for (int q = 0; q < 64; ++q) cs += bf16_to_f32(tt[nl * TP + mh + q]);
-                red[t] = cs;
-                __syncthreads();
-                const int n = tn * BN + nl;
-                if ((t & 1) == 0 && n < N) out.dbp[(size_t)tm * out.lddb + n] = red[t] + red[t + 1];
-                __syncthreads();
-            }
-            red[t] = lsum;
+            // gradients in the transposed tile (fixed order: two halves of 64 rows, then the pair),
+            // and the block's sum of squared errors (per wave by shuffles, then the four waves)
+            const int nl = t >> 1, mh = (t & 1) * 64;
+            float cs = 0.f;
+            for (int q = 0; q < 64; ++q) cs += bf16_to_f32(tt[nl * TP + mh + q]);
+            for (int o = 32; o > 0; o >>= 1) lsum += __shfl_xor(lsum, o);
+            float *red = (float *)(lds + BN * TP);  // past the transposed tile
+            red[t] = cs;
+            if (lane == 0) red[256 + wave] = lsum;
             __syncthreads();
-            for (int h = 128; h > 0; h >>= 1) {
-                if (t < h) red[t] += red[t + h];
-                __syncthreads();
-            }
-            if (t == 0) out.partial[blockIdx.x] = red[0];
+            const int n = tn * BN + nl;
+            if ((t & 1) == 0 && n < N) out.dbp[(size_t)tm * out.lddb + n] = red[t] + red[t + 1];
+            if (t == 0) out.partial[blockIdx.x] = (red[256] + red[257]) + (red[258] + red[259]);
         }
         return;
     }
 
-    // epilogue: full tiles store without per-element bounds checks; the bias comes from LDS (staged
-    // before the k loop), so no global load is pending here and the stores need no waits
+    // epilogue: full tiles store without per-element bounds checks; the bias sits in registers
+    // (loaded before the k loop), so no global load is pending here and the stores need no waits
     const bool full = (tm + 1) * BM <= M && (tn + 1) * BN <= N;
-    auto emit = [&](int m, int n, float v) {
+    auto emit = [&](int m, int n, float v, float b) {
         if (EPI == EPI_PARTIAL) {
             out.f32[(size_t)split * M * out.ldf + (size_t)m * out.ldf + n] = v;
             return;
         }
-        v += lbias[n - tn * BN];
+        v += b;
         if (out.pre) out.pre[(size_t)m * out.ldp + n] = v;
         if (EPI == EPI_GELU) v = gelu(v);
         if (out.f32) out.f32[(size_t)m * out.ldf + n] = v;
@@ -276,7 +317,7 @@ __global__ void __launch_bounds__(256) gemm_nt(const uint16_t *__restrict__ A, i
                 const int n = tn * BN + wn * 64 + j * 32 + (lane & 31);
 #pragma unroll
                 for (int e = 0; e < 16; ++e)
-                    emit(tm * BM + wm * 64 + i * 32 + (e >> 2) * 8 + (lane >> 5) * 4 + (e & 3), n, acc[i][j][e]);
+                    emit(tm * BM + wm * 64 + i * 32 + (e >> 2) * 8 + (lane >> 5) * 4 + (e & 3), n, acc[i][j][e], bj[j]);
             }
     } else {
 #pragma unroll
@@ -287,7 +328,7 @@ __global__ void __launch_bounds__(256) gemm_nt(const uint16_t *__restrict__ A, i
 #pragma unroll
                 for (int e = 0; e < 16; ++e) {
                     const int m = tm * BM + wm * 64 + i * 32 + (e >> 2) * 8 + (lane >> 5) * 4 + (e & 3);
-                    if (m < M && n < N) emit(m, n, acc[i][j][e]);
+                    if (m < M && n < N) emit(m, n, acc[i][j][e], bj[j]);
                 }
             }
     }
@@ -309,8 +350,19 @@ __global__ void reduce_splits(const float *__restrict__ part, int nsplit, int M,
     const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (i >= (size_t)M * N) return;
     const int m = (int)(i / N), n = (int)(i % N);
+    // eight partials in flight at a time, summed in split order
+    const float *p = part + (size_t)m * ldf + n;
+    const size_t stride = (size_t)M * ldf;
     float v = 0.f;
-    for (int s = 0; s < nsplit; ++s) v += part[(size_t)s * M * ldf + (size_t)m * ldf + n];
+    int s = 0;
+    for (; s + 8 <= nsplit; s += 8) {
+        float q[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) q[u] = p[(size_t)(s + u) * stride];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v += q[u];
+    }
+    for (; s < nsplit; ++s) v += p[(size_t)s * stride];
     if (out.bias) v += out.bias[n];
     if (out.pre) out.pre[(size_t)m * out.ldp + n] = v;
     if (gelu_on) v = gelu(v);
@@ -465,10 +517,14 @@ hipError_t gemm_bf16(const uint16_t *A, int lda, const uint16_t *B, int ldb, int
         if (out.ldg % 8 && epi == EPI_MSE) return hipErrorInvalidValue;
         const dim3 grid(tiles, 1);
         if (epi == EPI_MSE) {
-            if (!out.tgt || !out.g || !out.gt || !out.partial || !out.dbp || out.ldgt % 8) return hipErrorInvalidValue;
+            if (!out.tgt || !out.g || !out.gt || !out.partial || !out.dbp || out.ldgt % 8 || (uintptr_t)out.g % 16 ||
+                (uintptr_t)out.gt % 16)
+                return hipErrorInvalidValue;
             gemm_nt<EPI_MSE><<<grid, 256, 0, stream>>>(A, lda, B, ldb, M, N, K, 1, out);
         } else {
-            if (!out.w || !out.mom || !out.vel || !out.w16 || !out.w16t || out.ld16t % 8) return hipErrorInvalidValue;
+            if (!out.w || !out.mom || !out.vel || !out.w16 || !out.w16t || out.ld16t % 8 || out.ldw % 8 ||
+                (uintptr_t)out.w16 % 16 || (uintptr_t)out.w16t % 16)
+                return hipErrorInvalidValue;
             gemm_nt<EPI_ADAM><<<grid, 256, 0, stream>>>(A, lda, B, ldb, M, N, K, 1, out);
         }
         return hipGetLastError();
