@@ -109,8 +109,14 @@ struct poe2d_mlp {
     // resident dataset
     float *data_x = nullptr, *data_t = nullptr;
     int data_n = 0;
-    int *idx = nullptr;
+    int *idx = nullptr;      // batch row indices of up to kIndexSteps training steps
+    float *dloss = nullptr;  // their losses
 };
+
+// Training steps whose batch indices go to the device in one copy: a copy from pageable host
+// memory holds the host until the stream reaches it, so one per step would leave the device idle
+// while the host issues each step's launches.
+constexpr int kIndexSteps = 1024;
 
 namespace {
 
@@ -134,7 +140,7 @@ void free_mlp(poe2d_mlp *m)
     for (auto &p : m->g16) dfree(p);
     for (auto &p : m->g16t) dfree(p);
     dfree(m->dh); dfree(m->y); dfree(m->tgt); dfree(m->xin); dfree(m->work); dfree(m->partial);
-    dfree(m->acc_mean); dfree(m->acc_m2); dfree(m->data_x); dfree(m->data_t); dfree(m->idx);
+    dfree(m->acc_mean); dfree(m->acc_m2); dfree(m->data_x); dfree(m->data_t); dfree(m->idx); dfree(m->dloss);
     if (m->stream) (void)hipStreamDestroy(m->stream);
     m->stream = nullptr;
 }
@@ -209,6 +215,22 @@ __global__ void row_spread(const float *__restrict__ mean, const float *__restri
 }
 
 int blocks(size_t n) { return (int)std::min<size_t>((n + 255) / 256, 1u << 20); }
+
+// *loss = scale x the sum of count partials (FP64 accumulation: thread t sums partials t, t + 256,
+// ... in order, then a fixed tree), one block of 256 threads.
+__global__ void loss_sum(const float *__restrict__ partial, int count, double scale, float *__restrict__ loss)
+{
+    __shared__ double red[256];
+    double v = 0;
+    for (int i = threadIdx.x; i < count; i += 256) v += partial[i];
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *loss = (float)(red[0] * scale);
+}
 
 // rows x K FP32 values into rows x Kp bf16 values (round to nearest even), zeros in columns K..Kp-1.
 __global__ void pack_bf16_rows(const float *__restrict__ src, int rows, int K, int Kp, uint16_t *__restrict__ dst)
@@ -326,14 +348,16 @@ poe2d_status backward(poe2d_mlp *m, Member &e, int n, float *loss)
     return POE2D_OK;
 }
 
-// One training step of member e on the batch in h16[0] / tgt (n rows, a multiple of 32), loss
-// (mean) to *loss when non-null.  The output layer's forward product ends in the loss gradient
+// One training step of member e on the batch in h16[0] (n rows, a multiple of 32) against targets
+// tgt (row r of the batch: row tgt_rows[r], or r when tgt_rows is null; ldt apart), the mean loss
+// to device memory *loss_dev when non-null.  The output layer's forward product ends in the loss gradient
 // (EPI_MSE: no output array is written), and its weight-gradient product ends in the Adam step
 // (EPI_ADAM: the gradient never reaches memory); each layer's input gradient is formed from its
 // weights before they are updated.  Every value is computed as forward() and backward() compute
 // it, followed by the Adam step of each layer, except for the summation order of the output bias
 // gradient and of the loss.
-poe2d_status train_step(poe2d_mlp *m, Member &e, int n, float lr, float *loss)
+poe2d_status train_step(poe2d_mlp *m, Member &e, int n, const float *tgt, int ldt, const int *tgt_rows, float lr,
+                        float *loss_dev)
 {
     const int L = m->L;
     for (int li = 0; li < L - 1; ++li) {
@@ -351,8 +375,9 @@ poe2d_status train_step(poe2d_mlp *m, Member &e, int n, float lr, float *loss)
     {
         GemmOut o;
         o.bias = top.b;
-        o.tgt = m->tgt;
-        o.ldt = top.outp;
+        o.tgt = tgt;
+        o.ldt = ldt;
+        o.tgt_rows = tgt_rows;
         o.scale = 2.0f / ((float)n * (float)top.out);
         o.g = m->g16[L - 1];
         o.ldg = top.outp;
@@ -365,13 +390,9 @@ poe2d_status train_step(poe2d_mlp *m, Member &e, int n, float lr, float *loss)
                           m->work_floats, 1, m->stream));
     }
     HIP_TRY(sum_rows(m->work, (n + 127) / 128, top.outp, top.out, top.gb, m->stream));
-    if (loss) {
-        std::vector<float> part(gemm_tiles(n, top.out));
-        HIP_TRY(hipMemcpyAsync(part.data(), m->partial, part.size() * sizeof(float), hipMemcpyDeviceToHost, m->stream));
-        HIP_TRY(hipStreamSynchronize(m->stream));
-        double sum = 0;
-        for (float v : part) sum += v;
-        *loss = (float)(sum / ((double)n * top.out));
+    if (loss_dev) {
+        loss_sum<<<1, 256, 0, m->stream>>>(m->partial, gemm_tiles(n, top.out), 1.0 / ((double)n * top.out), loss_dev);
+        HIP_TRY(hipGetLastError());
     }
     e.t += 1;
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
@@ -519,7 +540,8 @@ poe2d_status poe2d_mlp_create(int nlayers, const int *sizes, int members, int ma
         (st = dmalloc(&m->y, (size_t)B * outp)) || (st = dmalloc(&m->tgt, (size_t)B * outp)) ||
         (st = dmalloc(&m->xin, (size_t)B * sizes[0])) || (st = dmalloc(&m->work, m->work_floats)) ||
         (st = dmalloc(&m->partial, m->nred)) || (st = dmalloc(&m->acc_mean, (size_t)B * sizes[nlayers])) ||
-        (st = dmalloc(&m->acc_m2, (size_t)B * sizes[nlayers])) || (st = dmalloc(&m->idx, B)))
+        (st = dmalloc(&m->acc_m2, (size_t)B * sizes[nlayers])) || (st = dmalloc(&m->idx, (size_t)kIndexSteps * B)) ||
+        (st = dmalloc(&m->dloss, kIndexSteps)))
         return bail(st);
     if (hipStreamSynchronize(m->stream) != hipSuccess) return bail(fail(POE2D_E_HIP, "mlp_create: initialisation failed"));
     *out = m;
@@ -630,14 +652,22 @@ poe2d_status poe2d_mlp_train(poe2d_mlp *m, int member, int nsteps, int batch, co
     for (long long i = 0; i < (long long)nsteps * batch; ++i)
         if (indices[i] < 0 || indices[i] >= m->data_n) return fail(POE2D_E_INVALID, "mlp_train: index outside the dataset");
     Member &e = m->members[member];
-    const int in = m->sizes[0], out = m->sizes[m->L], outp = pad128(out);
+    const int in = m->sizes[0], out = m->sizes[m->L];
     for (int s = 0; s < nsteps; ++s) {
-        HIP_TRY(hipMemcpyAsync(m->idx, indices + (size_t)s * batch, batch * sizeof(int), hipMemcpyHostToDevice, m->stream));
-        gather_rows<<<blocks((size_t)batch * in), 256, 0, m->stream>>>(m->data_x, in, m->idx, batch, in, m->xin, in);
-        gather_rows<<<blocks((size_t)batch * out), 256, 0, m->stream>>>(m->data_t, out, m->idx, batch, out, m->tgt, outp);
+        const int c = s % kIndexSteps;
+        if (c == 0)
+            HIP_TRY(hipMemcpyAsync(m->idx, indices + (size_t)s * batch,
+                                   (size_t)std::min(kIndexSteps, nsteps - s) * batch * sizeof(int),
+                                   hipMemcpyHostToDevice, m->stream));
+        const int *idx = m->idx + (size_t)c * batch;
+        gather_rows<<<blocks((size_t)batch * in), 256, 0, m->stream>>>(m->data_x, in, idx, batch, in, m->xin, in);
         pack_input<<<blocks((size_t)batch * in), 256, 0, m->stream>>>(m->xin, batch, in, m->h16[0], pad128(in));
         HIP_TRY(hipGetLastError());
-        if ((st = train_step(m, e, batch, lr, losses ? losses + s : nullptr))) return st;
+        // the targets are read in place from the dataset through the batch's row indices
+        if ((st = train_step(m, e, batch, m->data_t, out, idx, lr, losses ? m->dloss + c : nullptr))) return st;
+        if (losses && (c == kIndexSteps - 1 || s == nsteps - 1))
+            HIP_TRY(hipMemcpyAsync(losses + (s - c), m->dloss, (size_t)(c + 1) * sizeof(float), hipMemcpyDeviceToHost,
+                                   m->stream));
     }
     HIP_TRY(hipStreamSynchronize(m->stream));
     return POE2D_OK;
@@ -691,7 +721,7 @@ poe2d_status poe2d_mlp_time(poe2d_mlp *m, int batch, int iters, double *forward_
     HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
     *forward_ms = ms / iters;
     HIP_TRY(hipEventRecord(e0, m->stream));
-    for (int i = 0; i < iters && !st; ++i) st = train_step(m, e, batch, 1e-12f, nullptr);
+    for (int i = 0; i < iters && !st; ++i) st = train_step(m, e, batch, m->tgt, pad128(m->sizes[m->L]), nullptr, 1e-12f, nullptr);
     HIP_TRY(hipEventRecord(e1, m->stream));
     HIP_TRY(hipEventSynchronize(e1));
     HIP_TRY(hipEventElapsedTime(&ms, e0, e1));

@@ -47,9 +47,11 @@ struct GemmOut {
     const float *bias = nullptr;
     // EPI_MSE: g[m][n] = bf16(scale (out - tgt)), gt[n][m] its transpose, partial[block] = the
     // block's sum of (out - tgt)^2, dbp[tile row][n] = the sum of g over the tile's rows (lddb
-    // apart; sum_rows adds them up into the bias gradient)
+    // apart; sum_rows adds them up into the bias gradient).  Target row m is tgt row m, or row
+    // tgt_rows[m] when tgt_rows is given (a batch read in place from a dataset).
     const float *tgt = nullptr;
     int ldt = 0;
+    const int *tgt_rows = nullptr;
     float scale = 0.f;
     uint16_t *g = nullptr, *gt = nullptr;
     int ldg = 0, ldgt = 0;

@@ -200,13 +200,33 @@ __global__ void __launch_bounds__(256, 3) gemm_nt(const uint16_t *__restrict__ A
                 const int nl = wn * 64 + j * 32 + (lane & 31), n = tn * BN + nl;
                 auto row = [&](int e) { return wm * 64 + i * 32 + (e >> 2) * 8 + (lane >> 5) * 4 + (e & 3); };
                 float va[16], vb[16], vc[16];
+                if constexpr (EPI == EPI_MSE) {
+                    // target rows: the batch's own, or looked up four at a time (the lane's rows come in
+                    // runs of four, all below M or none: M % 4 == 0 with tgt_rows), lookups first
+                    int tr[16];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int mg = tm * BM + row(4 * q);
+                        if (out.tgt_rows) {
+                            const int4 r4 = mg < M ? *(const int4 *)(out.tgt_rows + mg) : make_int4(0, 0, 0, 0);
+                            tr[4 * q] = r4.x;
+                            tr[4 * q + 1] = r4.y;
+                            tr[4 * q + 2] = r4.z;
+                            tr[4 * q + 3] = r4.w;
+                        } else {
+#pragma unroll
+                            for (int u = 0; u < 4; ++u) tr[4 * q + u] = mg + u;
+                        }
+                    }
+#pragma unroll
+                    for (int e = 0; e < 16; ++e)
+                        va[e] = (tm * BM + row(e) < M && n < N) ? out.tgt[(size_t)tr[e] * out.ldt + n] : 0.f;
+                }
 #pragma unroll
                 for (int e = 0; e < 16; ++e) {
                     const int m = tm * BM + row(e);
                     const bool ok = m < M && n < N;
-                    if constexpr (EPI == EPI_MSE) {
-                        va[e] = ok ? out.tgt[(size_t)m * out.ldt + n] : 0.f;
-                    } else {
+                    if constexpr (EPI == EPI_ADAM) {
                         // the optimiser state streams through once per step: non-temporal
                         const size_t idx = (size_t)m * out.ldw + n;
                         va[e] = ok ? __builtin_nontemporal_load(out.w + idx) : 0.f;
@@ -518,7 +538,7 @@ hipError_t gemm_bf16(const uint16_t *A, int lda, const uint16_t *B, int ldb, int
         const dim3 grid(tiles, 1);
         if (epi == EPI_MSE) {
             if (!out.tgt || !out.g || !out.gt || !out.partial || !out.dbp || out.ldgt % 8 || (uintptr_t)out.g % 16 ||
-                (uintptr_t)out.gt % 16)
+                (uintptr_t)out.gt % 16 || (out.tgt_rows && (M % 4 || (uintptr_t)out.tgt_rows % 16)))
                 return hipErrorInvalidValue;
             gemm_nt<EPI_MSE><<<grid, 256, 0, stream>>>(A, lda, B, ldb, M, N, K, 1, out);
         } else {
