@@ -423,20 +423,24 @@ def surrogate_bench(args):
     dump = {"val_params": params[ntr:], "val_err": verr, "val_spread": vspread}
     cand = S.draw_params(ncand, seed=4)
     screens = []
-    for q in quantiles:
-        thr = S.spread_threshold(verr, vspread, 0.05, q)
+    specs = [(q, None) for q in quantiles]
+    if args.sur_confidence > 0:
+        specs.append((quantiles[-1], args.sur_confidence))
+    for q, conf in specs:
+        thr = S.spread_threshold(verr, vspread, 0.05, q, confidence=conf)
         res = sur.screen(cand, thr, progress=lambda d, n: d % 1024 == 0 and print(
             f"[surrogate] re-solved {d} of {n}", file=sys.stderr, flush=True))
         je = res.judge_errors
         ffb = len(res.fallback) / res.n
-        tag = f"p{round(q * 100)}"
+        tag = f"p{round(q * 100)}" + ("" if conf is None else f"_ucb{round(conf * 100)}")
         dump.update({f"{tag}_threshold": thr, f"{tag}_fallback": res.fallback, f"{tag}_judged": res.judged,
                      f"{tag}_judge_err": je})
         value = res.models_per_second
-        print(f"[surrogate] bound p{round(q * 100)} <= 5 %: {value:.1f} models/s, fallback {ffb:.4f}",
-              file=sys.stderr, flush=True)
+        how = ("estimated by" if conf is None else
+               f"its {round(conf * 100)} % upper confidence bound (an order statistic) over")
+        print(f"[surrogate] bound {tag} <= 5 %: {value:.1f} models/s, fallback {ffb:.4f}", file=sys.stderr, flush=True)
         screens.append({
-            "bound": f"p{round(q * 100)} of the error after fallback <= 5 % (threshold from {nval} held-out models)",
+            "bound": f"p{round(q * 100)} of the error after fallback <= 5 %, {how} {nval} held-out models",
             "models_per_s": round(value, 2),
             "speedup_vs_solver": round(value / solve.models_per_second, 2),
             "speedup_vs_solver_gpu_time": round(value / solve.gpu_models_per_second, 2),
@@ -486,12 +490,12 @@ def main():
                          "over a continuously varying medium (no material palette); reference: the reference "
                          "program's 13600x2440 grid over a synthetic layered model; batchA / batchB: "
                          "configuration 2's batches; surrogate: configurations 3 and 4")
-    ap.add_argument("--sur-train", type=int, default=4096,
+    ap.add_argument("--sur-train", type=int, default=32768,
                     help="surrogate: training models (0: time the network alone)")
-    ap.add_argument("--sur-val", type=int, default=512, help="surrogate: held-out models for the threshold")
+    ap.add_argument("--sur-val", type=int, default=2048, help="surrogate: held-out models for the threshold")
     ap.add_argument("--sur-cand", type=int, default=16384,
                     help="surrogate: candidates screened (configuration 4: 16 384)")
-    ap.add_argument("--sur-steps", type=int, default=30000, help="surrogate: training steps per member")
+    ap.add_argument("--sur-steps", type=int, default=60000, help="surrogate: training steps per member")
     ap.add_argument("--sur-receivers", type=int, default=1,
                     help="surrogate: 1 = per-receiver gas inputs besides the four parameters, 0 = the "
                          "four parameters alone")
@@ -501,6 +505,9 @@ def main():
     ap.add_argument("--sur-quantiles", default="0.5,0.99",
                     help="surrogate: error quantiles after fallback held within 5 %% by the threshold, one "
                          "screening pass each (the first gives the line's value)")
+    ap.add_argument("--sur-confidence", type=float, default=0.95,
+                    help="surrogate: one more screen for the last quantile, its threshold set so that the "
+                         "quantile's upper confidence bound at this level stays within 5 %% (0: none)")
     ap.add_argument("--cpu-steps", type=int, default=12,
                     help="time steps of the CPU baseline sample (0: skip the CPU leg, the oracle comparison "
                          "included)")

@@ -27,6 +27,7 @@ decimated by 6 to 500 samples, 101 receivers at x = 50 + 3k, 40 points below the
 500 = 151 500 outputs).  SMALL is a reduced geometry with the same layout for quick tests.
 """
 import ctypes
+import math
 import time
 from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass
@@ -355,15 +356,32 @@ def mixture_quantile(judge_errors, fallback_fraction, q):
     return float(np.quantile(judge_errors, (q - fallback_fraction) / (1.0 - fallback_fraction)))
 
 
-def spread_threshold(err, spread, bound=0.05, quantile=0.99):
+def quantile_upper_rank(n, q, confidence):
+    """Rank k (1-based) of the order statistic X_(k) of n samples that bounds the q-quantile of their
+    distribution from above with probability >= confidence: the smallest k with P(Binomial(n, q) <=
+    k - 1) >= confidence (the number of samples below the quantile is Binomial(n, q))."""
+    cdf = 0.0
+    logq, log1mq = math.log(q), math.log1p(-q)
+    for j in range(n + 1):
+        cdf += math.exp(math.lgamma(n + 1) - math.lgamma(j + 1) - math.lgamma(n - j + 1) + j * logq + (n - j) * log1mq)
+        if cdf >= confidence:
+            return j + 1
+    return n
+
+
+def spread_threshold(err, spread, bound=0.05, quantile=0.99, confidence=None):
     """Largest spread threshold (one of the given spreads) at which the `quantile` of the errors after
     fallback stays within `bound`; -inf when even the smallest spread exceeds it (everything falls
-    back)."""
+    back).  With `confidence`, the quantile is bounded from above at that confidence by an order
+    statistic of the errors after fallback (quantile_upper_rank) instead of estimated by it."""
     err = np.asarray(err, np.float64)
     spread = np.asarray(spread, np.float64)
+    rank = None if confidence is None else quantile_upper_rank(len(err), quantile, confidence)
     best = -np.inf
     for t in np.unique(spread):
-        if np.quantile(after_fallback(err, spread, t), quantile) <= bound:
+        a = after_fallback(err, spread, t)
+        v = np.quantile(a, quantile) if rank is None else np.partition(a, rank - 1)[rank - 1]
+        if v <= bound:
             best = t
     return float(best)
 

@@ -98,6 +98,34 @@ def test_spread_threshold_is_the_largest_that_keeps_the_bound():
             assert not any(ok(th) for th in spread[spread > t])
 
 
+def test_quantile_upper_rank_is_the_binomial_bound():
+    from scipy import stats
+    for n, q, c in ((2048, 0.99, 0.95), (2048, 0.5, 0.95), (100, 0.9, 0.9), (50, 0.5, 0.99), (10, 0.5, 0.5)):
+        k = S.quantile_upper_rank(n, q, c)
+        # the smallest k with P(Binomial(n, q) <= k - 1) >= c
+        assert stats.binom.cdf(k - 1, n, q) >= c and stats.binom.cdf(k - 2, n, q) < c
+    # coverage: X_(k) of n uniform samples lies above the true q-quantile (= q) in >= c of trials
+    rng = np.random.default_rng(8)
+    n, q, c = 400, 0.9, 0.95
+    k = S.quantile_upper_rank(n, q, c)
+    hits = np.mean([np.sort(rng.uniform(0, 1, n))[k - 1] >= q for _ in range(4000)])
+    assert hits >= c - 0.015
+
+
+def test_spread_threshold_with_confidence_uses_the_upper_order_statistic():
+    rng = np.random.default_rng(4)
+    n = 300
+    spread = rng.uniform(0, 1, n)
+    err = np.clip(0.08 * spread + rng.normal(0, 0.02, n), 0, None)
+    k = S.quantile_upper_rank(n, 0.9, 0.95)
+    t = S.spread_threshold(err, spread, 0.05, 0.9, confidence=0.95)
+    ok = lambda th: np.sort(np.where(spread <= th, err, 0.0))[k - 1] <= 0.05
+    assert np.isfinite(t) and ok(t)
+    assert not any(ok(th) for th in spread[spread > t])
+    # stricter than the plain estimate
+    assert t <= S.spread_threshold(err, spread, 0.05, 0.9)
+
+
 def test_mixture_quantile():
     je = np.linspace(0.0, 0.1, 101)
     assert S.mixture_quantile(je, 0.5, 0.4) == 0.0
