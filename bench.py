@@ -422,13 +422,17 @@ def surrogate_bench(args):
     del gathers
     dump = {"val_params": params[ntr:], "val_err": verr, "val_spread": vspread}
     cand = S.draw_params(ncand, seed=4)
+    # the screens' output: one page-locked buffer for the 16 384 gathers, allocated before the
+    # screens and reused by each (a serving loop's output buffer)
+    from poe2d_amd.mlp import pinned_empty
+    out = pinned_empty((ncand,) + sur.shape)
     screens = []
     specs = [(q, None) for q in quantiles]
     if args.sur_confidence > 0:
         specs.append((quantiles[-1], args.sur_confidence))
     for q, conf in specs:
         thr = S.spread_threshold(verr, vspread, 0.05, q, confidence=conf)
-        res = sur.screen(cand, thr, progress=lambda d, n: d % 1024 == 0 and print(
+        res = sur.screen(cand, thr, out=out, progress=lambda d, n: d % 1024 == 0 and print(
             f"[surrogate] re-solved {d} of {n}", file=sys.stderr, flush=True))
         je = res.judge_errors
         ffb = len(res.fallback) / res.n

@@ -363,6 +363,10 @@ poe2d_status poe2d_mlp_train(poe2d_mlp *m, int member, int nsteps, int batch, co
 /* Ensemble prediction of n rows: mean[n][sizes[nlayers]] (may be null) and spread[n] (may be
  * null) = ||standard deviation over members|| / ||mean|| per row. */
 poe2d_status poe2d_mlp_predict(poe2d_mlp *m, int n, const float *x, float *mean, float *spread);
+/* Per-output factors scale[sizes[nlayers]] applied on the device to every predicted mean (the
+ * outputs' units when the network was trained on normalised targets); the spread stays that of
+ * the unscaled outputs.  A null scale removes them. */
+poe2d_status poe2d_mlp_set_output_scale(poe2d_mlp *m, const float *scale);
 /* Device time of one forward pass and of one training step (forward, backward, Adam) of member
  * 0 on a batch of `batch` rows, averaged over iters (a measurement helper). */
 poe2d_status poe2d_mlp_time(poe2d_mlp *m, int batch, int iters, double *forward_ms, double *train_ms);
@@ -378,6 +382,11 @@ poe2d_status poe2d_gemm_bf16(int M, int N, int K, const float *A, const float *B
  * that GPU's scratch: calls that overlap on one GPU (from several host threads) race. */
 poe2d_status poe2d_gemm_bf16_noalloc(int M, int N, int K, const float *A, const float *B, float *C, int ksplit);
 void poe2d_mlp_destroy(poe2d_mlp *m);
+/* Page-locked host memory (no counterpart in the reference), for large results copied from the
+ * device such as a screen's predicted gathers: copies into it run at the link's DMA rate and its
+ * pages are resident, so no first-touch faults slow a copy down.  Freed by poe2d_host_free. */
+poe2d_status poe2d_host_alloc(size_t bytes, void **out);
+void poe2d_host_free(void *p);
 
 /* ---- Reference-format input and output (host only; no device needed) ---- */
 

@@ -141,10 +141,13 @@ def _declare(lib):
         "poe2d_mlp_set_dataset": ([vp, i, c_float_p, c_float_p], i),
         "poe2d_mlp_train": ([vp, i, i, i, c_int_p, ctypes.c_float, c_float_p], i),
         "poe2d_mlp_predict": ([vp, i, c_float_p, c_float_p, c_float_p], i),
+        "poe2d_mlp_set_output_scale": ([vp, c_float_p], i),
         "poe2d_mlp_time": ([vp, i, i, c_double_p, c_double_p], i),
         "poe2d_gemm_bf16": ([i, i, i, c_float_p, c_float_p, c_float_p, i], i),
         "poe2d_gemm_bf16_noalloc": ([i, i, i, c_float_p, c_float_p, c_float_p, i], i),
         "poe2d_mlp_destroy": ([vp], None),
+        "poe2d_host_alloc": ([ctypes.c_size_t, ctypes.POINTER(vp)], i),
+        "poe2d_host_free": ([vp], None),
         "poe2d_read_input1": ([ctypes.c_char_p, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p], i),
         "poe2d_read_input2": ([ctypes.c_char_p, c_double_p], i),
         "poe2d_read_grid_f32": ([ctypes.c_char_p, i, i, i, c_float_p], i),

@@ -106,6 +106,7 @@ struct poe2d_mlp {
     int nred = 512;
     // prediction accumulators
     float *acc_mean = nullptr, *acc_m2 = nullptr;
+    float *oscale = nullptr;  // per-output factor of the predicted mean (null: none)
     // resident dataset
     float *data_x = nullptr, *data_t = nullptr;
     int data_n = 0;
@@ -140,7 +141,7 @@ void free_mlp(poe2d_mlp *m)
     for (auto &p : m->g16) dfree(p);
     for (auto &p : m->g16t) dfree(p);
     dfree(m->dh); dfree(m->y); dfree(m->tgt); dfree(m->xin); dfree(m->work); dfree(m->partial);
-    dfree(m->acc_mean); dfree(m->acc_m2); dfree(m->data_x); dfree(m->data_t); dfree(m->idx); dfree(m->dloss);
+    dfree(m->acc_mean); dfree(m->acc_m2); dfree(m->oscale); dfree(m->data_x); dfree(m->data_t); dfree(m->idx); dfree(m->dloss);
     if (m->stream) (void)hipStreamDestroy(m->stream);
     m->stream = nullptr;
 }
@@ -212,6 +213,14 @@ __global__ void row_spread(const float *__restrict__ mean, const float *__restri
         __syncthreads();
     }
     if (threadIdx.x == 0) spread[r] = (float)(sqrt(sv[0] / (members > 1 ? members - 1 : 1)) / fmax(sqrt(sn[0]), 1e-30));
+}
+
+// mean[r][c] *= scale[c] for n rows of `cols` values.
+__global__ void scale_cols(float *__restrict__ mean, int n, int cols, const float *__restrict__ scale)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < (size_t)n * cols;
+         i += (size_t)gridDim.x * blockDim.x)
+        mean[i] *= scale[i % cols];
 }
 
 int blocks(size_t n) { return (int)std::min<size_t>((n + 255) / 256, 1u << 20); }
@@ -688,16 +697,38 @@ poe2d_status poe2d_mlp_predict(poe2d_mlp *m, int n, const float *x, float *mean,
             welford<<<blocks((size_t)nb * out), 256, 0, m->stream>>>(m->y, outp, nb, out, k, m->acc_mean, m->acc_m2);
             HIP_TRY(hipGetLastError());
         }
-        if (mean)
-            HIP_TRY(hipMemcpyAsync(mean + (size_t)r0 * out, m->acc_mean, (size_t)nb * out * sizeof(float),
-                                   hipMemcpyDeviceToHost, m->stream));
         if (spread) {
             row_spread<<<nb, 256, 0, m->stream>>>(m->acc_mean, m->acc_m2, out, E, m->partial);
             HIP_TRY(hipGetLastError());
             HIP_TRY(hipMemcpyAsync(spread + r0, m->partial, nb * sizeof(float), hipMemcpyDeviceToHost, m->stream));
         }
+        if (mean) {
+            // (the spread above is of the unscaled outputs)
+            if (m->oscale) {
+                scale_cols<<<blocks((size_t)nb * out), 256, 0, m->stream>>>(m->acc_mean, nb, out, m->oscale);
+                HIP_TRY(hipGetLastError());
+            }
+            HIP_TRY(hipMemcpyAsync(mean + (size_t)r0 * out, m->acc_mean, (size_t)nb * out * sizeof(float),
+                                   hipMemcpyDeviceToHost, m->stream));
+        }
         HIP_TRY(hipStreamSynchronize(m->stream));
     }
+    return POE2D_OK;
+}
+
+poe2d_status poe2d_mlp_set_output_scale(poe2d_mlp *m, const float *scale)
+{
+    poe2d_status st = check_mlp(m);
+    if (st) return st;
+    const int out = m->sizes[m->L];
+    if (!scale) {
+        dfree(m->oscale);
+        m->oscale = nullptr;
+        return POE2D_OK;
+    }
+    if (!m->oscale && (st = dmalloc(&m->oscale, out))) return st;
+    HIP_TRY(hipMemcpyAsync(m->oscale, scale, out * sizeof(float), hipMemcpyHostToDevice, m->stream));
+    HIP_TRY(hipStreamSynchronize(m->stream));
     return POE2D_OK;
 }
 
@@ -808,6 +839,22 @@ void poe2d_mlp_destroy(poe2d_mlp *m)
     if (m->stream) (void)hipStreamSynchronize(m->stream);
     free_mlp(m);
     delete m;
+}
+
+poe2d_status poe2d_host_alloc(size_t bytes, void **out)
+{
+    if (!out || bytes == 0) return fail(POE2D_E_INVALID, "host_alloc: bad arguments");
+    *out = nullptr;
+    if (hipHostMalloc(out, bytes, hipHostMallocDefault) != hipSuccess) {
+        *out = nullptr;
+        return fail(POE2D_E_NOMEM, "host_alloc: hipHostMalloc failed");
+    }
+    return POE2D_OK;
+}
+
+void poe2d_host_free(void *p)
+{
+    if (p) (void)hipHostFree(p);
 }
 
 }  // extern "C"

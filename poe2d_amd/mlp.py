@@ -1,6 +1,7 @@
 """The surrogate network through the C ABI (poe2d_mlp_*): an ensemble of bf16 fully connected
 networks with GELU, trained with Adam on the GPU's matrix cores."""
 import ctypes
+import weakref
 
 import numpy as np
 
@@ -89,11 +90,27 @@ class Network:
                                                         _f(losses)), "mlp_train")
         return losses
 
-    def predict(self, x, want_mean=True):
-        """(ensemble mean or None, spread per row)."""
+    def set_output_scale(self, scale):
+        """Per-output factors applied on the device to predicted means (None: none)."""
+        if scale is None:
+            _abi.check(self._lib, self._lib.poe2d_mlp_set_output_scale(self._h, None), "mlp_set_output_scale")
+            return
+        s = np.ascontiguousarray(scale, np.float32).ravel()
+        if s.size != self.sizes[-1]:
+            raise ValueError(f"output scale needs {self.sizes[-1]} values, got {s.size}")
+        _abi.check(self._lib, self._lib.poe2d_mlp_set_output_scale(self._h, _f(s)), "mlp_set_output_scale")
+
+    def predict(self, x, want_mean=True, out=None):
+        """(ensemble mean or None, spread per row).  out: a C-contiguous float32 [n, outputs] array to
+        receive the mean (page-locked memory from pinned_empty copies fastest)."""
         x = np.ascontiguousarray(x, np.float32)
         n = x.shape[0]
-        mean = np.zeros((n, self.sizes[-1]), np.float32) if want_mean else None
+        if out is not None:
+            if out.dtype != np.float32 or out.shape != (n, self.sizes[-1]) or not out.flags.c_contiguous:
+                raise ValueError("out must be a C-contiguous float32 array of shape (n, outputs)")
+            mean = out
+        else:
+            mean = np.zeros((n, self.sizes[-1]), np.float32) if want_mean else None
         spread = np.zeros(n, np.float32)
         _abi.check(self._lib, self._lib.poe2d_mlp_predict(self._h, n, _f(x), _f(mean) if want_mean else None,
                                                           _f(spread)), "mlp_predict")
@@ -121,6 +138,19 @@ class Network:
 
     def __exit__(self, *exc):
         self.close()
+
+
+def pinned_empty(shape, dtype=np.float32):
+    """An uninitialised numpy array in page-locked host memory (poe2d_host_alloc), released when the
+    array and every view of it are gone."""
+    lib = _abi.load_library()
+    dtype = np.dtype(dtype)
+    nbytes = int(np.prod(shape)) * dtype.itemsize
+    ptr = ctypes.c_void_p()
+    _abi.check(lib, lib.poe2d_host_alloc(ctypes.c_size_t(max(nbytes, 1)), ctypes.byref(ptr)), "host_alloc")
+    buf = (ctypes.c_char * max(nbytes, 1)).from_address(ptr.value)
+    weakref.finalize(buf, lib.poe2d_host_free, ctypes.c_void_p(ptr.value))
+    return np.frombuffer(buf, dtype=dtype, count=int(np.prod(shape))).reshape(shape)
 
 
 def gemm_bf16(a, b, ksplit=1):

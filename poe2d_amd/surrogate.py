@@ -429,6 +429,8 @@ class Surrogate:
         g = np.asarray(gathers, np.float32)
         rms = np.sqrt(np.mean(g.astype(np.float64) ** 2, axis=(0, 2, 3)))
         self.scale = np.where(rms > 0, rms, 1.0).astype(np.float32)
+        # predictions come back in the gathers' units: the device multiplies by the scale
+        self.net.set_output_scale(np.repeat(self.scale, self.nout // 3))
         self.net.set_dataset(features(params, self.geom, self.receivers), self._scaled(g))
         rng = np.random.default_rng(seed)
         n = len(params)
@@ -448,11 +450,26 @@ class Surrogate:
                 log(member, curves[-1])
         return curves
 
-    def predict(self, params):
-        """(gathers [n, 3, nrec, nsamp], ensemble spread [n] in units of the scaled targets)."""
-        mean, spread = self.net.predict(features(params, self.geom, self.receivers))
-        g = mean.reshape((len(params),) + self.shape)
-        g *= self.scale[None, :, None, None]
+    def predict(self, params, out=None, chunk=2048):
+        """(gathers [n, 3, nrec, nsamp], ensemble spread [n] in units of the scaled targets).  out: a
+        float32 array of that shape to fill (page-locked memory, mlp.pinned_empty, copies fastest).
+        The network inputs of the next `chunk` candidates are formed on a host thread while the
+        device predicts the current ones."""
+        params = np.asarray(params, np.float64).reshape(-1, 4)
+        n = len(params)
+        g = np.empty((n,) + self.shape, np.float32) if out is None else out
+        if g.shape != (n,) + self.shape or g.dtype != np.float32 or not g.flags.c_contiguous:
+            raise ValueError(f"out must be a C-contiguous float32 array of shape {(n,) + self.shape}")
+        flat = g.reshape(n, self.nout)
+        spread = np.empty(n, np.float32)
+        with ThreadPoolExecutor(1) as pool:
+            nxt = pool.submit(features, params[:chunk], self.geom, self.receivers)
+            for lo in range(0, n, chunk):
+                hi = min(n, lo + chunk)
+                x = nxt.result()
+                if hi < n:
+                    nxt = pool.submit(features, params[hi:hi + chunk], self.geom, self.receivers)
+                _, spread[lo:hi] = self.net.predict(x, out=flat[lo:hi])
         return g, spread
 
     def errors(self, pred, true):
@@ -467,13 +484,14 @@ class Surrogate:
         err = self.errors(pred, gathers)
         return spread_threshold(err, spread, bound, quantile), err, spread
 
-    def screen(self, params, threshold, seeds=None, judge_frac=0.05, seed=7, progress=None):
+    def screen(self, params, threshold, seeds=None, judge_frac=0.05, seed=7, progress=None, out=None):
         """Predict every candidate, re-solve those whose spread exceeds `threshold`, then re-solve
         a random `judge_frac` of the others to measure the surrogate's error (timed apart).  The
-        result's gathers hold, per candidate, the prediction or (re-solved) the solver's gather.
+        result's gathers hold, per candidate, the prediction or (re-solved) the solver's gather;
+        out: the array to hold them (as for predict), reused by a caller that screens repeatedly.
         progress: passed to simulate for both sets of re-solves."""
         t0 = time.perf_counter()
-        pred, spread = self.predict(params)
+        pred, spread = self.predict(params, out=out)
         t_pred = time.perf_counter() - t0
         fallback = np.nonzero(spread > threshold)[0]
         fb = SolveTiming(0, 0.0, 0.0)

@@ -4,7 +4,7 @@ ensemble spread."""
 import numpy as np
 import pytest
 
-from poe2d_amd.mlp import Network, gemm_bf16
+from poe2d_amd.mlp import Network, gemm_bf16, pinned_empty
 
 from .mlp_reference import bf16, forward, loss_and_grads
 
@@ -169,3 +169,23 @@ def test_adam_matches_the_bias_corrected_update():
         # the other member's weights are untouched
         for (a, c), (b, d) in zip(untouched, net.get_params(0)):
             assert np.array_equal(a, b) and np.array_equal(c, d)
+
+
+def test_predict_into_page_locked_memory_with_output_scale():
+    """predict writes the ensemble mean into a caller's page-locked array, the same bits as into
+    its own; the device-side output scale multiplies the mean as a host FP32 product would and
+    leaves the spread (of the unscaled outputs) alone."""
+    rng = np.random.default_rng(17)
+    with Network([8, 64, 200], members=3, max_batch=64, seed=5) as net:
+        x = rng.standard_normal((100, 8)).astype(np.float32)  # two batches of at most 64 rows
+        m0, s0 = net.predict(x)
+        out = pinned_empty((100, 200))
+        m1, s1 = net.predict(x, out=out)
+        assert m1 is out and np.array_equal(m1, m0) and np.array_equal(s1, s0)
+        scale = rng.uniform(0.5, 2.0, 200).astype(np.float32)
+        net.set_output_scale(scale)
+        m2, s2 = net.predict(x)
+        assert np.array_equal(m2, m0 * scale[None, :]) and np.array_equal(s2, s0)
+        net.set_output_scale(None)
+        m3, _ = net.predict(x)
+        assert np.array_equal(m3, m0)
