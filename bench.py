@@ -415,7 +415,8 @@ def surrogate_bench(args):
     sur = S.Surrogate(geom, members=5, max_batch=1024, receivers=bool(args.sur_receivers))
     t0 = time.perf_counter()
     sur.fit(params[:ntr], gathers[:ntr], args.sur_steps,
-            log=lambda k, l: print(f"[surrogate] member {k}: loss {l[-20:].mean():.5f}", file=sys.stderr, flush=True))
+            log=lambda k, l: print(f"[surrogate] member {k}: loss {l[-20:].mean():.5f}", file=sys.stderr, flush=True),
+            progress=lambda c, n: print(f"[surrogate] training chunk {c + 1} of {n}", file=sys.stderr, flush=True))
     train_s = time.perf_counter() - t0
     quantiles = [float(q) for q in args.sur_quantiles.split(",")]
     _, verr, vspread = sur.calibrate(params[ntr:], gathers[ntr:], bound=0.05, quantile=quantiles[0])

@@ -360,6 +360,13 @@ poe2d_status poe2d_mlp_set_dataset(poe2d_mlp *m, int n, const float *x, const fl
  * losses[step] (may be null) gets each batch's loss before its update. */
 poe2d_status poe2d_mlp_train(poe2d_mlp *m, int member, int nsteps, int batch, const int *indices, float lr,
                              float *losses);
+/* nsteps Adam steps of each of `count` distinct members (members[j]) on the dataset rows
+ * indices[(j * nsteps + step) * batch + i]; losses[j * nsteps + step] (may be null).  The members'
+ * steps run on two streams with a buffer set each, so one member's latency-bound launches overlap
+ * another's bandwidth-bound ones; every member's weights come out as poe2d_mlp_train leaves them,
+ * bit for bit. */
+poe2d_status poe2d_mlp_train_many(poe2d_mlp *m, int count, const int *members, int nsteps, int batch,
+                                  const int *indices, float lr, float *losses);
 /* Ensemble prediction of n rows: mean[n][sizes[nlayers]] (may be null) and spread[n] (may be
  * null) = ||standard deviation over members|| / ||mean|| per row. */
 poe2d_status poe2d_mlp_predict(poe2d_mlp *m, int n, const float *x, float *mean, float *spread);

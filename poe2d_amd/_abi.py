@@ -142,6 +142,7 @@ def _declare(lib):
         "poe2d_mlp_train": ([vp, i, i, i, c_int_p, ctypes.c_float, c_float_p], i),
         "poe2d_mlp_predict": ([vp, i, c_float_p, c_float_p, c_float_p], i),
         "poe2d_mlp_set_output_scale": ([vp, c_float_p], i),
+        "poe2d_mlp_train_many": ([vp, i, c_int_p, i, i, c_int_p, ctypes.c_float, c_float_p], i),
         "poe2d_mlp_time": ([vp, i, i, c_double_p, c_double_p], i),
         "poe2d_gemm_bf16": ([i, i, i, c_float_p, c_float_p, c_float_p, i], i),
         "poe2d_gemm_bf16_noalloc": ([i, i, i, c_float_p, c_float_p, c_float_p, i], i),

@@ -112,6 +112,15 @@ struct poe2d_mlp {
     int data_n = 0;
     int *idx = nullptr;      // batch row indices of up to kIndexSteps training steps
     float *dloss = nullptr;  // their losses
+    // poe2d_mlp_train_many: a second set of activations, gradients and partials with its own stream
+    // (same sizes, no members or dataset of its own), the step indices and losses of every member
+    // trained, and one event per member ordering its steps across the two streams
+    poe2d_mlp *twin = nullptr;
+    int *idx_all = nullptr;
+    size_t idx_all_cap = 0;
+    float *loss_all = nullptr;
+    size_t loss_all_cap = 0;
+    std::vector<hipEvent_t> step_done;
 };
 
 // Training steps whose batch indices go to the device in one copy: a copy from pageable host
@@ -142,6 +151,15 @@ void free_mlp(poe2d_mlp *m)
     for (auto &p : m->g16t) dfree(p);
     dfree(m->dh); dfree(m->y); dfree(m->tgt); dfree(m->xin); dfree(m->work); dfree(m->partial);
     dfree(m->acc_mean); dfree(m->acc_m2); dfree(m->oscale); dfree(m->data_x); dfree(m->data_t); dfree(m->idx); dfree(m->dloss);
+    dfree(m->idx_all); dfree(m->loss_all);
+    for (hipEvent_t ev : m->step_done) (void)hipEventDestroy(ev);
+    m->step_done.clear();
+    if (m->twin) {
+        if (m->twin->stream) (void)hipStreamSynchronize(m->twin->stream);
+        free_mlp(m->twin);
+        delete m->twin;
+        m->twin = nullptr;
+    }
     if (m->stream) (void)hipStreamDestroy(m->stream);
     m->stream = nullptr;
 }
@@ -281,6 +299,19 @@ poe2d_status scratch_address(unsigned char **out)
     return POE2D_OK;
 }
 
+// k splits of a product with a plain epilogue (M x N over k = K): doubled while the tiles times
+// the splits stay under 1024 blocks (about four per CU) and each split keeps at least 2 k tiles of
+// 32, within the partials buffer.  The 512-wide hidden layers' products (8 or 16 tiles) run on 64
+// blocks instead of 8; the output layer's input gradient (K = 151 552) on 1024.
+int product_splits(const poe2d_mlp *m, int M, int N, int K)
+{
+    int ksplit = 1;
+    const int tiles = ((M + 127) / 128) * ((N + 127) / 128);
+    while (tiles * ksplit < 1024 && K / (2 * ksplit) >= 64 && (size_t)ksplit * 2 * M * N <= m->work_floats)
+        ksplit *= 2;
+    return ksplit;
+}
+
 // Forward pass of member e on the first n rows of m->xin (already packed into h16[0]).
 poe2d_status forward(poe2d_mlp *m, Member &e, int n)
 {
@@ -299,21 +330,11 @@ poe2d_status forward(poe2d_mlp *m, Member &e, int n)
             o.ldb = l.outp;
         }
         HIP_TRY(gemm_bf16(m->h16[li], l.inp, l.w16, l.inp, n, l.out, l.inp, last ? EPI_LINEAR : EPI_GELU, o, m->work,
-                          m->work_floats, 1, m->stream));
+                          m->work_floats, product_splits(m, n, l.out, l.inp), m->stream));
     }
     return POE2D_OK;
 }
 
-// k splits of an input-gradient product (n x in over k = K): enough blocks for about four per CU
-// (1024 with the splits), each split at least 4 k tiles of 32, within the partials buffer.
-int input_grad_splits(const poe2d_mlp *m, int n, int in, int K)
-{
-    int ksplit = 1;
-    const int tiles = ((n + 127) / 128) * ((in + 127) / 128);
-    while (tiles * ksplit < 1024 && ksplit * 2 * 128 <= K / 4 && (size_t)ksplit * 2 * n * in <= m->work_floats)
-        ksplit *= 2;
-    return ksplit;
-}
 
 // Gradients of the mean squared error for member e on the batch in h16[0] / tgt (n rows, n a
 // multiple of 32); loss (mean) to *loss when non-null.
@@ -341,12 +362,12 @@ poe2d_status backward(poe2d_mlp *m, Member &e, int n, float *loss)
         o.f32 = l.gw;
         o.ldf = l.inp;
         HIP_TRY(gemm_bf16(m->g16t[li], m->max_batch, m->h16t[li], m->max_batch, l.outp, l.inp, n, EPI_LINEAR, o,
-                          m->work, m->work_floats, 1, m->stream));
+                          m->work, m->work_floats, product_splits(m, l.outp, l.inp, n), m->stream));
         if (li < L - 1) HIP_TRY(bias_gradient(m->g16t[li], m->max_batch, l.outp, n, l.gb, m->stream));
         if (li == 0) break;
         // G_{l-1} = (G_l W_l) * gelu'(Z_{l-1}): A = G_l [n][outp], B = W_l^T [inp][outp]
         const int K = l.outp;
-        const int ksplit = input_grad_splits(m, n, l.in, K);
+        const int ksplit = product_splits(m, n, l.in, K);
         GemmOut od;
         od.f32 = m->dh;
         od.ldf = l.inp;
@@ -377,8 +398,8 @@ poe2d_status train_step(poe2d_mlp *m, Member &e, int n, const float *tgt, int ld
         o.ldp = l.outp;
         o.b16 = m->h16[li + 1];
         o.ldb = l.outp;
-        HIP_TRY(gemm_bf16(m->h16[li], l.inp, l.w16, l.inp, n, l.out, l.inp, EPI_GELU, o, m->work, m->work_floats, 1,
-                          m->stream));
+        HIP_TRY(gemm_bf16(m->h16[li], l.inp, l.w16, l.inp, n, l.out, l.inp, EPI_GELU, o, m->work, m->work_floats,
+                          product_splits(m, n, l.out, l.inp), m->stream));
     }
     Layer &top = e.layers[L - 1];
     {
@@ -413,7 +434,7 @@ poe2d_status train_step(poe2d_mlp *m, Member &e, int n, const float *tgt, int ld
         if (li > 0) {
             // G_{l-1} = (G_l W_l) * gelu'(Z_{l-1}) with the weights of this step
             const int K = l.outp;
-            const int ksplit = input_grad_splits(m, n, l.in, K);
+            const int ksplit = product_splits(m, n, l.in, K);
             GemmOut od;
             od.f32 = m->dh;
             od.ldf = l.inp;
@@ -446,7 +467,7 @@ poe2d_status train_step(poe2d_mlp *m, Member &e, int n, const float *tgt, int ld
             o.f32 = l.gw;
             o.ldf = l.inp;
             HIP_TRY(gemm_bf16(m->g16t[li], m->max_batch, m->h16t[li], m->max_batch, l.outp, l.inp, n, EPI_LINEAR, o,
-                              m->work, m->work_floats, 1, m->stream));
+                              m->work, m->work_floats, product_splits(m, l.outp, l.inp, n), m->stream));
             HIP_TRY(bias_gradient(m->g16t[li], m->max_batch, l.outp, n, l.gb, m->stream));
             HIP_TRY(adam_matrix_step(l.w, l.mw, l.vw, l.gw, l.outp, l.inp, lr, b1, b2, eps, e.t, l.w16, l.w16t,
                                      m->stream));
@@ -473,6 +494,82 @@ poe2d_status check_member(poe2d_mlp *m, int member)
 }
 
 }  // namespace
+
+// One training step of member e on lane (m itself or its twin: buffers and stream) over the
+// dataset rows idx[0 .. batch) of `data` (the handle that holds the dataset): the batch's inputs
+// gathered and packed, the targets read in place through idx.
+static poe2d_status dataset_step(poe2d_mlp *lane, const poe2d_mlp *data, Member &e, int batch, const int *idx, float lr,
+                                 float *loss_dev)
+{
+    const int in = lane->sizes[0], out = lane->sizes[lane->L];
+    gather_rows<<<blocks((size_t)batch * in), 256, 0, lane->stream>>>(data->data_x, in, idx, batch, in, lane->xin, in);
+    pack_input<<<blocks((size_t)batch * in), 256, 0, lane->stream>>>(lane->xin, batch, in, lane->h16[0], pad128(in));
+    HIP_TRY(hipGetLastError());
+    return train_step(lane, e, batch, data->data_t, out, idx, lr, loss_dev);
+}
+
+// Activations, gradients and partials of a network of m's sizes, and with `predict` the output
+// and target rows and the prediction accumulators.
+static poe2d_status alloc_scratch(poe2d_mlp *m, bool predict)
+{
+    const int B = m->max_batch, nlayers = m->L;
+    const int *sizes = m->sizes.data();
+    poe2d_status st;
+    m->h16.assign(nlayers + 1, nullptr);
+    m->h16t.assign(nlayers + 1, nullptr);
+    m->pre.assign(nlayers, nullptr);
+    m->g16.assign(nlayers, nullptr);
+    m->g16t.assign(nlayers, nullptr);
+    int hidden_max = 1;
+    for (int li = 0; li <= nlayers; ++li) {
+        const int w = pad128(sizes[li]);
+        if ((st = dmalloc(&m->h16[li], (size_t)B * w)) || (st = dmalloc(&m->h16t[li], (size_t)B * w))) return st;
+        if (li < nlayers) {
+            const int wo = pad128(sizes[li + 1]);
+            if ((st = dmalloc(&m->pre[li], (size_t)B * wo)) || (st = dmalloc(&m->g16[li], (size_t)B * wo)) ||
+                (st = dmalloc(&m->g16t[li], (size_t)B * wo)))
+                return st;
+        }
+        if (li > 0 && li < nlayers) hidden_max = std::max(hidden_max, w);
+    }
+    const int outp = pad128(sizes[nlayers]);
+    // split-K partials of the input-gradient products, and the output bias gradient's per-tile sums
+    m->work_floats = std::max((size_t)128 * B * hidden_max, (size_t)((B + 127) / 128) * pad128(sizes[nlayers]));
+    m->nred = std::max({m->nred, B, mse_blocks(sizes[nlayers]), gemm_tiles(B, sizes[nlayers])});
+    if ((st = dmalloc(&m->dh, (size_t)B * std::max(hidden_max, pad128(sizes[0])))) ||
+        (st = dmalloc(&m->xin, (size_t)B * sizes[0])) || (st = dmalloc(&m->work, m->work_floats)) ||
+        (st = dmalloc(&m->partial, m->nred)) || (st = dmalloc(&m->idx, (size_t)kIndexSteps * B)) ||
+        (st = dmalloc(&m->dloss, kIndexSteps)))
+        return st;
+    if (predict &&
+        ((st = dmalloc(&m->y, (size_t)B * outp)) || (st = dmalloc(&m->tgt, (size_t)B * outp)) ||
+         (st = dmalloc(&m->acc_mean, (size_t)B * sizes[nlayers])) || (st = dmalloc(&m->acc_m2, (size_t)B * sizes[nlayers]))))
+        return st;
+    return POE2D_OK;
+}
+
+// The second set of training buffers and stream of poe2d_mlp_train_many, made on first use.
+static poe2d_status ensure_twin(poe2d_mlp *m)
+{
+    if (m->twin) return POE2D_OK;
+    poe2d_mlp *t = new (std::nothrow) poe2d_mlp;
+    if (!t) return fail(POE2D_E_NOMEM, "mlp_train_many: out of host memory");
+    t->device = m->device;
+    t->sizes = m->sizes;
+    t->L = m->L;
+    t->max_batch = m->max_batch;
+    poe2d_status st = POE2D_OK;
+    if (hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking) != hipSuccess)
+        st = fail(POE2D_E_HIP, "mlp_train_many: no stream");
+    if (!st) st = alloc_scratch(t, false);
+    if (st) {
+        free_mlp(t);
+        delete t;
+        return st;
+    }
+    m->twin = t;
+    return POE2D_OK;
+}
 
 extern "C" {
 
@@ -523,35 +620,7 @@ poe2d_status poe2d_mlp_create(int nlayers, const int *sizes, int members, int ma
             if ((st = sync_copies(m, l))) return bail(st);
         }
     }
-    const int B = max_batch;
-    m->h16.assign(nlayers + 1, nullptr);
-    m->h16t.assign(nlayers + 1, nullptr);
-    m->pre.assign(nlayers, nullptr);
-    m->g16.assign(nlayers, nullptr);
-    m->g16t.assign(nlayers, nullptr);
-    int hidden_max = 1;
-    for (int li = 0; li <= nlayers; ++li) {
-        const int w = pad128(sizes[li]);
-        if ((st = dmalloc(&m->h16[li], (size_t)B * w)) || (st = dmalloc(&m->h16t[li], (size_t)B * w))) return bail(st);
-        if (li < nlayers) {
-            const int wo = pad128(sizes[li + 1]);
-            if ((st = dmalloc(&m->pre[li], (size_t)B * wo)) || (st = dmalloc(&m->g16[li], (size_t)B * wo)) ||
-                (st = dmalloc(&m->g16t[li], (size_t)B * wo)))
-                return bail(st);
-        }
-        if (li > 0 && li < nlayers) hidden_max = std::max(hidden_max, w);
-    }
-    const int outp = pad128(sizes[nlayers]);
-    // split-K partials of the input-gradient products, and the output bias gradient's per-tile sums
-    m->work_floats = std::max((size_t)128 * B * hidden_max, (size_t)((B + 127) / 128) * pad128(sizes[nlayers]));
-    m->nred = std::max({m->nred, B, mse_blocks(sizes[nlayers]), gemm_tiles(B, sizes[nlayers])});
-    if ((st = dmalloc(&m->dh, (size_t)B * std::max(hidden_max, pad128(sizes[0])))) ||
-        (st = dmalloc(&m->y, (size_t)B * outp)) || (st = dmalloc(&m->tgt, (size_t)B * outp)) ||
-        (st = dmalloc(&m->xin, (size_t)B * sizes[0])) || (st = dmalloc(&m->work, m->work_floats)) ||
-        (st = dmalloc(&m->partial, m->nred)) || (st = dmalloc(&m->acc_mean, (size_t)B * sizes[nlayers])) ||
-        (st = dmalloc(&m->acc_m2, (size_t)B * sizes[nlayers])) || (st = dmalloc(&m->idx, (size_t)kIndexSteps * B)) ||
-        (st = dmalloc(&m->dloss, kIndexSteps)))
-        return bail(st);
+    if ((st = alloc_scratch(m, true))) return bail(st);
     if (hipStreamSynchronize(m->stream) != hipSuccess) return bail(fail(POE2D_E_HIP, "mlp_create: initialisation failed"));
     *out = m;
     return POE2D_OK;
@@ -661,23 +730,78 @@ poe2d_status poe2d_mlp_train(poe2d_mlp *m, int member, int nsteps, int batch, co
     for (long long i = 0; i < (long long)nsteps * batch; ++i)
         if (indices[i] < 0 || indices[i] >= m->data_n) return fail(POE2D_E_INVALID, "mlp_train: index outside the dataset");
     Member &e = m->members[member];
-    const int in = m->sizes[0], out = m->sizes[m->L];
     for (int s = 0; s < nsteps; ++s) {
         const int c = s % kIndexSteps;
         if (c == 0)
             HIP_TRY(hipMemcpyAsync(m->idx, indices + (size_t)s * batch,
                                    (size_t)std::min(kIndexSteps, nsteps - s) * batch * sizeof(int),
                                    hipMemcpyHostToDevice, m->stream));
-        const int *idx = m->idx + (size_t)c * batch;
-        gather_rows<<<blocks((size_t)batch * in), 256, 0, m->stream>>>(m->data_x, in, idx, batch, in, m->xin, in);
-        pack_input<<<blocks((size_t)batch * in), 256, 0, m->stream>>>(m->xin, batch, in, m->h16[0], pad128(in));
-        HIP_TRY(hipGetLastError());
-        // the targets are read in place from the dataset through the batch's row indices
-        if ((st = train_step(m, e, batch, m->data_t, out, idx, lr, losses ? m->dloss + c : nullptr))) return st;
+        if ((st = dataset_step(m, m, e, batch, m->idx + (size_t)c * batch, lr, losses ? m->dloss + c : nullptr)))
+            return st;
         if (losses && (c == kIndexSteps - 1 || s == nsteps - 1))
             HIP_TRY(hipMemcpyAsync(losses + (s - c), m->dloss, (size_t)(c + 1) * sizeof(float), hipMemcpyDeviceToHost,
                                    m->stream));
     }
+    HIP_TRY(hipStreamSynchronize(m->stream));
+    return POE2D_OK;
+}
+
+poe2d_status poe2d_mlp_train_many(poe2d_mlp *m, int count, const int *members, int nsteps, int batch,
+                                  const int *indices, float lr, float *losses)
+{
+    poe2d_status st = check_mlp(m);
+    if (st) return st;
+    if (!m->data_n) return fail(POE2D_E_STATE, "mlp_train_many: no dataset (call poe2d_mlp_set_dataset)");
+    if (count < 1 || !members || batch < 32 || batch > m->max_batch || batch % 32 || nsteps < 0 ||
+        (nsteps > 0 && !indices) || !(lr > 0))
+        return fail(POE2D_E_INVALID, "mlp_train_many: bad arguments");
+    for (int j = 0; j < count; ++j) {
+        if ((st = check_member(m, members[j]))) return st;
+        for (int q = 0; q < j; ++q)
+            if (members[q] == members[j]) return fail(POE2D_E_INVALID, "mlp_train_many: a member is listed twice");
+    }
+    const size_t ni = (size_t)count * nsteps * batch;
+    for (size_t i = 0; i < ni; ++i)
+        if (indices[i] < 0 || indices[i] >= m->data_n)
+            return fail(POE2D_E_INVALID, "mlp_train_many: index outside the dataset");
+    if (nsteps == 0) return POE2D_OK;
+    if (count > 1 && (st = ensure_twin(m))) return st;
+    if (ni > m->idx_all_cap) {
+        dfree(m->idx_all);
+        m->idx_all_cap = 0;
+        if ((st = dmalloc(&m->idx_all, ni))) return st;
+        m->idx_all_cap = ni;
+    }
+    const size_t nl = (size_t)count * nsteps;
+    if (nl > m->loss_all_cap) {
+        dfree(m->loss_all);
+        m->loss_all_cap = 0;
+        if ((st = dmalloc(&m->loss_all, nl))) return st;
+        m->loss_all_cap = nl;
+    }
+    while ((int)m->step_done.size() < count) {
+        hipEvent_t ev;
+        HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        m->step_done.push_back(ev);
+    }
+    HIP_TRY(hipMemcpyAsync(m->idx_all, indices, ni * sizeof(int), hipMemcpyHostToDevice, m->stream));
+    // (everything earlier on the handle's stream, these indices included, is done before the
+    // second stream starts)
+    HIP_TRY(hipStreamSynchronize(m->stream));
+    // Member-steps k = s * count + j in that order, on alternating streams: with an odd count a
+    // member's consecutive steps alternate too, and each waits for its previous one.
+    for (int s = 0; s < nsteps; ++s)
+        for (int j = 0; j < count; ++j) {
+            const long long k = (long long)s * count + j;
+            poe2d_mlp *lane = (count == 1 || k % 2 == 0) ? m : m->twin;
+            if (count > 1 && count % 2 == 1 && s > 0) HIP_TRY(hipStreamWaitEvent(lane->stream, m->step_done[j], 0));
+            if ((st = dataset_step(lane, m, m->members[members[j]], batch, m->idx_all + ((size_t)j * nsteps + s) * batch,
+                                   lr, losses ? m->loss_all + (size_t)j * nsteps + s : nullptr)))
+                return st;
+            if (count > 1) HIP_TRY(hipEventRecord(m->step_done[j], lane->stream));
+        }
+    if (m->twin) HIP_TRY(hipStreamSynchronize(m->twin->stream));
+    if (losses) HIP_TRY(hipMemcpyAsync(losses, m->loss_all, nl * sizeof(float), hipMemcpyDeviceToHost, m->stream));
     HIP_TRY(hipStreamSynchronize(m->stream));
     return POE2D_OK;
 }

@@ -100,6 +100,21 @@ class Network:
             raise ValueError(f"output scale needs {self.sizes[-1]} values, got {s.size}")
         _abi.check(self._lib, self._lib.poe2d_mlp_set_output_scale(self._h, _f(s)), "mlp_set_output_scale")
 
+    def train_many(self, indices, batch, lr, members):
+        """Adam steps of several members at once (two streams), member j on dataset rows
+        indices[j].reshape(-1, batch); returns their losses [len(members), steps].  The weights
+        come out as train() per member leaves them."""
+        members = np.ascontiguousarray(members, np.int32).ravel()
+        idx = np.ascontiguousarray(indices, np.int32).reshape(len(members), -1)
+        nsteps = idx.shape[1] // batch
+        if idx.shape[1] != nsteps * batch:
+            raise ValueError("every member needs a whole number of batches")
+        losses = np.zeros((len(members), nsteps), np.float32)
+        _abi.check(self._lib, self._lib.poe2d_mlp_train_many(
+            self._h, len(members), members.ctypes.data_as(_abi.c_int_p), nsteps, batch,
+            idx.ctypes.data_as(_abi.c_int_p), ctypes.c_float(lr), _f(losses)), "mlp_train_many")
+        return losses
+
     def predict(self, x, want_mean=True, out=None):
         """(ensemble mean or None, spread per row).  out: a C-contiguous float32 [n, outputs] array to
         receive the mean (page-locked memory from pinned_empty copies fastest)."""

@@ -422,10 +422,12 @@ class Surrogate:
     def _scaled(self, g):
         return (g / self.scale[None, :, None, None]).reshape(len(g), -1)
 
-    def fit(self, params, gathers, steps, batch=256, lr=1e-3, lr_final=1e-5, chunks=20, seed=1, log=None):
+    def fit(self, params, gathers, steps, batch=256, lr=1e-3, lr_final=1e-5, chunks=20, seed=1, log=None,
+            progress=None):
         """Train every member on its own shuffles of the data, the learning rate falling from `lr`
-        to `lr_final` on a cosine over `chunks` pieces; returns the loss curves.  log(member,
-        losses) is called after each member."""
+        to `lr_final` on a cosine over `chunks` pieces, the members side by side (train_many);
+        returns the loss curves.  log(member, losses) is called for each member at the end,
+        progress(chunk, chunks) after each chunk."""
         g = np.asarray(gathers, np.float32)
         rms = np.sqrt(np.mean(g.astype(np.float64) ** 2, axis=(0, 2, 3)))
         self.scale = np.where(rms > 0, rms, 1.0).astype(np.float32)
@@ -435,19 +437,21 @@ class Surrogate:
         rng = np.random.default_rng(seed)
         n = len(params)
         cuts = np.linspace(0, steps, chunks + 1).round().astype(int)
-        curves = []
-        for member in range(self.net.members):
-            need = steps * batch
-            idx = np.concatenate([rng.permutation(n) for _ in range(need // n + 1)])[:need]
-            losses = []
-            for c in range(chunks):
-                if cuts[c + 1] == cuts[c]:
-                    continue
-                rate = lr_final + 0.5 * (lr - lr_final) * (1 + np.cos(np.pi * c / chunks))
-                losses.append(self.net.train(idx[cuts[c] * batch:cuts[c + 1] * batch], batch, rate, member))
-            curves.append(np.concatenate(losses))
-            if log:
-                log(member, curves[-1])
+        need = steps * batch
+        members = list(range(self.net.members))
+        idx = np.stack([np.concatenate([rng.permutation(n) for _ in range(need // n + 1)])[:need] for _ in members])
+        losses = []
+        for c in range(chunks):
+            if cuts[c + 1] == cuts[c]:
+                continue
+            rate = lr_final + 0.5 * (lr - lr_final) * (1 + np.cos(np.pi * c / chunks))
+            losses.append(self.net.train_many(idx[:, cuts[c] * batch:cuts[c + 1] * batch], batch, rate, members))
+            if progress:
+                progress(c, chunks)
+        curves = list(np.concatenate(losses, axis=1)) if losses else [np.zeros(0, np.float32) for _ in members]
+        if log:
+            for member in members:
+                log(member, curves[member])
         return curves
 
     def predict(self, params, out=None, chunk=2048):

@@ -110,7 +110,11 @@ def test_screening_network_shape_forward_and_gradients():
         loss, grads = net.gradients(x, t)
     loss_ref, grads_ref = loss_and_grads(params, x, t, emulate=True)
     say(f"207-512-512-512-151500: forward rel {ey:.2e}; loss {loss:.6f} vs {loss_ref:.6f}")
-    assert ey < 1e-4
+    # The hidden products are split over k (64 blocks instead of 8) and their FP32 partial sums
+    # added in another order than the FP64 reference's sums: a pre-activation within an FP32
+    # rounding of a bf16 rounding boundary can round the other way, one bf16 ulp (2^-8 relative),
+    # which moves this output by about 1e-4 relative per such activation.  The bound allows a few.
+    assert ey < 1e-3
     assert abs(loss - loss_ref) <= 1e-4 * loss_ref
     for l, ((gw, gb), (rw, rb)) in enumerate(zip(grads, grads_ref)):
         ew, eb = rel(gw, rw), rel(gb, rb)
@@ -189,3 +193,27 @@ def test_predict_into_page_locked_memory_with_output_scale():
         net.set_output_scale(None)
         m3, _ = net.predict(x)
         assert np.array_equal(m3, m0)
+
+
+def test_train_many_equals_training_one_member_at_a_time():
+    """train_many (members side by side on two streams, an odd count so that each member's steps
+    alternate between them) leaves every member's weights and losses as train() does, bit for bit."""
+    rng = np.random.default_rng(23)
+    sizes = [16, 128, 128, 300]
+    x = rng.standard_normal((512, 16)).astype(np.float32)
+    t = np.tanh(x @ rng.standard_normal((16, 300))).astype(np.float32)
+    idx = [rng.integers(0, 512, size=40 * 64) for _ in range(3)]
+    with Network(sizes, members=3, max_batch=64, seed=9) as a, Network(sizes, members=3, max_batch=64, seed=9) as b:
+        a.set_dataset(x, t)
+        b.set_dataset(x, t)
+        la = [a.train(idx[k], 64, 1e-3, member=k) for k in range(3)]
+        lb = b.train_many(np.stack(idx), 64, 1e-3, members=[0, 1, 2])
+        for k in range(3):
+            assert np.array_equal(la[k], lb[k]), k
+            for layer, ((wa, ba), (wb, bb)) in enumerate(zip(a.get_params(k), b.get_params(k))):
+                assert np.array_equal(wa, wb) and np.array_equal(ba, bb), (k, layer)
+        # a second call continues from there (the Adam step counts and moments carry on)
+        a.train(idx[1][:640], 64, 1e-3, member=1)
+        b.train_many(idx[1][None, :640], 64, 1e-3, members=[1])
+        for (wa, ba), (wb, bb) in zip(a.get_params(1), b.get_params(1)):
+            assert np.array_equal(wa, wb) and np.array_equal(ba, bb)
