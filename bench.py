@@ -350,8 +350,9 @@ def network_timing(receivers=True):
     """One ensemble member of the screening network at its full shape (207 -> 512 -> 512 -> 512 ->
     151 500 with the layout inputs, 4 -> ... without; bf16 on the matrix cores), timed with hipEvents:
     one forward pass over an inference batch of 1024 and one training step (forward, backward,
-    Adam) over a batch of 256.  FLOPs are 2 x the MACs per sample forward and 3 x that per training
-    sample.  Returns (roofline dict, layer sizes)."""
+    Adam) over a batch of 256; then the 5-member training loop's wall time per member-step.  FLOPs
+    are 2 x the MACs per sample forward and 3 x that per training sample.  Returns (roofline dict,
+    layer sizes)."""
     from poe2d_amd import surrogate as S
     from poe2d_amd.mlp import Network
 
@@ -363,16 +364,33 @@ def network_timing(receivers=True):
         net.time(256, 2)
         fwd_ms, _ = net.time(1024, 10)
         _, train_ms = net.time(256, 10)
+    # the training loop of the 5-member ensemble (train_many: members side by side on two streams,
+    # batches gathered from a resident dataset of 2048 synthetic rows): wall time per member-step
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal((2048, sizes[0])).astype(np.float32)
+    t = (0.01 * rng.standard_normal((2048, sizes[-1]))).astype(np.float32)
+    with Network(sizes, members=5, max_batch=256) as net:
+        net.set_dataset(x, t)
+        idx = rng.integers(0, 2048, size=(5, 220 * 256))
+        net.train_many(idx[:, :20 * 256], 256, 1e-4, list(range(5)))
+        t0 = time.perf_counter()
+        net.train_many(idx[:, 20 * 256:], 256, 1e-4, list(range(5)))
+        loop_ms = (time.perf_counter() - t0) * 1e3 / (5 * 200)
+    del x, t
     fwd_tflops = 2 * macs * 1024 / (fwd_ms / 1e3) / 1e12
     train_tflops = 3 * 2 * macs * 256 / (train_ms / 1e3) / 1e12
     print(f"[bench] network {'-'.join(map(str, sizes))}: forward(1024) {fwd_ms:.3f} ms = {fwd_tflops:.0f} TFLOP/s; "
-          f"train step(256) {train_ms:.3f} ms = {train_tflops:.0f} TFLOP/s", file=sys.stderr, flush=True)
+          f"train step(256) {train_ms:.3f} ms = {train_tflops:.0f} TFLOP/s; 5-member training loop "
+          f"{loop_ms:.3f} ms per member-step", file=sys.stderr, flush=True)
     return ({"bound": "mfma", "achieved": round(fwd_tflops, 1), "peak": MFMA_BF16_PEAK_TFLOPS,
              "unit": "TFLOP/s", "frac": round(fwd_tflops / MFMA_BF16_PEAK_TFLOPS, 4), "traffic": None,
              "per_launch": f"one member's forward pass over 1024 samples: 2 x {macs / 1e6:.2f} M MACs each",
              "train_step_tflops": round(train_tflops, 1),
              "train_step_frac": round(train_tflops / MFMA_BF16_PEAK_TFLOPS, 4),
-             "forward_ms": round(fwd_ms, 4), "train_step_ms": round(train_ms, 4)}, sizes)
+             "forward_ms": round(fwd_ms, 4), "train_step_ms": round(train_ms, 4),
+             "train_loop_ms_per_member_step": round(loop_ms, 4),
+             "train_loop": "5 members side by side on two streams (poe2d_mlp_train_many), batch 256 gathered "
+                           "from a resident dataset, wall time over 200 steps each"}, sizes)
 
 
 def surrogate_bench(args):
