@@ -169,11 +169,11 @@ poe2d_status poe2d_set_receivers(poe2d_solver *s, const poe2d_receivers *rec);
  *   schedule 1: a stress launch and a velocity launch per step, intermediates through memory.
  *   schedule 2: as schedule 0, always reading the per-point coefficient arrays.
  *   schedule 3 (default of FP32 handles, block_w 256, tile_rows 24): two time steps per launch
- *     (FP32 grids with a material palette of at most 28 entries; tiles of block_w - 12 columns,
- *     block_w 64, 128 or 256, by tile_rows rows, multiples of 6 best), the intermediate state kept
- *     on chip; an odd last step of a run takes one schedule-0 launch.  FP64 handles and grids with
- *     no palette or a larger one run schedule 0 (block_w 64 as 128), and poe2d_autotune then
- *     tunes that kernel.
+ *     (tiles of block_w - 12 columns, block_w 64, 128 or 256, by tile_rows rows, multiples of 6
+ *     best), the intermediate state kept on chip; the coefficients through a palette of at most 28
+ *     entries, or from the per-point coefficient groups on grids without one; an odd last step of
+ *     a run takes one schedule-0 launch.  FP64 handles run schedule 0 (block_w 64 as 128), and
+ *     poe2d_autotune then tunes that kernel.
  * Tile heights do not change a bit; schedules and block widths agree to rounding.  The choice
  * only changes speed. */
 poe2d_status poe2d_set_schedule(poe2d_solver *s, int schedule, int block_w, int tile_rows);
@@ -224,8 +224,8 @@ poe2d_status poe2d_reset(poe2d_solver *s);
 
 /* Pick the fastest tile shape of the handle's schedule for its grid: times nsteps steps (after
  * one warm-up step) for each block width in {128, 256, 512} and tile height in {8, 14, 20, 26,
- * 32, 44, 62, 98} (two-step schedule: widths {64, 128, 256}, heights {6, 12, ..., 60}), keeps the
- * fastest and resets the handle.  The result is cached for the process per grid class (device,
+ * 32, 44, 62, 98} (two-step schedule: widths {64, 128, 256}, heights {6, 12, ..., 60}), times the
+ * three fastest again, twice each over 3 nsteps steps, keeps the best and resets the handle.  The result is cached for the process per grid class (device,
  * precision, schedule, grid size, palette): a later call for the same class applies the cached
  * shape without timing.  nsteps = 0 means 10; nsteps < 0 times -nsteps steps and ignores the
  * cache.  Needs step count 0 (POE2D_E_STATE otherwise).  The shape changes speed only, never a
