@@ -328,6 +328,14 @@ poe2d_status poe2d_patchy_mask(int nx, int nz, double dx, double dz, double gas_
                                uint64_t seed, double rmin, double rmax, uint8_t *mask,
                                double *realised);
 
+/* poe2d_patchy_mask of nmodels models (model k: gas_fraction gas[k], rmin = rmax = radius[k];
+ * all zero when gas[k] is 0) read at flat indices idx[0 .. nidx) of the mask: out[k * nidx + j].
+ * threads host threads share the models (0: one per hardware thread).  No counterpart in the
+ * reference; it forms the surrogate's layout inputs. */
+poe2d_status poe2d_patchy_samples(int nx, int nz, double dx, double dz, uint64_t seed, int nmodels,
+                                  const double *gas, const double *radius, int nidx, const int64_t *idx,
+                                  uint8_t *out, int threads);
+
 /* One of the six disk lattices of patchymodel.m:13-42 and the per-type blocks after it
  * (radius r points, rows x cols layer, 1 inside a disk).  type = 1..6. */
 poe2d_status poe2d_patchy_lattice(int type, int rows, int cols, int r, uint8_t *layer);

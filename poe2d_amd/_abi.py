@@ -174,6 +174,8 @@ def _declare(lib):
         "poe2d_patchy_mask": ([i, i, d, d, d, ctypes.c_uint64, d, d,
                                ctypes.POINTER(ctypes.c_uint8), c_double_p], i),
         "poe2d_patchy_lattice": ([i, i, i, i, ctypes.POINTER(ctypes.c_uint8)], i),
+        "poe2d_patchy_samples": ([i, i, d, d, ctypes.c_uint64, i, c_double_p, c_double_p, i,
+                                  ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_uint8), i], i),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(lib, name)

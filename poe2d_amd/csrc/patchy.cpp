@@ -6,9 +6,12 @@
 // until the painted fraction of grid points reaches a target.  The lattice layouts of
 // patchymodel.m are kept as a separate mode (poe2d_patchy_lattice).
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <thread>
+#include <vector>
 
 #include "common.h"
 
@@ -51,6 +54,30 @@ size_t paint_disk(uint8_t *mask, int nx, int nz, double dx, double dz, double cx
     return added;
 }
 
+// The mask of poe2d_patchy_mask (arguments already checked); false when the target cannot be
+// reached within 10^8 disks.
+bool paint_mask(int nx, int nz, double dx, double dz, double gas_fraction, uint64_t seed, double rmin, double rmax,
+                uint8_t *mask, size_t *painted_out)
+{
+    const size_t npts = (size_t)(nx + 1) * (nz + 1);
+    std::memset(mask, 0, npts);
+    const size_t target = (size_t)std::ceil(gas_fraction * (double)npts);
+    SplitMix64 rng{seed};
+    size_t painted = 0;
+    const double lr0 = std::log(rmin), lr1 = std::log(rmax);
+    const double W = nx * dx, H = nz * dz;
+    size_t draws = 0;
+    while (painted < target) {
+        const double cx = rng.uniform() * W;
+        const double cz = rng.uniform() * H;
+        const double r = std::exp(lr0 + (lr1 - lr0) * rng.uniform());
+        painted += paint_disk(mask, nx, nz, dx, dz, cx, cz, r);
+        if (++draws > 100000000ull) return false;
+    }
+    *painted_out = painted;
+    return true;
+}
+
 }  // namespace
 
 using poe2d::fail;
@@ -69,22 +96,54 @@ poe2d_status poe2d_patchy_mask(int nx, int nz, double dx, double dz, double gas_
     if (!(gas_fraction >= 0 && gas_fraction < 1))
         return fail(POE2D_E_INVALID, "patchy_mask: gas_fraction must be in [0, 1)");
     if (!(rmin > 0) || !(rmax >= rmin)) return fail(POE2D_E_INVALID, "patchy_mask: need 0 < rmin <= rmax");
-    const size_t npts = (size_t)(nx + 1) * (nz + 1);
-    std::memset(mask, 0, npts);
-    const size_t target = (size_t)std::ceil(gas_fraction * (double)npts);
-    SplitMix64 rng{seed};
     size_t painted = 0;
-    const double lr0 = std::log(rmin), lr1 = std::log(rmax);
-    const double W = nx * dx, H = nz * dz;
-    size_t draws = 0;
-    while (painted < target) {
-        const double cx = rng.uniform() * W;
-        const double cz = rng.uniform() * H;
-        const double r = std::exp(lr0 + (lr1 - lr0) * rng.uniform());
-        painted += paint_disk(mask, nx, nz, dx, dz, cx, cz, r);
-        if (++draws > 100000000ull) return fail(POE2D_E_INVALID, "patchy_mask: target fraction not reachable");
-    }
-    if (realised) *realised = (double)painted / (double)npts;
+    if (!paint_mask(nx, nz, dx, dz, gas_fraction, seed, rmin, rmax, mask, &painted))
+        return fail(POE2D_E_INVALID, "patchy_mask: target fraction not reachable");
+    if (realised) *realised = (double)painted / ((double)(nx + 1) * (nz + 1));
+    return POE2D_OK;
+}
+
+// Many models' masks read at a few points (the surrogate's layout inputs): model k's mask is
+// poe2d_patchy_mask(nx, nz, dx, dz, gas[k], seed, radius[k], radius[k]) (all zero when gas[k] is
+// 0), and out[k * nidx + j] its value at flat index idx[j].  The models are dealt round-robin to
+// `threads` host threads (0: one per hardware thread), each painting into a mask of its own.
+poe2d_status poe2d_patchy_samples(int nx, int nz, double dx, double dz, uint64_t seed, int nmodels,
+                                  const double *gas, const double *radius, int nidx, const int64_t *idx,
+                                  uint8_t *out, int threads)
+{
+    if (nx < 1 || nz < 1 || !(dx > 0) || !(dz > 0) || nmodels < 0 || nidx < 0 ||
+        (nmodels > 0 && (!gas || !radius || !out)) || (nidx > 0 && !idx))
+        return fail(POE2D_E_INVALID, "patchy_samples: bad arguments");
+    const int64_t npts = (int64_t)(nx + 1) * (nz + 1);
+    for (int j = 0; j < nidx; ++j)
+        if (idx[j] < 0 || idx[j] >= npts) return fail(POE2D_E_INVALID, "patchy_samples: index outside the grid");
+    for (int k = 0; k < nmodels; ++k)
+        if (!(gas[k] >= 0 && gas[k] < 1) || !(radius[k] > 0))
+            return fail(POE2D_E_INVALID, "patchy_samples: need gas in [0, 1) and radius > 0");
+    int nt = threads > 0 ? threads : (int)std::thread::hardware_concurrency();
+    nt = std::max(1, std::min(nt, std::max(nmodels, 1)));
+    std::atomic<bool> failed{false};
+    auto work = [&](int t0) {
+        std::vector<uint8_t> mask((size_t)npts);
+        for (int k = t0; k < nmodels; k += nt) {
+            uint8_t *row = out + (size_t)k * nidx;
+            if (gas[k] <= 0) {
+                std::memset(row, 0, (size_t)nidx);
+                continue;
+            }
+            size_t painted = 0;
+            if (!paint_mask(nx, nz, dx, dz, gas[k], seed, radius[k], radius[k], mask.data(), &painted)) {
+                failed = true;
+                return;
+            }
+            for (int j = 0; j < nidx; ++j) row[j] = mask[(size_t)idx[j]];
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; ++t) pool.emplace_back(work, t);
+    work(0);
+    for (std::thread &th : pool) th.join();
+    if (failed) return fail(POE2D_E_INVALID, "patchy_samples: target fraction not reachable");
     return POE2D_OK;
 }
 

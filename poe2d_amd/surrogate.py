@@ -70,7 +70,7 @@ def patch_count(params, geom=CONFIG3):
 PATH_SAMPLES = 64
 
 
-def layout_inputs(params, geom=CONFIG3, workers=8):
+def layout_inputs(params, geom=CONFIG3, workers=0):
     """[n, 2 nrec + 1] float32 from the model's gas mask (the fixed layout, painted for its gas
     fraction and radius; the mask the solver's model is built from): 1 where gas covers receiver
     k, else 0 (columns [0, nrec)); the share of gas along the straight segment from the source to
@@ -88,23 +88,14 @@ def layout_inputs(params, geom=CONFIG3, workers=8):
     p = np.asarray(params, np.float64).reshape(-1, 4)
     samples = np.zeros((len(p), len(flat)), np.uint8)  # the mask at the flat indices, per model
     lib = _abi.load_library()
-
-    def run(lo, hi):
-        buf = np.zeros((n + 1) * (n + 1), np.uint8)
-        ptr = buf.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
-        realised = ctypes.c_double()
-        for k in range(lo, hi):
-            gas, radius = float(p[k, 0]), float(p[k, 1])
-            if gas <= 0:
-                continue
-            _abi.check(lib, lib.poe2d_patchy_mask(n, n, dx, dx, gas, ctypes.c_uint64(LAYOUT_SEED), radius, radius,
-                                                  ptr, ctypes.byref(realised)), "patchy_mask")
-            np.take(buf, flat, out=samples[k])
-
-    cuts = np.linspace(0, len(p), workers + 1).round().astype(int)
-    with ThreadPoolExecutor(workers) as pool:
-        for f in [pool.submit(run, cuts[w], cuts[w + 1]) for w in range(workers)]:
-            f.result()
+    gas = np.ascontiguousarray(p[:, 0])
+    radius = np.ascontiguousarray(p[:, 1])
+    flat = np.ascontiguousarray(flat, np.int64)
+    # every model's mask painted and sampled in the library, on `workers` host threads
+    _abi.check(lib, lib.poe2d_patchy_samples(
+        n, n, dx, dx, ctypes.c_uint64(LAYOUT_SEED), len(p), gas.ctypes.data_as(_abi.c_double_p),
+        radius.ctypes.data_as(_abi.c_double_p), len(flat), flat.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+        samples.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), workers), "patchy_samples")
     out = np.empty((len(p), 2 * nrec + 1), np.float32)
     out[:, :nrec] = samples[:, :nrec]
     out[:, nrec:2 * nrec] = samples[:, nrec:nrec * (PATH_SAMPLES + 1)].reshape(len(p), nrec, PATH_SAMPLES).mean(

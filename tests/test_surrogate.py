@@ -198,3 +198,36 @@ def test_model_coefficients_equal_the_whole_grid_evaluation():
         assert set(built) == set(whole)
         for k in whole:
             assert built[k].shape == whole[k].shape and np.array_equal(built[k], whole[k]), k
+
+
+def test_patchy_samples_equal_one_mask_at_a_time():
+    """The library's many-model sampler (threads sharing the models) reads each model's mask at the
+    given points exactly as painting that model's mask alone and indexing it does."""
+    import ctypes
+
+    from poe2d_amd import _abi
+    from poe2d_amd.patchy import patchy_mask
+
+    rng = np.random.default_rng(9)
+    n, dx = 120, 0.5
+    gas = np.concatenate([[0.0], rng.uniform(0.0, 0.3, 12)])
+    radius = np.exp(rng.uniform(np.log(2.0), np.log(20.0), len(gas)))
+    idx = np.ascontiguousarray(rng.integers(0, (n + 1) ** 2, 300), np.int64)
+    out = np.full((len(gas), len(idx)), 7, np.uint8)
+    lib = _abi.load_library()
+    for threads in (1, 3, 0):
+        _abi.check(lib, lib.poe2d_patchy_samples(
+            n, n, dx, dx, ctypes.c_uint64(S.LAYOUT_SEED), len(gas), gas.ctypes.data_as(_abi.c_double_p),
+            radius.ctypes.data_as(_abi.c_double_p), len(idx), idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+            out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), threads), "patchy_samples")
+        for k in range(len(gas)):
+            want = np.zeros(len(idx), np.uint8) if gas[k] <= 0 else \
+                patchy_mask(n, n, dx, dx, gas[k], S.LAYOUT_SEED, radius[k], radius[k]).ravel()[idx]
+            assert np.array_equal(out[k], want), (threads, k)
+    bad = idx.copy()
+    bad[5] = (n + 1) ** 2
+    with pytest.raises(_abi.Poe2dError):
+        _abi.check(lib, lib.poe2d_patchy_samples(
+            n, n, dx, dx, ctypes.c_uint64(1), 1, gas.ctypes.data_as(_abi.c_double_p),
+            radius.ctypes.data_as(_abi.c_double_p), len(bad), bad.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+            out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), 1), "patchy_samples")
