@@ -405,8 +405,9 @@ def surrogate_bench(args):
     the layout inputs) and 3 x that per training sample.  The pipeline part draws models, solves them
     with the batched solver (solver alone: models/s over every candidate), trains the 5-member
     ensemble, calibrates the spread threshold on held-out models, screens candidates (prediction +
-    spread-triggered re-solves, the models/s of the surrogate) and re-solves a random 5 % of the
-    accepted ones to judge it (timed apart)."""
+    spread-triggered re-solves, the models/s of the surrogate; the gathers stay in HBM) and
+    re-solves a random 5 % of the accepted ones to judge it (timed apart); a last screen at the
+    first bound delivers the gathers to page-locked host memory instead."""
     from poe2d_amd import surrogate as S
 
     geom = S.CONFIG3
@@ -441,10 +442,12 @@ def surrogate_bench(args):
     del gathers
     dump = {"val_params": params[ntr:], "val_err": verr, "val_spread": vspread}
     cand = S.draw_params(ncand, seed=4)
-    # the screens' output: one page-locked buffer for the 16 384 gathers, allocated before the
-    # screens and reused by each (a serving loop's output buffer)
-    from poe2d_amd.mlp import pinned_empty
-    out = pinned_empty((ncand,) + sur.shape)
+    # the screens' output: the 16 384 gathers stay in HBM (device memory allocated before the
+    # screens and reused by each, a serving loop's output buffer), as the solver's inputs are
+    # resident in HBM before its timed runs; one more screen delivers them to page-locked host
+    # memory, for the rate that includes the copy over the host link
+    from poe2d_amd.mlp import DeviceArray, pinned_empty
+    out = DeviceArray((ncand,) + sur.shape)
     screens = []
     specs = [(q, None) for q in quantiles]
     if args.sur_confidence > 0:
@@ -474,6 +477,15 @@ def surrogate_bench(args):
             "judged_rel_l2_p99": round(float(np.quantile(je, 0.99)), 4) if len(je) else None,
             "after_fallback_rel_l2_p50": round(S.mixture_quantile(je, ffb, 0.5), 4),
             "after_fallback_rel_l2_p99": round(S.mixture_quantile(je, ffb, 0.99), 4)})
+    del out
+    host_out = pinned_empty((ncand,) + sur.shape)
+    thr = S.spread_threshold(verr, vspread, 0.05, quantiles[0])
+    res = sur.screen(cand, thr, out=host_out, judge_frac=0.0)
+    to_host = {"bound": screens[0]["bound"], "models_per_s": round(res.models_per_second, 2),
+               "screen_s": round(res.screen_seconds, 3), "predict_s": round(res.predict_seconds, 3),
+               "gathers": f"{ncand * sur.nout * 4 / 1e9:.2f} GB copied to page-locked host memory"}
+    print(f"[surrogate] gathers to the host: {res.models_per_second:.1f} models/s", file=sys.stderr, flush=True)
+    del host_out
     sur.close()
     if args.sur_dump:
         # the held-out errors and spreads and the screening outcome, for study off the GPU
@@ -489,10 +501,12 @@ def surrogate_bench(args):
                                f"ensemble trained {args.sur_steps} steps (batch 256) on {ntr} solver gathers "
                                f"(401^2, 3000 steps / 6, 101 receivers x 3)"
                                + (", inputs: 4 parameters + gas at each receiver, on its path from the source and at the source" if sur.receivers else ""),
-                   "step": "one screening pass: prediction + re-solves of the high-spread candidates",
+                   "step": "one screening pass: prediction + re-solves of the high-spread candidates, "
+                           "every gather left in HBM",
                    "solver_alone_models_per_s": round(solve.models_per_second, 2),
                    "solver_alone_gpu_models_per_s": round(solve.gpu_models_per_second, 2),
                    "screens": screens,
+                   "screen_gathers_to_host": to_host,
                    "validation_rel_l2_p50": round(float(np.median(verr)), 4),
                    "validation_rel_l2_p99": round(float(np.quantile(verr, 0.99)), 4),
                    "train_s": round(train_s, 1)},

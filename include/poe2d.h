@@ -376,7 +376,8 @@ poe2d_status poe2d_mlp_train(poe2d_mlp *m, int member, int nsteps, int batch, co
 poe2d_status poe2d_mlp_train_many(poe2d_mlp *m, int count, const int *members, int nsteps, int batch,
                                   const int *indices, float lr, float *losses);
 /* Ensemble prediction of n rows: mean[n][sizes[nlayers]] (may be null) and spread[n] (may be
- * null) = ||standard deviation over members|| / ||mean|| per row. */
+ * null) = ||standard deviation over members|| / ||mean|| per row.  Either may be host memory or
+ * device memory (poe2d_device_alloc): a consumer on the GPU keeps the predictions in HBM. */
 poe2d_status poe2d_mlp_predict(poe2d_mlp *m, int n, const float *x, float *mean, float *spread);
 /* Per-output factors scale[sizes[nlayers]] applied on the device to every predicted mean (the
  * outputs' units when the network was trained on normalised targets); the spread stays that of
@@ -402,6 +403,14 @@ void poe2d_mlp_destroy(poe2d_mlp *m);
  * pages are resident, so no first-touch faults slow a copy down.  Freed by poe2d_host_free. */
 poe2d_status poe2d_host_alloc(size_t bytes, void **out);
 void poe2d_host_free(void *p);
+/* Device memory on the calling thread's current device (no counterpart in the reference), for
+ * results that stay in HBM for a consumer on the GPU, such as a screen's gathers; freed by
+ * poe2d_device_free. */
+poe2d_status poe2d_device_alloc(size_t bytes, void **out);
+void poe2d_device_free(void *p);
+/* Copy bytes from src to dst, each host or device memory (the direction comes from the
+ * pointers); returns when the copy is done. */
+poe2d_status poe2d_copy(void *dst, const void *src, size_t bytes);
 
 /* ---- Reference-format input and output (host only; no device needed) ---- */
 

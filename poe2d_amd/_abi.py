@@ -149,6 +149,9 @@ def _declare(lib):
         "poe2d_mlp_destroy": ([vp], None),
         "poe2d_host_alloc": ([ctypes.c_size_t, ctypes.POINTER(vp)], i),
         "poe2d_host_free": ([vp], None),
+        "poe2d_device_alloc": ([ctypes.c_size_t, ctypes.POINTER(vp)], i),
+        "poe2d_device_free": ([vp], None),
+        "poe2d_copy": ([vp, vp, ctypes.c_size_t], i),
         "poe2d_read_input1": ([ctypes.c_char_p, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p], i),
         "poe2d_read_input2": ([ctypes.c_char_p, c_double_p], i),
         "poe2d_read_grid_f32": ([ctypes.c_char_p, i, i, i, c_float_p], i),

@@ -824,7 +824,7 @@ poe2d_status poe2d_mlp_predict(poe2d_mlp *m, int n, const float *x, float *mean,
         if (spread) {
             row_spread<<<nb, 256, 0, m->stream>>>(m->acc_mean, m->acc_m2, out, E, m->partial);
             HIP_TRY(hipGetLastError());
-            HIP_TRY(hipMemcpyAsync(spread + r0, m->partial, nb * sizeof(float), hipMemcpyDeviceToHost, m->stream));
+            HIP_TRY(hipMemcpyAsync(spread + r0, m->partial, nb * sizeof(float), hipMemcpyDefault, m->stream));
         }
         if (mean) {
             // (the spread above is of the unscaled outputs)
@@ -832,8 +832,9 @@ poe2d_status poe2d_mlp_predict(poe2d_mlp *m, int n, const float *x, float *mean,
                 scale_cols<<<blocks((size_t)nb * out), 256, 0, m->stream>>>(m->acc_mean, nb, out, m->oscale);
                 HIP_TRY(hipGetLastError());
             }
+            // (host or device memory: the direction comes from the pointers)
             HIP_TRY(hipMemcpyAsync(mean + (size_t)r0 * out, m->acc_mean, (size_t)nb * out * sizeof(float),
-                                   hipMemcpyDeviceToHost, m->stream));
+                                   hipMemcpyDefault, m->stream));
         }
         HIP_TRY(hipStreamSynchronize(m->stream));
     }
@@ -979,6 +980,30 @@ poe2d_status poe2d_host_alloc(size_t bytes, void **out)
 void poe2d_host_free(void *p)
 {
     if (p) (void)hipHostFree(p);
+}
+
+poe2d_status poe2d_device_alloc(size_t bytes, void **out)
+{
+    if (!out || bytes == 0) return fail(POE2D_E_INVALID, "device_alloc: bad arguments");
+    *out = nullptr;
+    if (hipMalloc(out, bytes) != hipSuccess) {
+        *out = nullptr;
+        return fail(POE2D_E_NOMEM, "device_alloc: hipMalloc failed");
+    }
+    return POE2D_OK;
+}
+
+void poe2d_device_free(void *p)
+{
+    if (p) (void)hipFree(p);
+}
+
+poe2d_status poe2d_copy(void *dst, const void *src, size_t bytes)
+{
+    if (bytes == 0) return POE2D_OK;
+    if (!dst || !src) return fail(POE2D_E_INVALID, "copy: null pointer");
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDefault));
+    return POE2D_OK;
 }
 
 }  // extern "C"

@@ -116,19 +116,24 @@ class Network:
         return losses
 
     def predict(self, x, want_mean=True, out=None):
-        """(ensemble mean or None, spread per row).  out: a C-contiguous float32 [n, outputs] array to
-        receive the mean (page-locked memory from pinned_empty copies fastest)."""
+        """(ensemble mean or None, spread per row).  out: where the mean goes, a C-contiguous float32
+        [n, outputs] array (page-locked memory from pinned_empty copies fastest) or a DeviceArray of
+        n rows of `outputs` values (the mean stays in HBM)."""
         x = np.ascontiguousarray(x, np.float32)
         n = x.shape[0]
-        if out is not None:
+        if isinstance(out, DeviceArray):
+            if out.shape[0] != n or out.row_size != self.sizes[-1]:
+                raise ValueError("out must hold n rows of the network's outputs")
+            mean, mean_p = out, ctypes.cast(ctypes.c_void_p(out.ptr), _abi.c_float_p)
+        elif out is not None:
             if out.dtype != np.float32 or out.shape != (n, self.sizes[-1]) or not out.flags.c_contiguous:
                 raise ValueError("out must be a C-contiguous float32 array of shape (n, outputs)")
-            mean = out
+            mean, mean_p = out, _f(out)
         else:
             mean = np.zeros((n, self.sizes[-1]), np.float32) if want_mean else None
+            mean_p = _f(mean) if want_mean else None
         spread = np.zeros(n, np.float32)
-        _abi.check(self._lib, self._lib.poe2d_mlp_predict(self._h, n, _f(x), _f(mean) if want_mean else None,
-                                                          _f(spread)), "mlp_predict")
+        _abi.check(self._lib, self._lib.poe2d_mlp_predict(self._h, n, _f(x), mean_p, _f(spread)), "mlp_predict")
         return mean, spread
 
     def time(self, batch, iters=5):
@@ -166,6 +171,64 @@ def pinned_empty(shape, dtype=np.float32):
     buf = (ctypes.c_char * max(nbytes, 1)).from_address(ptr.value)
     weakref.finalize(buf, lib.poe2d_host_free, ctypes.c_void_p(ptr.value))
     return np.frombuffer(buf, dtype=dtype, count=int(np.prod(shape))).reshape(shape)
+
+
+class DeviceArray:
+    """A C-contiguous float32 array in device memory (poe2d_device_alloc), for results that stay in
+    HBM for a consumer on the GPU.  rows(lo, hi) is a view of rows [lo, hi) along the first axis;
+    to_host and set_rows move whole rows between it and the host.  The memory is released when the
+    array and every view of it are gone."""
+
+    def __init__(self, shape, _base=None, _ptr=None):
+        self._lib = _abi.load_library()
+        self.shape = tuple(int(d) for d in shape)
+        self.dtype = np.dtype(np.float32)
+        self.row_size = int(np.prod(self.shape[1:], dtype=np.int64))
+        self.nbytes = self.shape[0] * self.row_size * 4
+        if _base is not None:
+            self._base, self.ptr = _base, _ptr
+            return
+        ptr = ctypes.c_void_p()
+        _abi.check(self._lib, self._lib.poe2d_device_alloc(ctypes.c_size_t(max(self.nbytes, 1)), ctypes.byref(ptr)),
+                   "device_alloc")
+        self._base, self.ptr = None, ptr.value
+        weakref.finalize(self, self._lib.poe2d_device_free, ctypes.c_void_p(self.ptr))
+
+    def __len__(self):
+        return self.shape[0]
+
+    def rows(self, lo, hi):
+        if not 0 <= lo <= hi <= self.shape[0]:
+            raise IndexError("rows out of range")
+        return DeviceArray((hi - lo,) + self.shape[1:], _base=self if self._base is None else self._base,
+                           _ptr=self.ptr + lo * self.row_size * 4)
+
+    def _row_ptr(self, r):
+        if not 0 <= r < self.shape[0]:
+            raise IndexError("row out of range")
+        return ctypes.c_void_p(self.ptr + int(r) * self.row_size * 4)
+
+    def to_host(self, rows=None):
+        """The array (rows None) or the listed rows, as a numpy array."""
+        if rows is None:
+            out = np.empty(self.shape, np.float32)
+            _abi.check(self._lib, self._lib.poe2d_copy(out.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(self.ptr),
+                                                       ctypes.c_size_t(self.nbytes)), "copy")
+            return out
+        rows = np.asarray(rows, np.int64).reshape(-1)
+        out = np.empty((len(rows),) + self.shape[1:], np.float32)
+        for i, r in enumerate(rows):
+            _abi.check(self._lib, self._lib.poe2d_copy(out[i].ctypes.data_as(ctypes.c_void_p), self._row_ptr(r),
+                                                       ctypes.c_size_t(self.row_size * 4)), "copy")
+        return out
+
+    def set_rows(self, rows, values):
+        """Rows `rows` take values[i] (host arrays of a row's shape)."""
+        rows = np.asarray(rows, np.int64).reshape(-1)
+        values = np.ascontiguousarray(values, np.float32).reshape(len(rows), self.row_size)
+        for i, r in enumerate(rows):
+            _abi.check(self._lib, self._lib.poe2d_copy(self._row_ptr(r), values[i].ctypes.data_as(ctypes.c_void_p),
+                                                       ctypes.c_size_t(self.row_size * 4)), "copy")
 
 
 def gemm_bf16(a, b, ksplit=1):

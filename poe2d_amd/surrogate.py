@@ -37,7 +37,7 @@ import numpy as np
 from . import _abi
 from .configs import BRINE, GAS, biot_fields
 from .material import material_biot
-from .mlp import Network
+from .mlp import DeviceArray, Network
 from .patchy import patchy_mask
 from .solver import Batch, Problem, ReceiverSpec, Solver, SourceSpec
 
@@ -388,7 +388,8 @@ class ScreenResult:
     fallback_timing: SolveTiming
     judge_timing: SolveTiming
     gathers: np.ndarray = None  # [n, 3, nrec, nsamp]: the surrogate's prediction where accepted,
-                                # the solver's gather where re-solved for its spread
+                                # the solver's gather where re-solved for its spread (a numpy
+                                # array, or the DeviceArray the screen was given)
 
     @property
     def models_per_second(self):
@@ -447,15 +448,17 @@ class Surrogate:
 
     def predict(self, params, out=None, chunk=2048):
         """(gathers [n, 3, nrec, nsamp], ensemble spread [n] in units of the scaled targets).  out: a
-        float32 array of that shape to fill (page-locked memory, mlp.pinned_empty, copies fastest).
-        The network inputs of the next `chunk` candidates are formed on a host thread while the
-        device predicts the current ones."""
+        float32 array of that shape to fill (page-locked memory, mlp.pinned_empty, copies fastest),
+        or an mlp.DeviceArray of that shape, where the gathers stay in HBM.  The network inputs of
+        the next `chunk` candidates are formed on a host thread while the device predicts the
+        current ones."""
         params = np.asarray(params, np.float64).reshape(-1, 4)
         n = len(params)
         g = np.empty((n,) + self.shape, np.float32) if out is None else out
-        if g.shape != (n,) + self.shape or g.dtype != np.float32 or not g.flags.c_contiguous:
+        on_device = isinstance(g, DeviceArray)
+        if g.shape != (n,) + self.shape or g.dtype != np.float32 or not (on_device or g.flags.c_contiguous):
             raise ValueError(f"out must be a C-contiguous float32 array of shape {(n,) + self.shape}")
-        flat = g.reshape(n, self.nout)
+        flat = g if on_device else g.reshape(n, self.nout)
         spread = np.empty(n, np.float32)
         with ThreadPoolExecutor(1) as pool:
             nxt = pool.submit(features, params[:chunk], self.geom, self.receivers)
@@ -464,7 +467,7 @@ class Surrogate:
                 x = nxt.result()
                 if hi < n:
                     nxt = pool.submit(features, params[hi:hi + chunk], self.geom, self.receivers)
-                _, spread[lo:hi] = self.net.predict(x, out=flat[lo:hi])
+                _, spread[lo:hi] = self.net.predict(x, out=flat.rows(lo, hi) if on_device else flat[lo:hi])
         return g, spread
 
     def errors(self, pred, true):
@@ -483,8 +486,10 @@ class Surrogate:
         """Predict every candidate, re-solve those whose spread exceeds `threshold`, then re-solve
         a random `judge_frac` of the others to measure the surrogate's error (timed apart).  The
         result's gathers hold, per candidate, the prediction or (re-solved) the solver's gather;
-        out: the array to hold them (as for predict), reused by a caller that screens repeatedly.
-        progress: passed to simulate for both sets of re-solves."""
+        out: the array to hold them (as for predict; a DeviceArray keeps them in HBM, the re-solved
+        gathers copied in), reused by a caller that screens repeatedly.  judge_frac 0 judges
+        nothing.  progress: passed to simulate for both sets of re-solves."""
+        on_device = isinstance(out, DeviceArray)
         t0 = time.perf_counter()
         pred, spread = self.predict(params, out=out)
         t_pred = time.perf_counter() - t0
@@ -492,19 +497,23 @@ class Surrogate:
         fb = SolveTiming(0, 0.0, 0.0)
         gathers = pred  # re-solved rows replaced in place; the judged rows are accepted ones
         if len(fallback):
-            gathers[fallback], fb = simulate(params[fallback], None if seeds is None else seeds[fallback],
-                                             self.geom, progress=progress, pools=self.pools)
+            solved, fb = simulate(params[fallback], None if seeds is None else seeds[fallback],
+                                  self.geom, progress=progress, pools=self.pools)
+            if on_device:
+                gathers.set_rows(fallback, solved)
+            else:
+                gathers[fallback] = solved
         t_screen = time.perf_counter() - t0
         accepted = np.setdiff1d(np.arange(len(params)), fallback)
         rng = np.random.default_rng(seed)
-        nj = max(1, int(round(judge_frac * len(accepted)))) if len(accepted) else 0
+        nj = max(1, int(round(judge_frac * len(accepted)))) if judge_frac > 0 and len(accepted) else 0
         judged = np.sort(rng.choice(accepted, nj, replace=False)) if nj else np.zeros(0, int)
         err = np.zeros(0)
         jt = SolveTiming(0, 0.0, 0.0)
         if nj:
             true, jt = simulate(params[judged], None if seeds is None else seeds[judged], self.geom,
                                 progress=progress, pools=self.pools)
-            err = self.errors(pred[judged], true)
+            err = self.errors(pred.to_host(judged) if on_device else pred[judged], true)
         return ScreenResult(len(params), fallback, judged, err, t_screen, t_pred, fb, jt, gathers)
 
     def close(self):

@@ -4,7 +4,7 @@ ensemble spread."""
 import numpy as np
 import pytest
 
-from poe2d_amd.mlp import Network, gemm_bf16, pinned_empty
+from poe2d_amd.mlp import DeviceArray, Network, gemm_bf16, pinned_empty
 
 from .mlp_reference import bf16, forward, loss_and_grads
 
@@ -193,6 +193,29 @@ def test_predict_into_page_locked_memory_with_output_scale():
         net.set_output_scale(None)
         m3, _ = net.predict(x)
         assert np.array_equal(m3, m0)
+
+
+def test_predict_into_device_memory():
+    """predict writes the ensemble mean into device memory, whole or into a view of some rows, the
+    same bits as into host memory; rows move between host and device unchanged."""
+    rng = np.random.default_rng(19)
+    with Network([8, 64, 200], members=3, max_batch=64, seed=6) as net:
+        x = rng.standard_normal((100, 8)).astype(np.float32)
+        m0, s0 = net.predict(x)
+        dev = DeviceArray((130, 200))
+        m1, s1 = net.predict(x, out=dev.rows(10, 110))
+        assert np.array_equal(s1, s0)
+        assert isinstance(m1, DeviceArray) and m1.shape == (100, 200)
+        assert np.array_equal(m1.to_host(), m0)
+        assert np.array_equal(dev.to_host([10, 109, 50]), m0[[0, 99, 40]])
+        vals = rng.standard_normal((2, 200)).astype(np.float32)
+        dev.set_rows([0, 129], vals)
+        assert np.array_equal(dev.to_host([0, 129]), vals)
+        assert np.array_equal(dev.to_host()[10:110], m0)  # rows outside the set ones untouched
+        with pytest.raises(ValueError):
+            net.predict(x, out=dev)  # 130 rows for 100 inputs
+        with pytest.raises(IndexError):
+            dev.rows(0, 131)
 
 
 def test_train_many_equals_training_one_member_at_a_time():

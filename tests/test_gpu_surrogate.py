@@ -5,6 +5,7 @@ import numpy as np
 import pytest
 
 from poe2d_amd import surrogate as S
+from poe2d_amd.mlp import DeviceArray
 from poe2d_amd.solver import Problem, Solver
 
 pytestmark = pytest.mark.gpu
@@ -87,6 +88,17 @@ def test_screening_gains_over_the_solver_alone(trained):
     assert np.median(res.judge_errors) < np.median(err)
     # the surrogate's own rate, fallback aside
     assert res.n / res.predict_seconds > 20 * timing.gpu_models_per_second
+    # the same screen with its gathers kept in device memory: the same gathers, the same judged
+    # errors
+    dev = DeviceArray((res.n,) + S.gather_shape(GEOM))
+    rd = sur.screen(cand, thr, judge_frac=0.25, out=dev)
+    say(f"gathers in device memory: {rd.models_per_second:.0f} models/s (predict {rd.predict_seconds:.3f} s)")
+    assert rd.gathers is dev
+    assert np.array_equal(rd.fallback, res.fallback) and np.array_equal(rd.judged, res.judged)
+    assert np.array_equal(rd.judge_errors, res.judge_errors)
+    assert np.array_equal(dev.to_host(), res.gathers)
+    # judge_frac 0 judges nothing
+    assert len(sur.screen(cand[:64], thr, judge_frac=0.0).judged) == 0
 
 
 def test_receiver_inputs_lower_the_error(trained):
