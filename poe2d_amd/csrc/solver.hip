@@ -593,17 +593,18 @@ poe2d_status build_list(StepList &L, const std::vector<poe2d_solver *> &members,
         ptr.insert(ptr.end(), cnt.begin(), cnt.end());
         for (int r : order) ent.push_back(int2{hx[r], r});
         if (two) {
-            // edge tiles (stored as ~tile): a C-PML strip point within the rows and columns the
-            // kernel computes for the tile, or receivers in its rows
+            // edge tiles (stored as ~tile): a C-PML strip point, or a point of the grid's two outer
+            // rows or columns on either side (whose updates only edge tiles guard), within the rows
+            // and columns the kernel computes for the tile, or receivers in its rows
             const Model *md = s->m;
-            const int nz = s->grid.nz;
+            const int nx = s->grid.nx, nz = s->grid.nz;
             for (int t = 0; t < args[i].ntiles; ++t) {
                 const int tx = t % ntx, tz = t / ntx;
                 const int zA = tz * th, zB = std::min(zA + th, nz + 1);
                 const int c0 = tx * W - kFused2ColsBefore, c1 = c0 + bw - 1;
                 const int r0 = zA - kFused2RowsBefore, r1 = zB + kFused2RowsAfter;
-                const bool xs = c0 < md->xlo || c1 >= md->xhi;
-                const bool zs = r0 < md->ztop || r1 >= md->zbot;
+                const bool xs = c0 < std::max(md->xlo, 2) || c1 >= std::min(md->xhi, nx - 1);
+                const bool zs = r0 < std::max(md->ztop, 2) || r1 >= std::min(md->zbot, nz - 1);
                 const size_t p0 = (size_t)tx * nrow + zA + 1, p1 = (size_t)tx * nrow + zB + 1;
                 const bool rec = s->nrec > 0 && cnt[p1 - 1] > cnt[p0 - 1];
                 work.push_back(WorkItem{i, xs || zs || rec ? ~t : t});

@@ -326,7 +326,8 @@ template <class T>
 hipError_t launch_fused2(const StepArgs<T> *models, const WorkItem *work, int nwork, int per_xcd, int bw, bool palette,
                          bool avg, const long long *base, int k, hipStream_t stream);
 // Rows and columns around a two-step tile whose points the kernel computes (halos of both steps and
-// the over-run of its row loop): a tile with a C-PML strip point in this range is an edge tile.
+// the over-run of its row loop): a tile with a C-PML strip point, or a point of the grid's two
+// outer rows or columns on either side, in this range is an edge tile.
 constexpr int kFused2RowsBefore = 10, kFused2RowsAfter = 12, kFused2ColsBefore = 6;
 
 }  // namespace poe2d
