@@ -19,7 +19,7 @@ pytestmark = pytest.mark.gpu
 FIELDS = ("vx", "vy", "wx", "wy", "exx", "eyy", "zeta", "sxy", "p")
 
 
-def _problem(n=300, nt=301, kind=_abi.SRC_PRESSURE_DGAUSS, src=(150, 140), decimate=1, seed=21, **opts):
+def _problem(n=300, nt=301, kind=_abi.SRC_PRESSURE_DGAUSS, src=(150, 140), decimate=1, seed=21, npml=20, **opts):
     mask = random_patchy_mask(n, n, 0.25, seed=seed)
     coeffs = material_biot(**biot_fields(n, n, mask, BRINE, dict(GAS, mu_fr=6.0e9)))
     # receivers on a line through the source row and on the first tile-column edges (bw 256:
@@ -28,7 +28,7 @@ def _problem(n=300, nt=301, kind=_abi.SRC_PRESSURE_DGAUSS, src=(150, 140), decim
     ix = ix[ix < n - 20]
     rec = ReceiverSpec(ix=ix, iz=np.full(len(ix), 2 * n // 3, np.int32), decimate=decimate)
     return Problem(nx=n, nz=n, dx=0.5, dz=0.5, dt=5e-5, nt=nt, coeffs=coeffs,
-                   source=SourceSpec(kind=kind, ix=src[0], iz=src[1], amp=1e10), receivers=rec, npml=20,
+                   source=SourceSpec(kind=kind, ix=src[0], iz=src[1], amp=1e10), receivers=rec, npml=npml,
                    f0=80.0, **opts)
 
 
@@ -200,7 +200,7 @@ def test_batch_two_step_equals_members_alone():
             c.close()
 
 
-def _graded(n=300, nt=151, kind=_abi.SRC_PRESSURE_DGAUSS, **opts):
+def _graded(n=300, nt=151, kind=_abi.SRC_PRESSURE_DGAUSS, npml=20, **opts):
     # porosity grading with depth and offset: every row and column its own coefficients (no palette)
     fields = biot_fields(n, n, random_patchy_mask(n, n, 0.2, seed=31), BRINE, dict(GAS, mu_fr=6.0e9))
     ramp = np.linspace(-1, 1, n + 1)
@@ -210,7 +210,7 @@ def _graded(n=300, nt=151, kind=_abi.SRC_PRESSURE_DGAUSS, **opts):
     ix = ix[ix < n - 20]
     rec = ReceiverSpec(ix=ix, iz=np.full(len(ix), 2 * n // 3, np.int32))
     return Problem(nx=n, nz=n, dx=0.5, dz=0.5, dt=5e-5, nt=nt, coeffs=coeffs,
-                   source=SourceSpec(kind=kind, ix=150, iz=140, amp=1e10), receivers=rec, npml=20, f0=80.0, **opts)
+                   source=SourceSpec(kind=kind, ix=150, iz=140, amp=1e10), receivers=rec, npml=npml, f0=80.0, **opts)
 
 
 @pytest.mark.parametrize("material", [_abi.MATERIAL_AS_WRITTEN, _abi.MATERIAL_AVERAGED])
@@ -226,6 +226,23 @@ def test_two_step_without_palette_equals_single_step(material):
     for bw, th in ((256, 12), (128, 18), (64, 24)):
         say(f"two-step on the coefficient groups, material {material}: bw {bw} th {th}")
         _same(_run(prob, 3, bw, th), ref, f"groups bw {bw} th {th}")
+
+
+@pytest.mark.parametrize("npml", [1, 2])
+@pytest.mark.parametrize("palette", [True, False])
+def test_thin_cpml_strips(npml, palette):
+    """C-PML strips of one or two points leave tiles next to the grid's outer two rows and columns
+    outside the strips; those must still run as edge tiles, whose updates keep the grid's border
+    points fixed (interior tiles without a palette compile those guards out): bitwise equal to the
+    single-step kernel over tile shapes whose interior tiles come close to every side."""
+    prob = _problem(nt=151, npml=npml) if palette else _graded(npml=npml)
+    with Solver(prob) as s:
+        assert (s.material_palette() > 0) == palette
+    ref = _run(prob, 0, 256, 14)
+    assert np.abs(ref[0]).max() > 0
+    for bw, th in ((64, 12), (128, 18), (256, 24)):
+        say(f"two-step, npml {npml}, palette {palette}: bw {bw} th {th}")
+        _same(_run(prob, 3, bw, th), ref, f"npml {npml} bw {bw} th {th}")
 
 
 def test_two_step_continuous_model():
