@@ -330,8 +330,8 @@ poe2d_status poe2d_patchy_mask(int nx, int nz, double dx, double dz, double gas_
 
 /* poe2d_patchy_mask of nmodels models (model k: gas_fraction gas[k], rmin = rmax = radius[k];
  * all zero when gas[k] is 0) read at flat indices idx[0 .. nidx) of the mask: out[k * nidx + j].
- * threads host threads share the models (0: one per hardware thread).  No counterpart in the
- * reference; it forms the surrogate's layout inputs. */
+ * threads host threads share the models (0: one per hardware thread, at most 16).  No
+ * counterpart in the reference; it forms the surrogate's layout inputs. */
 poe2d_status poe2d_patchy_samples(int nx, int nz, double dx, double dz, uint64_t seed, int nmodels,
                                   const double *gas, const double *radius, int nidx, const int64_t *idx,
                                   uint8_t *out, int threads);

@@ -106,7 +106,8 @@ poe2d_status poe2d_patchy_mask(int nx, int nz, double dx, double dz, double gas_
 // Many models' masks read at a few points (the surrogate's layout inputs): model k's mask is
 // poe2d_patchy_mask(nx, nz, dx, dz, gas[k], seed, radius[k], radius[k]) (all zero when gas[k] is
 // 0), and out[k * nidx + j] its value at flat index idx[j].  The models are dealt round-robin to
-// `threads` host threads (0: one per hardware thread), each painting into a mask of its own.
+// `threads` host threads (0: one per hardware thread, at most 16), each painting into a mask of
+// its own.
 poe2d_status poe2d_patchy_samples(int nx, int nz, double dx, double dz, uint64_t seed, int nmodels,
                                   const double *gas, const double *radius, int nidx, const int64_t *idx,
                                   uint8_t *out, int threads)
@@ -120,7 +121,7 @@ poe2d_status poe2d_patchy_samples(int nx, int nz, double dx, double dz, uint64_t
     for (int k = 0; k < nmodels; ++k)
         if (!(gas[k] >= 0 && gas[k] < 1) || !(radius[k] > 0))
             return fail(POE2D_E_INVALID, "patchy_samples: need gas in [0, 1) and radius > 0");
-    int nt = threads > 0 ? threads : (int)std::thread::hardware_concurrency();
+    int nt = threads > 0 ? threads : std::min(16, (int)std::thread::hardware_concurrency());
     nt = std::max(1, std::min(nt, std::max(nmodels, 1)));
     std::atomic<bool> failed{false};
     auto work = [&](int t0) {
