@@ -31,25 +31,44 @@ struct SplitMix64 {
 };
 
 // Paints the points within distance r of (cx, cz) (metres); returns how many were newly painted.
+// A point (x, z) is inside when (x dx - cx)^2 + (z dz - cz)^2 <= r^2, evaluated in double precision
+// as written.  In a row the inside points form one run of x (the left-hand side falls, then rises,
+// with x also after rounding), close to |x dx - cx| <= h = sqrt(r^2 - (z dz - cz)^2): points more
+// than three columns inside that estimate (its ends truncated to integers, so within a column of
+// the real ends) are inside without a doubt (the rounding moves the run's ends by far less than a
+// column), and only the few columns around each end are tested.
 size_t paint_disk(uint8_t *mask, int nx, int nz, double dx, double dz, double cx, double cz, double r)
 {
     const int x0 = std::max(0, (int)std::ceil((cx - r) / dx));
     const int x1 = std::min(nx, (int)std::floor((cx + r) / dx));
     const int z0 = std::max(0, (int)std::ceil((cz - r) / dz));
     const int z1 = std::min(nz, (int)std::floor((cz + r) / dz));
+    const double r2 = r * r, rdx = 1.0 / dx;
     size_t added = 0;
     for (int z = z0; z <= z1; ++z) {
         const double ddz = z * dz - cz;
-        for (int x = x0; x <= x1; ++x) {
+        const double dz2 = ddz * ddz;
+        uint8_t *row = mask + (size_t)z * (nx + 1);
+        auto test = [&](int x) {
             const double ddx = x * dx - cx;
-            if (ddx * ddx + ddz * ddz <= r * r) {
-                uint8_t &m = mask[(size_t)z * (nx + 1) + x];
-                if (!m) {
-                    m = 1;
-                    ++added;
-                }
-            }
+            const uint8_t in = ddx * ddx + dz2 <= r2;
+            added += in & (row[x] ^ 1);
+            row[x] |= in;
+        };
+        const double h = r2 > dz2 ? std::sqrt(r2 - dz2) : 0.0;
+        const int ea = (int)((cx - h) * rdx), eb = (int)((cx + h) * rdx);
+        const int lo = std::max(x0, ea - 3), hi = std::min(x1, eb + 3);  // every inside point
+        const int ca = std::max(lo, ea + 3), cb = std::min(hi, eb - 3);  // surely inside
+        if (ca > cb) {
+            for (int x = lo; x <= hi; ++x) test(x);
+            continue;
         }
+        for (int x = lo; x < ca; ++x) test(x);
+        size_t set = 0;
+        for (int x = ca; x <= cb; ++x) set += row[x];
+        added += (size_t)(cb - ca + 1) - set;
+        std::memset(row + ca, 1, (size_t)(cb - ca + 1));
+        for (int x = cb + 1; x <= hi; ++x) test(x);
     }
     return added;
 }

@@ -136,3 +136,46 @@ def test_patchymodel_lattice_counts():
     assert med.shape == (2800, 5000)
     assert set(np.unique(med)) == {1.0, 2.0, 3.0}
     assert np.all(med[:2000] == 1) and np.all(med[2240:] == 1)
+
+
+def _patchy_restated(nx, nz, dx, dz, gas, seed, rmin, rmax):
+    """The random patchy mask restated from its definition: SplitMix64 (Steele, Lea and Flood 2014)
+    draws each disk's centre uniform over [0, nx dx] x [0, nz dz] and its radius log-uniform in
+    [rmin, rmax]; every grid point within the radius of the centre is painted; disks are drawn
+    until ceil(gas * points) points are painted."""
+    import math
+    mask64 = (1 << 64) - 1
+    state = [seed & mask64]
+
+    def uniform():
+        state[0] = (state[0] + 0x9E3779B97F4A7C15) & mask64
+        z = state[0]
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & mask64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & mask64
+        return ((z ^ (z >> 31)) >> 11) * (1.0 / 9007199254740992.0)
+
+    mask = np.zeros((nz + 1, nx + 1), np.uint8)
+    target = math.ceil(gas * (nx + 1) * (nz + 1))
+    lr0, lr1 = math.log(rmin), math.log(rmax)
+    xs = np.arange(nx + 1) * dx
+    zs = np.arange(nz + 1) * dz
+    while mask.sum() < target:
+        cx, cz = uniform() * nx * dx, uniform() * nz * dz
+        r = math.exp(lr0 + (lr1 - lr0) * uniform())
+        inside = (xs[None, :] - cx) ** 2 + (zs[:, None] - cz) ** 2 <= r * r
+        mask[inside] = 1
+    return mask
+
+
+@pytest.mark.parametrize("nx,nz,dx,dz,gas,seed,rmin,rmax", [
+    (120, 90, 0.5, 0.5, 0.2, 1, 2.0, 20.0),
+    (97, 131, 1.25, 0.75, 0.12, 7, 1.0, 6.0),
+    (80, 80, 0.5, 0.5, 0.3, 11, 3.3, 3.3),
+    (150, 40, 2.0, 1.0, 0.05, 5, 0.6, 30.0),
+])
+def test_patchy_mask_equals_disk_restatement(nx, nz, dx, dz, gas, seed, rmin, rmax):
+    """The library's painter against the restatement point for point: every disk's points, its
+    radius (exp of a log-uniform draw, also when rmin = rmax) and the stopping count."""
+    got = patchy_mask(nx, nz, dx, dz, gas, seed, rmin, rmax)
+    want = _patchy_restated(nx, nz, dx, dz, gas, seed, rmin, rmax)
+    assert got.shape == want.shape and np.array_equal(got, want)
