@@ -1,9 +1,8 @@
 """Shared test helpers: small problems, Biot closed forms, trace error measure."""
 import numpy as np
 
-from poe2d_amd import _abi
 from poe2d_amd.configs import BRINE, GAS, biot_fields
-from poe2d_amd.solver import Problem, ReceiverSpec, SourceSpec
+from poe2d_amd.solver import Problem
 
 
 def oracle_coeffs(nx, nz, mask=None, background=BRINE, patch=GAS):

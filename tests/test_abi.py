@@ -1,6 +1,5 @@
 """CPU checks of the product library boundary: it loads, exports what include/poe2d.h declares,
 and its host-side set-up routines agree with the oracle's independent restatements."""
-import ctypes
 import os
 import re
 
@@ -10,7 +9,7 @@ import pytest
 from oracle import pyoracle
 from poe2d_amd import _abi
 from poe2d_amd import material as pm
-from poe2d_amd.configs import BRINE, GAS, biot_fields
+from poe2d_amd.configs import biot_fields
 from poe2d_amd.patchy import patchy_lattice, patchy_mask, patchymodel_medium
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))

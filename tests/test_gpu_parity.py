@@ -6,10 +6,9 @@ every recorded sample.  FP64 kernels are held to 1e-9 against the FP64 oracle.
 import numpy as np
 import pytest
 
-from oracle import pyoracle
 from oracle.pyoracle import OracleSolver
 from poe2d_amd import _abi
-from poe2d_amd.configs import BRINE, GAS, biot_fields, config0
+from poe2d_amd.configs import biot_fields, config0
 from poe2d_amd.solver import Problem, ReceiverSpec, Solver, SourceSpec
 
 from .helpers import random_patchy_mask, trace_rel_l2

@@ -6,7 +6,7 @@ import pytest
 
 from poe2d_amd import surrogate as S
 from poe2d_amd.mlp import DeviceArray
-from poe2d_amd.solver import Problem, Solver
+from poe2d_amd.solver import Solver
 
 pytestmark = pytest.mark.gpu
 
