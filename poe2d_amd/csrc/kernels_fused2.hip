@@ -71,10 +71,12 @@ __device__ __forceinline__ void step2_tile(StepArgs<T> A, const int tile, const 
     const int zB = min(zA + A.th, nz + 1);
     const int P = A.pitch;
     // Interior tiles compute only columns [2, nx - 2] and rows [2, nz - 2] (the host makes every
-    // other tile an edge tile), where each update's column and row guards hold.  Without a palette
-    // they are compiled out, so that a row step is a few large blocks of straight-line code (with
-    // the palette the longer live ranges this brings make the kernel spill).
-    constexpr bool GUARD = EDGE || PAL;
+    // other tile an edge tile), where each update's column and row guards hold.  The row guards
+    // are compiled out of every interior tile and the column guards out of those without a
+    // palette, so that a row step is a few large blocks of straight-line code (with the palette,
+    // the longer live ranges of dropping the column guards too make the kernel spill).
+    constexpr bool GUARD = EDGE || PAL;  // column guards
+    constexpr bool ZGUARD = EDGE;        // row guards
     const bool own = lane >= 6 && lane < BW - 6 && (!GUARD || c <= nx);
     const bool ps1 = lane >= 2 && lane <= BW - 3;  // stage-1 stress records worth keeping
     const bool pv1 = lane >= 3 && lane <= BW - 4;  // stage-1 velocity records worth keeping
@@ -180,8 +182,8 @@ __device__ __forceinline__ void step2_tile(StepArgs<T> A, const int tile, const 
                       const Vec4<T> &vp2, const Vec4<T> &so, const T (*xv)[BW + 4], int id_cur, int id_nxt,
                       const Vec4<T> &m_cur, const Vec4<T> &m_nxt, auto pxo, auto pxn_, auto pzo, auto pzn_,
                       bool keep, T srcv, T &sxx, T &p, T &syy) __attribute__((always_inline)) -> Vec4<T> {
-        const bool zr1 = !GUARD || (r >= 2 && r < nz - 1);
-        const bool zr2 = !GUARD || (r >= 1 && r < nz - 1);
+        const bool zr1 = !ZGUARD || (r >= 2 && r < nz - 1);
+        const bool zr2 = !ZGUARD || (r >= 1 && r < nz - 1);
         const bool u1 = xr1 && zr1, u2 = xr2 && zr2;
         const int szs = EDGE ? zstrip(A, r) : -1;
         // C-PML: the update's old records and row coefficients, all read before any record is written
@@ -248,7 +250,7 @@ __device__ __forceinline__ void step2_tile(StepArgs<T> A, const int tile, const 
                         const Vec4<T> &m_j, const Vec4<T> &m_j1, auto pxo, auto pxn_, auto pzo, auto pzn_, bool keep,
                         T srcv) __attribute__((always_inline)) -> Vec4<T> {
         const int szj = EDGE ? zstrip(A, j) : -1;
-        const bool u4 = xr4 && (!GUARD || (j >= 2 && j < nz)), u5 = xr5 && (!GUARD || (j >= 1 && j < nz - 1));
+        const bool u4 = xr4 && (!ZGUARD || (j >= 2 && j < nz)), u5 = xr5 && (!ZGUARD || (j >= 1 && j < nz - 1));
         // C-PML: the update's old records and row coefficients, all read before any record is written
         const long long qx = (long long)j * A.wxs + sx, qz = (long long)szj * P + c;
         T o_dsxx = 0, o_dpx = 0, o_dsxy = 0, o_dsxyz = 0, o_dsyy = 0, o_dpz = 0;
